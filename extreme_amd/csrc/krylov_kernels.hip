@@ -1,0 +1,211 @@
+// Krylov-basis kernels for the FGMRES driver (FP64): block dot V^H w, the classical
+// Gram-Schmidt update with the new norm, norms, scaling and the solution update.
+//
+// Reductions are two-stage and deterministic: kRedBlocks workgroups write per-block
+// partials (grid-stride loops, wavefront shuffles, then LDS), and one workgroup per output
+// sums the partials in a fixed tree order, so repeated solves give identical iterates.
+#include "xtr_common.h"
+#include "xtr_kernels.h"
+
+namespace xtr {
+
+constexpr int kThreads = 256;
+constexpr int kDotChunk = 8;   // basis columns per pass of block_dots
+
+__device__ inline double warp_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// sum over the workgroup of v; valid in thread 0
+__device__ inline double block_sum(double v, double *sh)
+{
+    v = warp_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// partial[(k0 + k) * kRedBlocks + blk] = sum_i conj(V[k][i]) w[i] over this block's i
+__global__ __launch_bounds__(kThreads) void block_dots_kernel(const double2 *__restrict__ V, int cnt,
+                                                              const double2 *__restrict__ w, int64_t n,
+                                                              double2 *partial)
+{
+    __shared__ double sh[kThreads / 64];
+    const int k0 = blockIdx.y * kDotChunk;
+    const int kc = min(kDotChunk, cnt - k0);
+    double2 acc[kDotChunk];
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) acc[k] = make_double2(0, 0);
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 wv = w[i];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < kc) acc[k] = cfma_conj<double>(acc[k], V[(int64_t)(k0 + k) * n + i], wv);
+    }
+#pragma unroll
+    for (int k = 0; k < kDotChunk; ++k) {
+        if (k >= kc) break;
+        const double re = block_sum(acc[k].x, sh);
+        const double im = block_sum(acc[k].y, sh);
+        if (threadIdx.x == 0) partial[(int64_t)(k0 + k) * kRedBlocks + blockIdx.x] = make_double2(re, im);
+    }
+}
+
+void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s)
+{
+    dim3 grid(kRedBlocks, (unsigned)ceil_div(cnt, kDotChunk));
+    hipLaunchKernelGGL(block_dots_kernel, grid, dim3(kThreads), 0, s, V, cnt, w, n, partial);
+    XTR_LAUNCH_CHECK("block_dots_kernel");
+}
+
+// out[k] = sum_b partial[k][b], one workgroup per k, fixed order
+__global__ __launch_bounds__(kThreads) void reduce_c_kernel(const double2 *partial, double2 *out)
+{
+    __shared__ double sh[kThreads / 64];
+    const int k = blockIdx.x;
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += kThreads) {
+        re += partial[(int64_t)k * kRedBlocks + b].x;
+        im += partial[(int64_t)k * kRedBlocks + b].y;
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) out[k] = make_double2(re, im);
+}
+
+void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_c_kernel, dim3(cnt), dim3(kThreads), 0, s, partial, out);
+    XTR_LAUNCH_CHECK("reduce_c_kernel");
+}
+
+// w[i] -= sum_k V[k][i] h[k]; partial[blk] = sum |w_new|^2
+__global__ __launch_bounds__(kThreads) void cgs_update_kernel(const double2 *__restrict__ V, int cnt,
+                                                              const double2 *__restrict__ h, double2 *w,
+                                                              int64_t n, double *partial)
+{
+    __shared__ double sh[kThreads / 64];
+    __shared__ double2 hs[32];
+    if (threadIdx.x < cnt) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    double nn = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        double2 a = make_double2(0, 0);
+        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, V[(int64_t)k * n + i], hs[k]);
+        const double2 nw = csub<double>(w[i], a);
+        w[i] = nw;
+        nn = fma(nw.x, nw.x, nn);
+        nn = fma(nw.y, nw.y, nn);
+    }
+    nn = block_sum(nn, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = nn;
+}
+
+void launch_cgs_update(const double2 *V, int cnt, const double2 *h, double2 *w, int64_t n, double *partial,
+                       hipStream_t s)
+{
+    if (cnt > 32) throw hip_error("cgs_update: more than 32 basis vectors");
+    hipLaunchKernelGGL(cgs_update_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, V, cnt, h, w, n, partial);
+    XTR_LAUNCH_CHECK("cgs_update_kernel");
+}
+
+__global__ __launch_bounds__(kThreads) void norm2_kernel(const double2 *__restrict__ w, int64_t n, double *partial)
+{
+    __shared__ double sh[kThreads / 64];
+    double nn = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 v = w[i];
+        nn = fma(v.x, v.x, nn);
+        nn = fma(v.y, v.y, nn);
+    }
+    nn = block_sum(nn, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = nn;
+}
+
+void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s)
+{
+    hipLaunchKernelGGL(norm2_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, w, n, partial);
+    XTR_LAUNCH_CHECK("norm2_kernel");
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_r_kernel(const double *partial, double *out)
+{
+    __shared__ double sh[kThreads / 64];
+    double v = 0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += kThreads) v += partial[b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = v;
+}
+
+void launch_reduce_r(const double *partial, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_r_kernel, dim3(1), dim3(kThreads), 0, s, partial, out);
+    XTR_LAUNCH_CHECK("reduce_r_kernel");
+}
+
+__global__ void div_kernel(const double2 *src, double d, double2 *dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_double2(v.x / d, v.y / d);
+    }
+}
+
+void launch_div(const double2 *src, double d, double2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(div_kernel, dim3(2048), dim3(256), 0, s, src, d, dst, n);
+    XTR_LAUNCH_CHECK("div_kernel");
+}
+
+__global__ void mul_kernel(const double2 *src, double a, double2 *dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_double2(a * v.x, a * v.y);
+    }
+}
+
+void launch_mul(const double2 *src, double a, double2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(mul_kernel, dim3(2048), dim3(256), 0, s, src, a, dst, n);
+    XTR_LAUNCH_CHECK("mul_kernel");
+}
+
+__global__ void sub_kernel(const double2 *a, const double2 *b, double2 *dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = csub<double>(a[i], b[i]);
+}
+
+void launch_sub(const double2 *a, const double2 *b, double2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(sub_kernel, dim3(2048), dim3(256), 0, s, a, b, dst, n);
+    XTR_LAUNCH_CHECK("sub_kernel");
+}
+
+// x[i] += sum_k B[k][i] y[k]  (ConstructCurrentSolution: r0 = Z y; x = x + r0)
+__global__ __launch_bounds__(kThreads) void update_solution_kernel(const double2 *__restrict__ B, int cnt, Coef32 y,
+                                                                   double2 *x, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        double2 a = make_double2(0, 0);
+        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, B[(int64_t)k * n + i], y.c[k]);
+        x[i] = cadd<double>(x[i], a);
+    }
+}
+
+void launch_update_solution(const double2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(update_solution_kernel, dim3(2048), dim3(kThreads), 0, s, B, cnt, y, x, n);
+    XTR_LAUNCH_CHECK("update_solution_kernel");
+}
+
+}  // namespace xtr

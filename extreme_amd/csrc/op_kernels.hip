@@ -1,0 +1,487 @@
+// Operator-apply kernels for gfx950: LDS-staged pencil FFTs with the R/f scaling fused into
+// the first forward pass and the b scaling plus the last step fused into the last inverse
+// pass, and the per-wavenumber 3x3-block Green-tensor contraction.
+//
+// Spectrum layout (the reference's FFT buffer, Extreme.Cartesian/Fft/FftBuffer.cs:78-96,
+// ConvolutionOperator.cs:250-272): buf[(i*2Ny + j)*3Nz + c*Nz + k], one buffer per
+// right-hand side. 2-D transforms are unnormalised, sign -1 forward, +1 backward
+// (Fft/FftW/FftWTransform.cs:20-27); the 1/(4 Nx Ny) lives in the backward factors b.
+//
+// A pencil kernel owns TC adjacent columns and all P points of each: it stages the
+// TC x P tile in LDS (column index fastest, so global loads of a row of TC columns are
+// contiguous), runs a Stockham radix-4/2/3 transform with every stage reading its
+// butterfly inputs into registers before a barrier and writing after it, and writes
+// the tile back. The zero padding of the forward passes and the cropping of the inverse
+// passes are applied at load/store time, so padded rows are never read or written twice.
+#include "xtr_common.h"
+#include "xtr_kernels.h"
+
+namespace xtr {
+
+constexpr int kFftThreads = 256;
+
+// ---- compile-time radix plan: 4s first, then 2, then 3 -----------------------------------
+constexpr int radix_of(int len)
+{
+    return (len % 4 == 0) ? 4 : (len % 2 == 0) ? 2 : 3;
+}
+
+template <typename T, int R, int DIR>
+__device__ inline void dft_small(cplx<T> *a)
+{
+    if constexpr (R == 2) {
+        cplx<T> t = a[0];
+        a[0] = cadd<T>(t, a[1]);
+        a[1] = csub<T>(t, a[1]);
+    } else if constexpr (R == 4) {
+        cplx<T> t0 = cadd<T>(a[0], a[2]), t1 = csub<T>(a[0], a[2]);
+        cplx<T> t2 = cadd<T>(a[1], a[3]), d = csub<T>(a[1], a[3]);
+        // t3 = d * (DIR i)
+        cplx<T> t3 = DIR < 0 ? mk<T>(d.y, -d.x) : mk<T>(-d.y, d.x);
+        a[0] = cadd<T>(t0, t2);
+        a[1] = cadd<T>(t1, t3);
+        a[2] = csub<T>(t0, t2);
+        a[3] = csub<T>(t1, t3);
+    } else {  // R == 3, w = exp(DIR 2 pi i / 3)
+        const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+        cplx<T> s = cadd<T>(a[1], a[2]), d = csub<T>(a[1], a[2]);
+        cplx<T> m = mk<T>(a[0].x - (T)0.5 * s.x, a[0].y - (T)0.5 * s.y);
+        // (DIR h) i d
+        cplx<T> id = DIR < 0 ? mk<T>(h * d.y, -h * d.x) : mk<T>(-h * d.y, h * d.x);
+        a[0] = cadd<T>(a[0], s);
+        a[1] = cadd<T>(m, id);
+        a[2] = csub<T>(m, id);
+    }
+}
+
+// One Stockham stage over the LDS tile s[p*TC + c] and the recursion to the next stage.
+template <typename T, int P, int TC, int DIR, int LEN, int ST>
+__device__ inline void fft_stage(cplx<T> *s, const cplx<T> *tw, int tid)
+{
+    constexpr int R = radix_of(LEN);
+    constexpr int M = LEN / R;
+    constexpr int NB = (P / R) * TC;
+    constexpr int NPT = (NB + kFftThreads - 1) / kFftThreads;
+    cplx<T> y[NPT][R];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (NB % kFftThreads == 0 || b < NB) {
+            const int cl = b % TC, bb = b / TC;
+            const int p = bb / ST, q = bb % ST;
+            cplx<T> a[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) a[t] = s[(q + ST * (p + t * M)) * TC + cl];
+            dft_small<T, R, DIR>(a);
+#pragma unroll
+            for (int u = 1; u < R; ++u) a[u] = cmul<T>(a[u], tw[p * u * (P / LEN)]);
+#pragma unroll
+            for (int u = 0; u < R; ++u) y[k][u] = a[u];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int b = tid + k * kFftThreads;
+        if (NB % kFftThreads == 0 || b < NB) {
+            const int cl = b % TC, bb = b / TC;
+            const int p = bb / ST, q = bb % ST;
+#pragma unroll
+            for (int u = 0; u < R; ++u) s[(q + ST * (R * p + u)) * TC + cl] = y[k][u];
+        }
+    }
+    __syncthreads();
+    if constexpr (M > 1) fft_stage<T, P, TC, DIR, M, ST * R>(s, tw, tid);
+}
+
+// ---- load/store policies -------------------------------------------------------------------
+
+// Generic strided pencils of one array per right-hand side. Column c lives at
+// (c / cpr) * row_stride + (c % cpr); point p at + p * pstride.
+template <typename T>
+struct IoPlain {
+    cplx<T> *buf;
+    int64_t rhs_stride;
+    int64_t ncols, cpr, row_stride, pstride;
+    int n_in, n_out;
+    __device__ int64_t addr(int rhs, int64_t c, int p) const
+    {
+        return rhs * rhs_stride + (c / cpr) * row_stride + (c % cpr) + (int64_t)p * pstride;
+    }
+    __device__ cplx<T> load(int rhs, int64_t c, int p) const
+    {
+        if (c >= ncols || p >= n_in) return mk<T>(0, 0);
+        return buf[addr(rhs, c, p)];
+    }
+    __device__ void store(int rhs, int64_t c, int p, cplx<T> v) const
+    {
+        if (c < ncols && p < n_out) buf[addr(rhs, c, p)] = v;
+    }
+};
+
+// First forward pass along y, fused with ApplyOperatorR and PrepareForForwardFft
+// (ConvolutionOperator.cs:233-272): reads x and R, writes f * R * x into rows i < Nx of the
+// padded buffer; columns c = i*3Nz + col; points j < Ny are data, j >= Ny are the padding.
+template <typename T>
+struct IoFwdY {
+    RhsPtrs v;
+    const double2 *R;
+    const double *f;
+    cplx<T> *buf;
+    int64_t M;
+    int nx, ny, nz;
+    __device__ cplx<T> load(int rhs, int64_t c, int p) const
+    {
+        const int64_t C = 3LL * nz;
+        if (c >= nx * C || p >= ny) return mk<T>(0, 0);
+        const int64_t i = c / C, col = c % C;
+        const int k = (int)(col % nz);
+        const int64_t ij = i * ny + p;
+        double2 rx = cmul<double>(R[ij * nz + k], v.x[rhs][ij * C + col]);
+        return from_d<T>(cscale<double>(rx, f[k]));
+    }
+    __device__ void store(int rhs, int64_t c, int p, cplx<T> val) const
+    {
+        const int64_t C = 3LL * nz;
+        if (c >= nx * C) return;
+        const int64_t i = c / C, col = c % C;
+        buf[rhs * M + (i * 2 * ny + p) * C + col] = val;
+    }
+};
+
+// Last inverse pass along y, fused with ExtractData and PerformLastStep
+// (ConvolutionOperator.cs:187-231): keeps j < Ny of rows i < Nx, scales by b, and writes
+// y = x - b*conv - R*x (A) or y = b*conv + R*x (Chi0) in FP64.
+template <typename T>
+struct IoInvY {
+    RhsPtrs v;
+    const double2 *R;
+    const double *b;
+    cplx<T> *buf;
+    int64_t M;
+    int nx, ny, nz, type;
+    __device__ cplx<T> load(int rhs, int64_t c, int p) const
+    {
+        const int64_t C = 3LL * nz;
+        if (c >= nx * C) return mk<T>(0, 0);
+        const int64_t i = c / C, col = c % C;
+        return buf[rhs * M + (i * 2 * ny + p) * C + col];
+    }
+    __device__ void store(int rhs, int64_t c, int p, cplx<T> val) const
+    {
+        const int64_t C = 3LL * nz;
+        if (c >= nx * C || p >= ny) return;
+        const int64_t i = c / C, col = c % C;
+        const int k = (int)(col % nz);
+        const int64_t ij = i * ny + p, t = ij * C + col;
+        double2 conv;
+        if constexpr (sizeof(T) == 8) conv = cscale<double>(val, b[k]);
+        else conv = cscale<double>(to_d(val), b[k]);
+        const double2 xv = v.x[rhs][t];
+        const double2 rx = cmul<double>(R[ij * nz + k], xv);
+        double2 out;
+        if (type == 0) out = csub<double>(csub<double>(xv, conv), rx);
+        else out = cadd<double>(conv, rx);
+        v.y[rhs][t] = out;
+    }
+};
+
+template <typename T, int P, int TC, int DIR, class IO>
+__global__ __launch_bounds__(kFftThreads) void pencil_fft_kernel(IO io, const cplx<T> *__restrict__ twg)
+{
+    __shared__ cplx<T> s[P * TC];
+    __shared__ cplx<T> tw[P];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < P; k += kFftThreads) {
+        cplx<T> w = twg[k];
+        tw[k] = DIR < 0 ? w : cconj<T>(w);
+    }
+    const int64_t c0 = (int64_t)blockIdx.x * TC;
+    const int rhs = blockIdx.y;
+    constexpr int NE = (P * TC + kFftThreads - 1) / kFftThreads;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = tid + k * kFftThreads;
+        if ((P * TC) % kFftThreads == 0 || e < P * TC) {
+            const int p = e / TC, cl = e % TC;
+            s[p * TC + cl] = io.load(rhs, c0 + cl, p);
+        }
+    }
+    __syncthreads();
+    fft_stage<T, P, TC, DIR, P, 1>(s, tw, tid);
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = tid + k * kFftThreads;
+        if ((P * TC) % kFftThreads == 0 || e < P * TC) {
+            const int p = e / TC, cl = e % TC;
+            io.store(rhs, c0 + cl, p, s[p * TC + cl]);
+        }
+    }
+}
+
+// Column tile per pencil length: the LDS tile stays at or below 32 KB in FP64.
+template <typename T, int P>
+constexpr int tile_cols()
+{
+    return (P * 16 * (int)sizeof(cplx<T>) <= 32768) ? 16 : 8;
+}
+
+template <typename T, int DIR, class IO>
+void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw, hipStream_t s)
+{
+    auto go = [&](auto pc) {
+        constexpr int PP = decltype(pc)::value;
+        constexpr int TC = tile_cols<T, PP>();
+        dim3 grid((unsigned)ceil_div(ncols, TC), nrhs);
+        hipLaunchKernelGGL((pencil_fft_kernel<T, PP, TC, DIR, IO>), grid, dim3(kFftThreads), 0, s, io,
+                           (const cplx<T> *)tw);
+    };
+    switch (P) {
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    case 24: go(std::integral_constant<int, 24>{}); break;
+    case 32: go(std::integral_constant<int, 32>{}); break;
+    case 48: go(std::integral_constant<int, 48>{}); break;
+    case 64: go(std::integral_constant<int, 64>{}); break;
+    case 96: go(std::integral_constant<int, 96>{}); break;
+    case 128: go(std::integral_constant<int, 128>{}); break;
+    case 192: go(std::integral_constant<int, 192>{}); break;
+    case 256: go(std::integral_constant<int, 256>{}); break;
+    case 384: go(std::integral_constant<int, 384>{}); break;
+    case 512: go(std::integral_constant<int, 512>{}); break;
+    default: throw hip_error("unsupported FFT length " + std::to_string(P));
+    }
+    XTR_LAUNCH_CHECK("pencil_fft_kernel");
+}
+
+bool fft_length_supported(int p)
+{
+    switch (p) {
+    case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512:
+        return true;
+    default:
+        return false;
+    }
+}
+
+// ---- stage launchers -------------------------------------------------------------------------
+
+template <typename T>
+static void fwd_y(const ApplyArgs &a, hipStream_t s)
+{
+    IoFwdY<T> io{a.v, a.R, a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
+    launch_pencil<T, -1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
+}
+
+template <typename T>
+static void fwd_x(const ApplyArgs &a, hipStream_t s)
+{
+    const int64_t ncols = 2LL * a.g.ny * a.g.C();
+    IoPlain<T> io{(cplx<T> *)a.buf, a.g.M(), ncols, ncols, 0, ncols, a.g.nx, 2 * a.g.nx};
+    launch_pencil<T, -1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s);
+}
+
+template <typename T>
+static void inv_x(const ApplyArgs &a, hipStream_t s)
+{
+    const int64_t ncols = 2LL * a.g.ny * a.g.C();
+    IoPlain<T> io{(cplx<T> *)a.buf, a.g.M(), ncols, ncols, 0, ncols, 2 * a.g.nx, a.g.nx};
+    launch_pencil<T, +1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s);
+}
+
+template <typename T>
+static void inv_y(const ApplyArgs &a, hipStream_t s)
+{
+    IoInvY<T> io{a.v, a.R, a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
+    launch_pencil<T, +1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
+}
+
+void launch_fwd_y(const ApplyArgs &a, hipStream_t s) { a.precision ? fwd_y<float>(a, s) : fwd_y<double>(a, s); }
+void launch_fwd_x(const ApplyArgs &a, hipStream_t s) { a.precision ? fwd_x<float>(a, s) : fwd_x<double>(a, s); }
+void launch_inv_x(const ApplyArgs &a, hipStream_t s) { a.precision ? inv_x<float>(a, s) : inv_x<double>(a, s); }
+void launch_inv_y(const ApplyArgs &a, hipStream_t s) { a.precision ? inv_y<float>(a, s) : inv_y<double>(a, s); }
+
+void launch_fft2_plain(double2 *data, int n1, int n2, int64_t ncols, const double2 *tw1, const double2 *tw2,
+                       int sign, hipStream_t s)
+{
+    // along n2: columns (i1, c), pencil stride ncols
+    IoPlain<double> y{data, 0, (int64_t)n1 * ncols, ncols, (int64_t)n2 * ncols, ncols, n2, n2};
+    // along n1: columns (i2, c) flattened, pencil stride n2*ncols
+    IoPlain<double> x{data, 0, (int64_t)n2 * ncols, (int64_t)n2 * ncols, 0, (int64_t)n2 * ncols, n1, n1};
+    if (sign < 0) {
+        launch_pencil<double, -1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s);
+        launch_pencil<double, -1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s);
+    } else {
+        launch_pencil<double, +1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s);
+        launch_pencil<double, +1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s);
+    }
+}
+
+// ---- contraction -----------------------------------------------------------------------------
+// ApplyGreenTensorAlongZ (ConvolutionOperator.cs:274-327): for every wavenumber l
+//   y_x = XX u_x + XY u_y + XZ u_z
+//   y_y = XY u_x + YY u_y + YZ u_z
+//   y_z = -XZ^T u_x - YZ^T u_y + ZZ u_z
+// XX, XY, YY, ZZ complex-symmetric (stored packed), XZ, YZ full row-major [rc][tr].
+// A workgroup owns W wavenumbers and all right-hand sides; thread = (w, rhs, component, row).
+// Each of the six blocks is staged in LDS as a padded full nz x (nz+1) matrix (the packed
+// blocks are expanded on the way in), so both row and column reads are conflict-light.
+// The spectrum is updated in place: every workgroup reads its u into LDS before writing y.
+template <typename T>
+__global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t L,
+                                                         int nz, int nrhs, int W, int64_t M,
+                                                         const uint16_t *__restrict__ packed_rc)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *blk = reinterpret_cast<cplx<T> *>(smem_raw);   // [W][nz][nz+1]
+    const int ld = nz + 1;
+    cplx<T> *u = blk + (int64_t)W * nz * ld;                  // [W][nrhs][3nz]
+    const int C = 3 * nz;
+    const int S = nz * (nz + 1) / 2;
+    const int64_t gblk = 4LL * S + 2LL * nz * nz;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int64_t l0 = (int64_t)blockIdx.x * W;
+    const int per_w = nrhs * C;
+
+    for (int e = tid; e < W * per_w; e += nth) {
+        const int w = e / per_w, rest = e % per_w;
+        const int rhs = rest / C, q = rest % C;
+        const int64_t l = l0 + w;
+        u[e] = (l < L) ? buf[rhs * M + l * C + q] : mk<T>(0, 0);
+    }
+
+    const bool active = tid < W * per_w;
+    const int w = active ? tid / per_w : 0;
+    const int rest = active ? tid % per_w : 0;
+    const int rhs = rest / C, q = rest % C;
+    const int a = q / nz, r = q % nz;
+    const cplx<T> *uw = u + (w * nrhs + rhs) * C;
+    const cplx<T> *bw = blk + (int64_t)w * nz * ld;
+    cplx<T> acc = mk<T>(0, 0);
+
+    // block order in memory: xx, xy, yy, zz (packed S), xz, yz (nz*nz)
+    for (int bi = 0; bi < 6; ++bi) {
+        const bool sym = bi < 4;
+        const int64_t off = sym ? (int64_t)bi * S : 4LL * S + (int64_t)(bi - 4) * nz * nz;
+        const int sz = sym ? S : nz * nz;
+        __syncthreads();
+        for (int e = tid; e < W * sz; e += nth) {
+            const int ww = e / sz, o = e % sz;
+            const int64_t l = l0 + ww;
+            const cplx<T> g = (l < L) ? G[l * gblk + off + o] : mk<T>(0, 0);
+            cplx<T> *bb = blk + (int64_t)ww * nz * ld;
+            if (sym) {
+                const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
+                bb[rc * ld + tr] = g;
+                bb[tr * ld + rc] = g;
+            } else {
+                bb[(o / nz) * ld + (o % nz)] = g;
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        // row product acc += B[r][:] . u_col  (col component cc) ; or transposed
+        int cc = -1;       // component of u for the row product
+        int ct = -1;       // component of u for the transposed product (subtracted)
+        switch (bi) {
+        case 0: cc = (a == 0) ? 0 : -1; break;                           // XX
+        case 1: cc = (a == 0) ? 1 : (a == 1) ? 0 : -1; break;            // XY
+        case 2: cc = (a == 1) ? 1 : -1; break;                           // YY
+        case 3: cc = (a == 2) ? 2 : -1; break;                           // ZZ
+        case 4: cc = (a == 0) ? 2 : -1; ct = (a == 2) ? 0 : -1; break;   // XZ
+        case 5: cc = (a == 1) ? 2 : -1; ct = (a == 2) ? 1 : -1; break;   // YZ
+        }
+        if (cc >= 0) {
+            const cplx<T> *uc = uw + cc * nz;
+            const cplx<T> *row = bw + r * ld;
+            cplx<T> s0 = mk<T>(0, 0);
+            for (int c = 0; c < nz; ++c) s0 = cfma<T>(s0, row[c], uc[c]);
+            acc = cadd<T>(acc, s0);
+        }
+        if (ct >= 0) {
+            const cplx<T> *uc = uw + ct * nz;
+            cplx<T> s0 = mk<T>(0, 0);
+            for (int m = 0; m < nz; ++m) s0 = cfma<T>(s0, bw[m * ld + r], uc[m]);
+            acc = csub<T>(acc, s0);
+        }
+    }
+    if (active) {
+        const int64_t l = l0 + w;
+        if (l < L) buf[rhs * M + l * C + q] = acc;
+    }
+}
+
+template <typename T>
+static void contract(const ApplyArgs &a, hipStream_t s)
+{
+    const int nz = a.g.nz, C = 3 * nz;
+    const int per_w = a.nrhs * C;
+    int W = per_w >= 256 ? 1 : 256 / per_w;
+    if (W < 1) W = 1;
+    int threads = (int)ceil_div((int64_t)W * per_w, 64) * 64;
+    if (threads > 1024) throw hip_error("contraction: 3*nz*nrhs exceeds 1024 threads");
+    const size_t lds = sizeof(cplx<T>) * ((size_t)W * nz * (nz + 1) + (size_t)W * per_w);
+    const int64_t L = a.g.L();
+    dim3 grid((unsigned)ceil_div(L, W));
+    hipLaunchKernelGGL(contract_kernel<T>, grid, dim3(threads), lds, s, (const cplx<T> *)a.G, (cplx<T> *)a.buf, L,
+                       nz, a.nrhs, W, a.g.M(), a.packed_rc);
+    XTR_LAUNCH_CHECK("contract_kernel");
+}
+
+void launch_contract(const ApplyArgs &a, hipStream_t s) { a.precision ? contract<float>(a, s) : contract<double>(a, s); }
+
+// ---- Green tensor layout conversion -----------------------------------------------------------
+__global__ void green_interleave_kernel(const double2 *xx, const double2 *xy, const double2 *xz, const double2 *yy,
+                                        const double2 *yz, const double2 *zz, double2 *dev, int64_t L, int nz,
+                                        int dir)
+{
+    const int S = nz * (nz + 1) / 2, A = nz * nz;
+    const int64_t gblk = 4LL * S + 2LL * A;
+    const int64_t total = L * gblk;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = e / gblk;
+        const int64_t o = e % gblk;
+        const double2 *src;
+        int64_t si;
+        if (o < 4LL * S) {
+            const int b = (int)(o / S);
+            const double2 *comps[4] = {xx, xy, yy, zz};
+            src = comps[b];
+            si = l * S + (o % S);
+        } else {
+            const int b = (int)((o - 4LL * S) / A);
+            src = b == 0 ? xz : yz;
+            si = l * A + ((o - 4LL * S) % A);
+        }
+        if (dir == 0) dev[e] = src[si];
+        else const_cast<double2 *>(src)[si] = dev[e];
+    }
+}
+
+void launch_green_interleave(const double2 *comp[6], double2 *dev, const Geom &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(green_interleave_kernel, dim3(2048), dim3(256), 0, s, comp[0], comp[1], comp[2], comp[3],
+                       comp[4], comp[5], dev, g.L(), g.nz, 0);
+    XTR_LAUNCH_CHECK("green_interleave_kernel");
+}
+
+void launch_green_deinterleave(const double2 *dev, double2 *comp[6], const Geom &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(green_interleave_kernel, dim3(2048), dim3(256), 0, s, comp[0], comp[1], comp[2], comp[3],
+                       comp[4], comp[5], const_cast<double2 *>(dev), g.L(), g.nz, 1);
+    XTR_LAUNCH_CHECK("green_deinterleave_kernel");
+}
+
+__global__ void to_f32_kernel(const double2 *src, float2 *dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = to_f(src[i]);
+}
+
+void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_f32_kernel, dim3(4096), dim3(256), 0, s, src, dst, n);
+    XTR_LAUNCH_CHECK("to_f32_kernel");
+}
+
+}  // namespace xtr

@@ -1,0 +1,933 @@
+// C ABI of libxtreme_hip.so: operator handle, operator apply, FGMRES with the inner-GMRES right
+// preconditioner, and the side-by-side solve of several right-hand sides.
+//
+// The Krylov solver restates Extreme.Fgmres (Gmres.cs, FlexibleGmresSolver.cs,
+// ClassicGmresSolver.cs, FlexibleGmresWithGmresPreconditioner.cs) with the basis vectors in
+// HBM: the host keeps the (m+1)^2 Hessenberg matrix and the Givens rotations in FP64 and
+// drives the kernels; every Gram-Schmidt pass costs one device->host copy of the j+1
+// projections and the new norm, which is exactly what the iterated classical Gram-Schmidt
+// test (Gmres.cs:297-300) needs to decide on a second pass.
+#include "xtr_green.h"
+#include "xtr_kernels.h"
+#include "xtr_common.h"
+#include "../../include/xtreme_hip.h"
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <condition_variable>
+#include <cstring>
+#include <exception>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1.0E-07;   // OmegaModelUtils.cs:12
+const double kEps0 = 1.0 / (kMu0 * 299792458.0 * 299792458.0);      // OmegaModelUtils.cs:13
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return -1;
+}
+
+void require(bool ok, const std::string &msg)
+{
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    void alloc(size_t n)
+    {
+        release();
+        if (n) XTR_CHECK(hipMalloc((void **)&p, n * sizeof(T)));
+        count = n;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        count = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+template <typename T>
+struct PinBuf {
+    T *p = nullptr;
+    explicit PinBuf(size_t n) { XTR_CHECK(hipHostMalloc((void **)&p, n * sizeof(T), hipHostMallocDefault)); }
+    ~PinBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+};
+
+}  // namespace
+
+struct xtr_op {
+    xtr::Geom g{};
+    int precision = XTR_FP64;
+    int max_rhs = 1;
+    int type = -1;
+    double omega = 0;
+    double dx = 0, dy = 0;
+    std::vector<double> depth, thick;
+    xtr::Section section;
+    int number_of_hankels = 10;
+    std::vector<double> hank0, hank1;
+
+    hipStream_t stream = nullptr;
+    DevBuf<double2> G64;          // [L][gblk] FP64 Green tensor
+    DevBuf<float2> G32;           // [L][gblk] complex64 copy for XTR_FP32
+    bool green_ready = false;
+    DevBuf<double2> R;            // [nx*ny*nz]
+    DevBuf<double> f, b;          // [nz]
+    DevBuf<uint16_t> packed_rc;   // [S][2]
+    DevBuf<double2> tw64_x, tw64_y;
+    DevBuf<float2> tw32_x, tw32_y;
+    DevBuf<unsigned char> buf;    // max_rhs * M spectrum
+    DevBuf<double2> stage_x, stage_y;   // host-pointer apply staging
+
+    bool profiling = false;
+    hipEvent_t ev[6] = {};
+    double stage_ms[5] = {0, 0, 0, 0, 0};
+    std::mutex apply_mu;
+
+    ~xtr_op()
+    {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t elem() const { return precision == XTR_FP64 ? sizeof(double2) : sizeof(float2); }
+
+    // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr).
+    void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s)
+    {
+        require(green_ready, "the Green tensor is not set");
+        require(type == XTR_OP_A || type == XTR_OP_CHI0, "the operator is not prepared");
+        require(nrhs >= 1 && nrhs <= max_rhs, "nrhs out of range");
+        std::lock_guard<std::mutex> lk(apply_mu);
+        xtr::ApplyArgs a{};
+        a.g = g;
+        a.precision = precision;
+        a.nrhs = nrhs;
+        a.type = type;
+        a.R = R.p;
+        a.f = f.p;
+        a.b = b.p;
+        a.G = precision == XTR_FP64 ? (const void *)G64.p : (const void *)G32.p;
+        a.packed_rc = packed_rc.p;
+        a.buf = buf.p;
+        a.tw_x = precision == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
+        a.tw_y = precision == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
+        a.v = v;
+        if (profiling) XTR_CHECK(hipEventRecord(ev[0], s));
+        xtr::launch_fwd_y(a, s);
+        if (profiling) XTR_CHECK(hipEventRecord(ev[1], s));
+        xtr::launch_fwd_x(a, s);
+        if (profiling) XTR_CHECK(hipEventRecord(ev[2], s));
+        xtr::launch_contract(a, s);
+        if (profiling) XTR_CHECK(hipEventRecord(ev[3], s));
+        xtr::launch_inv_x(a, s);
+        if (profiling) XTR_CHECK(hipEventRecord(ev[4], s));
+        xtr::launch_inv_y(a, s);
+        if (profiling) XTR_CHECK(hipEventRecord(ev[5], s));
+    }
+};
+
+namespace {
+
+std::vector<double2> twiddles64(int p)
+{
+    std::vector<double2> t(p);
+    for (int k = 0; k < p; ++k) {
+        const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)p;
+        t[k] = make_double2(std::cos(a), -std::sin(a));
+    }
+    return t;
+}
+
+std::vector<float2> twiddles32(int p)
+{
+    auto d = twiddles64(p);
+    std::vector<float2> t(p);
+    for (int k = 0; k < p; ++k) t[k] = make_float2((float)d[k].x, (float)d[k].y);
+    return t;
+}
+
+template <typename T>
+void upload(DevBuf<T> &d, const std::vector<T> &h)
+{
+    d.alloc(h.size());
+    XTR_CHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52)
+int background_layer(const xtr::Section &s, double start, double end)
+{
+    double ss = s.zero_level;
+    for (int i = 0; i < (int)s.thickness.size(); ++i) {
+        const double es = ss + s.thickness[i];
+        if (start >= ss && end <= es) return i;
+        ss = es;
+    }
+    throw std::invalid_argument("an anomaly layer is not inside one background layer");
+}
+
+cd zeta_of(double sigma, double omega) { return cd(sigma, -omega * kEps0); }   // OmegaModelUtils.cs:41-42
+
+// ---------------------------------------------------------------------------------------------
+// FGMRES / GMRES on device vectors
+// ---------------------------------------------------------------------------------------------
+
+using ApplyFn = std::function<void(const double2 *x, double2 *y)>;
+
+struct SolveCtx {
+    hipStream_t stream;
+    ApplyFn apply;
+    int64_t matvecs = 0;
+    int64_t dots = 0;
+    xtr_iteration_fn on_iter = nullptr;
+    void *user = nullptr;
+    int rhs_index = 0;
+    // reduction scratch shared by the outer and inner solvers of one solve
+    DevBuf<double2> partial_c;   // [32][kRedBlocks]
+    DevBuf<double> partial_r;    // [kRedBlocks]
+    DevBuf<double2> h_dev;       // [32] projections, then [32] norm slot
+    DevBuf<double> norm_dev;     // [1]
+    PinBuf<double2> h_host{33};
+    SolveCtx() : partial_c(32 * xtr::kRedBlocks), partial_r(xtr::kRedBlocks), h_dev(32), norm_dev(1) {}
+};
+
+struct Gmres {
+    int64_t n;
+    int m;
+    bool flexible;
+    double tol = 1e-10;
+    int max_repeat = 1;
+    int backward_check = 0;
+    SolveCtx *ctx;
+    Gmres *inner = nullptr;
+    DevBuf<double2> V, Z, w, r0;
+    std::vector<cd> H;      // (m+1) x (m+1), column-major (FortranMatrix.cs:11-52)
+    std::vector<double> gc;
+    std::vector<cd> gs;
+    int iter_counter = 0;
+
+    Gmres(int64_t n_, int m_, bool flex, SolveCtx *c)
+        : n(n_), m(m_), flexible(flex), ctx(c), V((size_t)n_ * m_), w(n_), r0(n_),
+          H((size_t)(m_ + 1) * (m_ + 1)), gc(m_), gs(m_)
+    {
+        if (flex) Z.alloc((size_t)n_ * m_);
+        require(m_ >= 1 && m_ <= 31, "buffer length must be within 1..31");
+    }
+
+    cd &h(int i, int j) { return H[(size_t)j * (m + 1) + i]; }
+    double2 *col(DevBuf<double2> &B, int j) { return B.p + (int64_t)j * n; }
+
+    void matvec(const double2 *x, double2 *y)
+    {
+        ctx->apply(x, y);
+        ctx->matvecs++;
+    }
+
+    double norm(const double2 *v)
+    {
+        xtr::launch_norm2(v, n, ctx->partial_r.p, ctx->stream);
+        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream);
+        double nn = 0;
+        XTR_CHECK(hipMemcpyAsync(&ctx->h_host.p[32], ctx->norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        XTR_CHECK(hipStreamSynchronize(ctx->stream));
+        std::memcpy(&nn, &ctx->h_host.p[32], sizeof(double));
+        ctx->dots++;
+        return std::sqrt(nn);
+    }
+
+    // ClassicalGramSchmidtIteration (Gmres.cs:302-313) fused with the norm of the new w.
+    void cgs(int jH, double &dloo, double &dnormw)
+    {
+        const int cnt = jH + 1;
+        xtr::launch_block_dots(V.p, cnt, w.p, n, ctx->partial_c.p, ctx->stream);
+        xtr::launch_reduce_c(ctx->partial_c.p, cnt, ctx->h_dev.p, ctx->stream);
+        xtr::launch_cgs_update(V.p, cnt, ctx->h_dev.p, w.p, n, ctx->partial_r.p, ctx->stream);
+        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream);
+        XTR_CHECK(hipMemcpyAsync(ctx->h_host.p, ctx->h_dev.p, cnt * sizeof(double2), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        XTR_CHECK(hipMemcpyAsync(&ctx->h_host.p[32], ctx->norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        XTR_CHECK(hipStreamSynchronize(ctx->stream));
+        double s = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const cd d(ctx->h_host.p[k].x, ctx->h_host.p[k].y);
+            h(k, jH) += d;                              // Zaxpy(jH+1, One, dots, H[:,jH])
+            s += d.real() * d.real() + d.imag() * d.imag();
+        }
+        dloo = s;                                       // Dznrm2^2
+        double nn;
+        std::memcpy(&nn, &ctx->h_host.p[32], sizeof(double));
+        dnormw = std::sqrt(nn);
+        ctx->dots += cnt + 1;
+    }
+
+    // zrot on one pair (green_tensor.c:21-28)
+    static void zrot1(cd &x, cd &y, double c, cd s)
+    {
+        const cd t = c * x + s * y;
+        y = c * y - std::conj(s) * x;
+        x = t;
+    }
+
+    // reference BLAS zrotg (green_tensor.c:30-33)
+    static void zrotg(cd a, cd b, double &c, cd &s)
+    {
+        const double aa = std::abs(a);
+        if (aa == 0.0) {
+            c = 0.0;
+            s = 1.0;
+            return;
+        }
+        const double scale = aa + std::abs(b);
+        const double t1 = std::abs(a / scale), t2 = std::abs(b / scale);
+        const double nrm = scale * std::sqrt(t1 * t1 + t2 * t2);
+        const cd alpha = a / aa;
+        c = aa / nrm;
+        s = alpha * std::conj(b) / nrm;
+    }
+
+    // ApplyPreconditionerRight + MatrixVectorMult: w = A M^-1 v_jH (Gmres.cs:239-241)
+    void precond_and_apply(int jH)
+    {
+        if (flexible) {
+            double bea, be;
+            int conv;
+            inner->solve(col(V, jH), col(V, jH), col(Z, jH), bea, be, conv);   // FlexibleGmresWithGmresPreconditioner.cs:57
+            matvec(col(Z, jH), w.p);
+        } else {
+            matvec(col(V, jH), w.p);
+        }
+    }
+
+    // ConstructCurrentSolution (Gmres.cs:187-196): y = H^-1 g, x += B y
+    void construct(int jH, double2 *x)
+    {
+        std::vector<cd> y(jH);
+        for (int i = 0; i < jH; ++i) y[i] = h(i, m);
+        for (int i = jH - 1; i >= 0; --i) {          // SimplifiedZtrsv (green_tensor.c:34-37)
+            cd t = y[i];
+            for (int k = i + 1; k < jH; ++k) t -= h(i, k) * y[k];
+            y[i] = t / h(i, i);
+        }
+        xtr::Coef32 c{};
+        for (int i = 0; i < jH; ++i) c.c[i] = make_double2(y[i].real(), y[i].imag());
+        if (jH > 0) xtr::launch_update_solution(flexible ? Z.p : V.p, jH, c, x, n, ctx->stream);
+    }
+
+    // CheckBackwardError (Gmres.cs:200-230)
+    double check_backward_error(double bn, const double2 *rhs, const double2 *x)
+    {
+        matvec(x, w.p);
+        xtr::launch_sub(rhs, w.p, w.p, n, ctx->stream);
+        const double be = norm(w.p) / bn;
+        if (be >= tol && backward_check == 2) throw std::runtime_error("be = " + std::to_string(be) + ", tolerance = " + std::to_string(tol));
+        return be;
+    }
+
+    double final_actions(int jH, double bn, const double2 *rhs, double2 *x)
+    {
+        construct(jH, x);
+        if (backward_check != 0) return check_backward_error(bn, rhs, x);
+        return -1;
+    }
+
+    // KrylovSubspaceConstruction (Gmres.cs:233-295)
+    double krylov(double bn, int &jH)
+    {
+        double bea = -1;
+        do {
+            precond_and_apply(jH);
+            for (int j = 0; j <= jH; ++j) h(j, jH) = 0;
+            int north = 0;
+            double dloo = 0, dnormw = 0;
+            do {
+                ++north;
+                cgs(jH, dloo, dnormw);
+            } while ((2.0 * dnormw <= std::sqrt(dloo)) && (north < 3));
+            h(jH + 1, jH) = dnormw;
+            if (jH < m - 1) xtr::launch_div(w.p, dnormw, col(V, jH + 1), n, ctx->stream);
+            for (int j = 0; j < jH; ++j) zrot1(h(j, jH), h(j + 1, jH), gc[j], gs[j]);
+            zrotg(h(jH, jH), h(jH + 1, jH), gc[jH], gs[jH]);
+            zrot1(h(jH, m), h(jH + 1, m), gc[jH], gs[jH]);
+            zrot1(h(jH, jH), h(jH + 1, jH), gc[jH], gs[jH]);
+            h(jH + 1, jH) = 0;
+            bea = std::abs(h(jH + 1, m)) / bn;
+            ++iter_counter;
+            if (flexible && ctx->on_iter) ctx->on_iter(ctx->user, ctx->rhs_index, iter_counter, bea);
+            ++jH;
+        } while (bea > tol && jH < m);
+        return bea;
+    }
+
+    // PrepareRestart, ComputeTheTrue (Gmres.cs:145-162)
+    double restart(const double2 *rhs, double2 *x, int jH)
+    {
+        construct(jH, x);
+        matvec(x, r0.p);
+        xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
+        return norm(w.p);
+    }
+
+    // GmresSolver.Solve (Gmres.cs:77-143), initial guess supplied
+    void solve(const double2 *rhs, const double2 *x0, double2 *result, double &bea_out, double &be_out, int &conv)
+    {
+        iter_counter = 0;
+        const double bn = norm(rhs);
+        if (bn == 0) {
+            bea_out = 0; be_out = 0; conv = 1;
+            return;
+        }
+        matvec(x0, r0.p);
+        xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
+        double beta = norm(w.p);
+        if (beta == 0) {
+            if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
+            bea_out = 0; be_out = 0; conv = 1;
+            return;
+        }
+        double bea = -1, be = -1;
+        int iter = 0;
+        if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
+        for (;;) {
+            xtr::launch_mul(w.p, 1 / beta, col(V, 0), n, ctx->stream);
+            h(0, m) = beta;
+            for (int i = 1; i <= m; ++i) h(i, m) = 0;
+            int jH = 0;
+            bea = krylov(bn, jH);
+            if (bea < tol) {
+                be = final_actions(jH, bn, rhs, result);
+                bea_out = bea; be_out = be; conv = 1;
+                return;
+            }
+            ++iter;
+            if (iter >= max_repeat) break;
+            beta = restart(rhs, result, jH);
+        }
+        be = final_actions(m, bn, rhs, result);
+        bea_out = bea; be_out = be; conv = 0;
+    }
+};
+
+// FlexibleGmresWithGmresPreconditioner (:29-58) on one right-hand side.
+void run_fgmres(int64_t n, const xtr_gmres_params &p, SolveCtx &ctx, const double2 *b, double2 *x,
+                xtr_result_info &info)
+{
+    require(p.outer_buffer_length >= 1 && p.outer_buffer_length <= 31, "OuterBufferLength must be 1..31");
+    require(p.inner_buffer_length >= 1 && p.inner_buffer_length <= 31, "InnerBufferLength must be 1..31");
+    require(p.max_repeat_number >= 1, "MaxRepeatsNumber must be >= 1");
+    auto t0 = std::chrono::steady_clock::now();
+    Gmres inner(n, p.inner_buffer_length, false, &ctx);
+    inner.tol = p.tolerance;
+    inner.max_repeat = 1;
+    inner.backward_check = 0;
+    Gmres outer(n, p.outer_buffer_length, true, &ctx);
+    outer.tol = p.tolerance;
+    outer.max_repeat = p.max_repeat_number;
+    outer.backward_check = p.backward_error_checking;
+    outer.inner = &inner;
+    double bea = -1, be = -1;
+    int conv = 0;
+    outer.solve(b, x, x, bea, be, conv);
+    XTR_CHECK(hipStreamSynchronize(ctx.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    info.converged = conv;
+    info.arnoldi_backward_error = bea;
+    info.backward_error = be;
+    info.number_of_iterations = outer.iter_counter;
+    info.matvecs = ctx.matvecs;
+    info.dot_products = ctx.dots;
+    info.seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Pairs up the operator applies of solves that run side by side on their own streams: the
+// apply fires when every still-running solve has submitted one vector, so the Green tensor
+// is streamed once per round for all of them.
+struct ApplyGate {
+    xtr_op *op;
+    std::mutex mu;
+    std::condition_variable cv;
+    int active;
+    int pending = 0;
+    uint64_t round = 0;
+    struct Sub { const double2 *x; double2 *y; hipStream_t s; hipEvent_t ready; };
+    std::vector<Sub> subs;
+    hipEvent_t done = nullptr;
+    std::exception_ptr err;
+
+    ApplyGate(xtr_op *o, int n) : op(o), active(n) { XTR_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming)); }
+    ~ApplyGate()
+    {
+        if (done) (void)hipEventDestroy(done);
+    }
+
+    void fire()
+    {
+        try {
+            xtr::RhsPtrs v{};
+            for (size_t i = 0; i < subs.size(); ++i) {
+                XTR_CHECK(hipStreamWaitEvent(op->stream, subs[i].ready, 0));
+                v.x[i] = subs[i].x;
+                v.y[i] = subs[i].y;
+            }
+            op->apply_raw(v, (int)subs.size(), op->stream);
+            XTR_CHECK(hipEventRecord(done, op->stream));
+            for (auto &s : subs) XTR_CHECK(hipStreamWaitEvent(s.s, done, 0));
+        } catch (...) {
+            err = std::current_exception();
+        }
+        subs.clear();
+        pending = 0;
+        ++round;
+        cv.notify_all();
+    }
+
+    void submit(const double2 *x, double2 *y, hipStream_t s, hipEvent_t ready)
+    {
+        XTR_CHECK(hipEventRecord(ready, s));
+        std::unique_lock<std::mutex> lk(mu);
+        if (err) std::rethrow_exception(err);
+        subs.push_back({x, y, s, ready});
+        ++pending;
+        const uint64_t mine = round;
+        if (pending == active) fire();
+        else cv.wait(lk, [&] { return round != mine; });
+        if (err) std::rethrow_exception(err);
+    }
+
+    void leave()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        --active;
+        if (pending > 0 && pending == active) fire();
+    }
+};
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+
+extern "C" {
+
+int xtr_last_error(char *buf, int len)
+{
+    const int n = (int)g_last_error.size();
+    if (buf && len > 0) {
+        const int c = n < len - 1 ? n : len - 1;
+        std::memcpy(buf, g_last_error.data(), c);
+        buf[c] = 0;
+    }
+    return n;
+}
+
+int xtr_op_create(const xtr_op_desc *d, xtr_op **out)
+{
+    return guarded([&] {
+        require(d && out, "null argument");
+        require(d->nx > 0 && d->ny > 0 && d->nz > 0, "grid sizes must be positive");
+        require(3 * d->nz * 1 <= 1024, "Nz too large for the contraction kernel");
+        require(xtr::fft_length_supported(2 * d->nx) && xtr::fft_length_supported(2 * d->ny),
+                "2*Nx and 2*Ny must be one of 8,16,24,32,48,64,96,128,192,256,384,512");
+        require(d->precision == XTR_FP64 || d->precision == XTR_FP32, "precision must be XTR_FP64 or XTR_FP32");
+        require(d->max_rhs >= 1 && d->max_rhs <= xtr::kMaxRhs, "max_rhs must be 1..8");
+        require(3 * d->nz * d->max_rhs <= 1024, "3*Nz*max_rhs must not exceed 1024");
+        require(d->nlayers >= 3, "the section needs the air, at least one layer and the lower half-space");
+        auto op = std::make_unique<xtr_op>();
+        op->g = xtr::Geom{d->nx, d->ny, d->nz};
+        op->precision = d->precision;
+        op->max_rhs = d->max_rhs;
+        op->omega = d->omega;
+        op->dx = d->dx;
+        op->dy = d->dy;
+        op->depth.assign(d->depth, d->depth + d->nz);
+        op->thick.assign(d->thick, d->thick + d->nz);
+        op->section.zero_level = d->zero_level;
+        op->section.thickness.assign(d->layer_thickness, d->layer_thickness + d->nlayers);
+        op->section.sigma.assign(d->layer_sigma, d->layer_sigma + d->nlayers);
+        op->number_of_hankels = d->number_of_hankels;
+        if (d->hank0 && d->hank1 && d->ntaps > 0) {
+            op->hank0.assign(d->hank0, d->hank0 + d->ntaps);
+            op->hank1.assign(d->hank1, d->hank1 + d->ntaps);
+        }
+        for (int k = 0; k < d->nz; ++k) background_layer(op->section, op->depth[k], op->depth[k] + op->thick[k]);
+        XTR_CHECK(hipStreamCreateWithFlags(&op->stream, hipStreamDefault));
+        for (auto &e : op->ev) XTR_CHECK(hipEventCreate(&e));
+        const xtr::Geom &g = op->g;
+        // packed (rc, tr) table in CombineSymm order (TranceiverUtils.cs:65-74)
+        std::vector<uint16_t> prc(2 * g.S());
+        for (int tr = 0; tr < g.nz; ++tr)
+            for (int rc = 0; rc <= tr; ++rc) {
+                const int o = rc + tr * (tr + 1) / 2;
+                prc[2 * o] = (uint16_t)rc;
+                prc[2 * o + 1] = (uint16_t)tr;
+            }
+        upload(op->packed_rc, prc);
+        upload(op->tw64_x, twiddles64(2 * g.nx));
+        upload(op->tw64_y, twiddles64(2 * g.ny));
+        upload(op->tw32_x, twiddles32(2 * g.nx));
+        upload(op->tw32_y, twiddles32(2 * g.ny));
+        op->buf.alloc((size_t)op->max_rhs * g.M() * op->elem());
+        op->R.alloc((size_t)g.nx * g.ny * g.nz);
+        op->f.alloc(g.nz);
+        op->b.alloc(g.nz);
+        *out = op.release();
+    });
+}
+
+void xtr_op_destroy(xtr_op *op)
+{
+    if (op) {
+        (void)hipStreamSynchronize(op->stream);
+        delete op;
+    }
+}
+
+static void finish_green(xtr_op *op)
+{
+    if (op->precision == XTR_FP32) {
+        op->G32.alloc(op->G64.count);
+        xtr::launch_to_f32(op->G64.p, op->G32.p, (int64_t)op->G64.count, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        op->G64.release();
+    }
+    op->green_ready = true;
+}
+
+int xtr_op_set_green(xtr_op *op, const xtr_complex *xx, const xtr_complex *xy, const xtr_complex *xz,
+                     const xtr_complex *yy, const xtr_complex *yz, const xtr_complex *zz)
+{
+    return guarded([&] {
+        require(op && xx && xy && xz && yy && yz && zz, "null argument");
+        const xtr::Geom &g = op->g;
+        const int64_t S = g.S(), A = (int64_t)g.nz * g.nz, L = g.L();
+        const xtr_complex *host[6] = {xx, xy, xz, yy, yz, zz};
+        const int64_t sizes[6] = {S, S, A, S, A, S};
+        DevBuf<double2> tmp[6];
+        const double2 *dev[6];
+        for (int c = 0; c < 6; ++c) {
+            tmp[c].alloc((size_t)(L * sizes[c]));
+            XTR_CHECK(hipMemcpy(tmp[c].p, host[c], L * sizes[c] * sizeof(double2), hipMemcpyHostToDevice));
+            dev[c] = tmp[c].p;
+        }
+        op->G64.alloc((size_t)(L * g.gblk()));
+        op->G32.release();
+        xtr::launch_green_interleave(dev, op->G64.p, g, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        finish_green(op);
+    });
+}
+
+int xtr_op_build_green(xtr_op *op)
+{
+    return guarded([&] {
+        require(op, "null argument");
+        require(op->hank0.size() == 431 && op->hank1.size() == 431, "the 431-tap Hankel filter is required");
+        require(op->number_of_hankels > 0, "NumberOfHankels must be positive");
+        const xtr::Geom &g = op->g;
+        op->G64.alloc((size_t)(g.L() * g.gblk()));
+        op->G32.release();
+        xtr::GreenInput in;
+        in.g = g;
+        in.dx = op->dx;
+        in.dy = op->dy;
+        in.depth = op->depth;
+        in.thick = op->thick;
+        in.section = op->section;
+        in.omega = op->omega;
+        in.number_of_hankels = op->number_of_hankels;
+        in.hank0 = op->hank0;
+        in.hank1 = op->hank1;
+        xtr::build_green_atoa(in, op->G64.p, op->tw64_x.p, op->tw64_y.p, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        finish_green(op);
+    });
+}
+
+int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *xz, xtr_complex *yy,
+                     xtr_complex *yz, xtr_complex *zz)
+{
+    return guarded([&] {
+        require(op && xx && xy && xz && yy && yz && zz, "null argument");
+        require(op->green_ready, "the Green tensor is not set");
+        const xtr::Geom &g = op->g;
+        const int64_t S = g.S(), A = (int64_t)g.nz * g.nz, L = g.L();
+        xtr_complex *host[6] = {xx, xy, xz, yy, yz, zz};
+        const int64_t sizes[6] = {S, S, A, S, A, S};
+        DevBuf<double2> tmp[6];
+        double2 *dev[6];
+        for (int c = 0; c < 6; ++c) {
+            tmp[c].alloc((size_t)(L * sizes[c]));
+            dev[c] = tmp[c].p;
+        }
+        DevBuf<double2> full;
+        const double2 *src = op->G64.p;
+        if (op->precision == XTR_FP32) {
+            // widen the complex64 copy for export
+            std::vector<float2> h32(op->G32.count);
+            XTR_CHECK(hipMemcpy(h32.data(), op->G32.p, h32.size() * sizeof(float2), hipMemcpyDeviceToHost));
+            std::vector<double2> h64(h32.size());
+            for (size_t i = 0; i < h32.size(); ++i) h64[i] = xtr::to_d(h32[i]);
+            full.alloc(h64.size());
+            XTR_CHECK(hipMemcpy(full.p, h64.data(), h64.size() * sizeof(double2), hipMemcpyHostToDevice));
+            src = full.p;
+        }
+        xtr::launch_green_deinterleave(src, dev, g, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        for (int c = 0; c < 6; ++c)
+            XTR_CHECK(hipMemcpy(host[c], dev[c], L * sizes[c] * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta)
+{
+    return guarded([&] {
+        require(op && zeta, "null argument");
+        require(type == XTR_OP_A || type == XTR_OP_CHI0, "type must be XTR_OP_A or XTR_OP_CHI0");
+        const xtr::Geom &g = op->g;
+        const int nz = g.nz;
+        // GetBackgroundZeta (ConvolutionOperator.cs:80-88)
+        std::vector<cd> z0(nz);
+        for (int k = 0; k < nz; ++k) {
+            const int li = background_layer(op->section, op->depth[k], op->depth[k] + op->thick[k]);
+            z0[k] = zeta_of(op->section.sigma[li], op->omega);
+        }
+        // PrepareForwardFactors / PrepareBackwardFactors (:90-111)
+        std::vector<double> f(nz), b(nz);
+        const double nf = 1 / ((double)g.nx * (double)g.ny * 4);
+        for (int k = 0; k < nz; ++k) {
+            f[k] = std::sqrt(z0[k].real());
+            b[k] = 2 * std::sqrt(z0[k].real()) / op->thick[k] * nf;
+        }
+        // PrepareRFunction (:113-143)
+        std::vector<double2> R((size_t)g.nx * g.ny * nz);
+        for (int64_t idx = 0; idx < (int64_t)R.size(); ++idx) {
+            const int k = (int)(idx % nz);
+            const cd z(zeta[idx].re, zeta[idx].im);
+            const cd cz0 = std::conj(z0[k]);
+            const cd r = type == XTR_OP_CHI0 ? std::sqrt(cd(z0[k].real(), 0)) / (z + cz0) : (z - z0[k]) / (z + cz0);
+            R[idx] = make_double2(r.real(), r.imag());
+        }
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        XTR_CHECK(hipMemcpy(op->R.p, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice));
+        XTR_CHECK(hipMemcpy(op->f.p, f.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+        XTR_CHECK(hipMemcpy(op->b.p, b.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+        op->type = type;
+    });
+}
+
+int xtr_op_apply_device(xtr_op *op, const void *x, void *y, int nrhs)
+{
+    return guarded([&] {
+        require(op && x && y, "null argument");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs out of range");
+        xtr::RhsPtrs v{};
+        const int64_t n = op->g.n();
+        for (int r = 0; r < nrhs; ++r) {
+            v.x[r] = (const double2 *)x + r * n;
+            v.y[r] = (double2 *)y + r * n;
+        }
+        op->apply_raw(v, nrhs, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    });
+}
+
+int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs)
+{
+    return guarded([&] {
+        require(op && x && y, "null argument");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs out of range");
+        const int64_t n = op->g.n();
+        if (op->stage_x.count < (size_t)(nrhs * n)) {
+            op->stage_x.alloc((size_t)(op->max_rhs * n));
+            op->stage_y.alloc((size_t)(op->max_rhs * n));
+        }
+        XTR_CHECK(hipMemcpy(op->stage_x.p, x, nrhs * n * sizeof(double2), hipMemcpyHostToDevice));
+        xtr::RhsPtrs v{};
+        for (int r = 0; r < nrhs; ++r) {
+            v.x[r] = op->stage_x.p + r * n;
+            v.y[r] = op->stage_y.p + r * n;
+        }
+        op->apply_raw(v, nrhs, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        XTR_CHECK(hipMemcpy(y, op->stage_y.p, nrhs * n * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_op_set_profiling(xtr_op *op, int enable)
+{
+    return guarded([&] {
+        require(op, "null argument");
+        op->profiling = enable != 0;
+    });
+}
+
+int xtr_op_stage_times(xtr_op *op, double *out5)
+{
+    return guarded([&] {
+        require(op && out5, "null argument");
+        require(op->profiling, "profiling is off");
+        XTR_CHECK(hipEventSynchronize(op->ev[5]));
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0;
+            XTR_CHECK(hipEventElapsedTime(&ms, op->ev[i], op->ev[i + 1]));
+            out5[i] = ms;
+        }
+    });
+}
+
+int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes)
+{
+    return guarded([&] {
+        require(op, "null argument");
+        if (n) *n = op->g.n();
+        if (green_bytes) *green_bytes = (int64_t)(op->g.L() * op->g.gblk() * op->elem());
+        if (buffer_bytes) *buffer_bytes = (int64_t)(op->g.M() * op->elem());
+    });
+}
+
+int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const void *b, void *x,
+                            xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
+{
+    return guarded([&] {
+        require(op && params && b && x && info, "null argument");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        SolveCtx ctx;
+        ctx.stream = op->stream;
+        ctx.on_iter = on_iteration;
+        ctx.user = user;
+        xtr_op *o = op;
+        ctx.apply = [o](const double2 *xx, double2 *yy) {
+            xtr::RhsPtrs v{};
+            v.x[0] = xx;
+            v.y[0] = yy;
+            o->apply_raw(v, 1, o->stream);
+        };
+        std::memset(info, 0, sizeof(*info));
+        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info);
+    });
+}
+
+int xtr_fgmres_solve(xtr_op *op, const xtr_gmres_params *params, const xtr_complex *b, xtr_complex *x,
+                     xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
+{
+    return guarded([&] {
+        require(op && params && b && x && info, "null argument");
+        const int64_t n = op->g.n();
+        DevBuf<double2> db(n), dx(n);
+        XTR_CHECK(hipMemcpy(db.p, b, n * sizeof(double2), hipMemcpyHostToDevice));
+        XTR_CHECK(hipMemcpy(dx.p, x, n * sizeof(double2), hipMemcpyHostToDevice));
+        if (xtr_fgmres_solve_device(op, params, db.p, dx.p, info, on_iteration, user) != 0)
+            throw std::runtime_error(g_last_error);
+        XTR_CHECK(hipMemcpy(x, dx.p, n * sizeof(double2), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, int nrhs, const void *const *b,
+                                  void *const *x, xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
+{
+    return guarded([&] {
+        require(op && params && b && x && info, "null argument");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs must be within 1..max_rhs");
+        ApplyGate gate(op, nrhs);
+        std::vector<std::unique_ptr<SolveCtx>> ctxs;
+        std::vector<hipStream_t> streams(nrhs, nullptr);
+        std::vector<hipEvent_t> ready(nrhs, nullptr);
+        std::vector<std::exception_ptr> errs(nrhs);
+        std::mutex cb_mu;
+        struct CbUser { xtr_iteration_fn fn; void *user; std::mutex *mu; };
+        CbUser cbu{on_iteration, user, &cb_mu};
+        auto cb = [](void *u, int r, int it, double bea) {
+            auto *c = (CbUser *)u;
+            std::lock_guard<std::mutex> lk(*c->mu);
+            c->fn(c->user, r, it, bea);
+        };
+        try {
+            for (int r = 0; r < nrhs; ++r) {
+                XTR_CHECK(hipStreamCreateWithFlags(&streams[r], hipStreamDefault));
+                XTR_CHECK(hipEventCreateWithFlags(&ready[r], hipEventDisableTiming));
+                auto c = std::make_unique<SolveCtx>();
+                c->stream = streams[r];
+                c->rhs_index = r;
+                if (on_iteration) {
+                    c->on_iter = cb;
+                    c->user = &cbu;
+                }
+                hipStream_t s = streams[r];
+                hipEvent_t ev = ready[r];
+                ApplyGate *gp = &gate;
+                c->apply = [gp, s, ev](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev); };
+                ctxs.push_back(std::move(c));
+            }
+            // inputs written on the caller's default stream must be visible to the solve streams
+            XTR_CHECK(hipDeviceSynchronize());
+            std::vector<std::thread> th;
+            for (int r = 0; r < nrhs; ++r) {
+                th.emplace_back([&, r] {
+                    try {
+                        std::memset(&info[r], 0, sizeof(info[r]));
+                        run_fgmres(op->g.n(), *params, *ctxs[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
+                    } catch (...) {
+                        errs[r] = std::current_exception();
+                    }
+                    gate.leave();
+                });
+            }
+            for (auto &t : th) t.join();
+            XTR_CHECK(hipStreamSynchronize(op->stream));
+        } catch (...) {
+            for (auto &s : streams)
+                if (s) (void)hipStreamSynchronize(s);
+            for (auto &e : ready)
+                if (e) (void)hipEventDestroy(e);
+            for (auto &s : streams)
+                if (s) (void)hipStreamDestroy(s);
+            throw;
+        }
+        for (auto &s : streams) (void)hipStreamSynchronize(s);
+        for (auto &e : ready) (void)hipEventDestroy(e);
+        for (auto &s : streams) (void)hipStreamDestroy(s);
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Host-side launchers of the gfx950 kernels. Every launcher enqueues on the given stream and
+// checks the launch; none synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace xtr {
+
+constexpr int kMaxRhs = 8;
+
+// Geometry of one operator (Nx, Ny, Nz) and derived sizes.
+struct Geom {
+    int nx, ny, nz;
+    int64_t n() const { return 3LL * nx * ny * nz; }          // unknowns
+    int64_t L() const { return 4LL * nx * ny; }               // padded lateral plane
+    int64_t C() const { return 3LL * nz; }                    // spectrum columns per wavenumber
+    int64_t M() const { return L() * C(); }                   // spectrum elements
+    int64_t S() const { return (int64_t)nz * (nz + 1) / 2; }  // packed symmetric block
+    int64_t gblk() const { return 4 * S() + 2LL * nz * nz; }  // Green elements per wavenumber
+};
+
+// Per-call vector pointers (device, FP64) for up to kMaxRhs right-hand sides.
+struct RhsPtrs {
+    const double2 *x[kMaxRhs];
+    double2 *y[kMaxRhs];
+};
+
+// ---- operator apply (op_kernels.hip) -------------------------------------------------------
+// Twiddle tables exp(-2 pi i k / P), k < P, in the apply precision.
+// precision: 0 = FP64 (double2 buffers / Green), 1 = FP32 (float2).
+struct ApplyArgs {
+    Geom g;
+    int precision;
+    int nrhs;
+    int type;                 // 0 = A, 1 = Chi0
+    const double2 *R;         // [nx*ny*nz] FP64
+    const double *f;          // [nz] forward factors (real)
+    const double *b;          // [nz] backward factors (real)
+    const void *G;            // [L][gblk] in apply precision
+    const uint16_t *packed_rc;// [S][2] (rc, tr) of each packed element
+    void *buf;                // nrhs * M spectrum in apply precision
+    const void *tw_x;         // twiddles for P = 2 nx
+    const void *tw_y;         // twiddles for P = 2 ny
+    RhsPtrs v;
+};
+
+// Stage launchers (used separately for per-stage timing).
+void launch_fwd_y(const ApplyArgs &a, hipStream_t s);
+void launch_fwd_x(const ApplyArgs &a, hipStream_t s);
+void launch_contract(const ApplyArgs &a, hipStream_t s);
+void launch_inv_x(const ApplyArgs &a, hipStream_t s);
+void launch_inv_y(const ApplyArgs &a, hipStream_t s);
+
+// In-place forward 2-D FFT of a [2nx][2ny][ncols] array (Green tensor), FP64.
+void launch_fft2_plain(double2 *data, int n1, int n2, int64_t ncols, const double2 *tw1, const double2 *tw2,
+                       int sign, hipStream_t s);
+bool fft_length_supported(int p);
+
+// Reorders the Green tensor between the reference layout (six component arrays) and the
+// device layout [L][xx S | xy S | yy S | zz S | xz nz^2 | yz nz^2].
+void launch_green_interleave(const double2 *comp[6], double2 *dev, const Geom &g, hipStream_t s);
+void launch_green_deinterleave(const double2 *dev, double2 *comp[6], const Geom &g, hipStream_t s);
+void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
+
+// ---- Krylov (krylov_kernels.hip) -----------------------------------------------------------
+constexpr int kRedBlocks = 512;   // partial-sum blocks of the reductions
+// partial[k][blk] of conj(V_k) . w, k < cnt (V columns contiguous, stride n)
+void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s);
+// out[k] = sum_blk partial[k][blk]
+void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s);
+// w -= sum_k V_k h_k (h on device), partial norms of the new w
+void launch_cgs_update(const double2 *V, int cnt, const double2 *h, double2 *w, int64_t n, double *partial,
+                       hipStream_t s);
+// partial sums of |w|^2 ; out = sum of partials
+void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s);
+void launch_reduce_r(const double *partial, double *out, hipStream_t s);
+// dst = src / d  (v_{j+1} = w / |w|, Gmres.cs:267-268)
+void launch_div(const double2 *src, double d, double2 *dst, int64_t n, hipStream_t s);
+// dst = a * src  (V_0 = (1/beta) w, Gmres.cs:117)
+void launch_mul(const double2 *src, double a, double2 *dst, int64_t n, hipStream_t s);
+// dst = a - b
+void launch_sub(const double2 *a, const double2 *b, double2 *dst, int64_t n, hipStream_t s);
+// x += sum_k B_k y_k  (ConstructCurrentSolution, y passed by value)
+struct Coef32 { double2 c[32]; };
+void launch_update_solution(const double2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s);
+
+}  // namespace xtr

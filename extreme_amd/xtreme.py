@@ -1,0 +1,284 @@
+"""Python host interface of libxtreme_hip.so, mirroring the reference's operator and solver surface.
+
+Names follow the C# classes they stand in for:
+  ConvolutionOperator          Extreme.Cartesian/Forward/AtoA/ConvolutionOperator.cs
+  AnomalyCurrentFgmresSolver   Extreme.Cartesian/Forward/AnomalyCurrentFgmresSolver.cs
+  ForwardSettings              Extreme.Cartesian/Forward/Project/ForwardSettings.cs
+  ResultInfo                   Extreme.Fgmres/ResultInfo.cs
+
+Device vectors are torch tensors (complex128, on the GPU) used only as memory; every computation
+runs in the HIP library. There is no CPU fallback: without the built library or a GPU, the
+constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import cartesian_model as cm
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxtreme_hip.so")
+_lib = None
+
+XTR_FP64, XTR_FP32 = 0, 1
+XTR_OP_A, XTR_OP_CHI0 = 0, 1
+
+
+class XtrError(RuntimeError):
+    pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("depth", C.POINTER(C.c_double)), ("thick", C.POINTER(C.c_double)), ("nlayers", C.c_int),
+                ("zero_level", C.c_double), ("layer_thickness", C.POINTER(C.c_double)),
+                ("layer_sigma", C.POINTER(C.c_double)), ("omega", C.c_double), ("number_of_hankels", C.c_int),
+                ("hank0", C.POINTER(C.c_double)), ("hank1", C.POINTER(C.c_double)), ("ntaps", C.c_int),
+                ("precision", C.c_int), ("max_rhs", C.c_int)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("outer_buffer_length", C.c_int), ("inner_buffer_length", C.c_int),
+                ("max_repeat_number", C.c_int), ("tolerance", C.c_double), ("backward_error_checking", C.c_int)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("converged", C.c_int), ("arnoldi_backward_error", C.c_double), ("backward_error", C.c_double),
+                ("number_of_iterations", C.c_int), ("matvecs", C.c_int64), ("dot_products", C.c_int64),
+                ("seconds", C.c_double)]
+
+
+ITER_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double)
+
+
+def lib():
+    """Loads libxtreme_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise XtrError(f"{LIB_PATH} is missing: build it with `python -m extreme_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.xtr_op_create.argtypes = [C.POINTER(_Desc), C.POINTER(C.c_void_p)]
+        L.xtr_op_destroy.argtypes = [C.c_void_p]
+        L.xtr_op_destroy.restype = None
+        L.xtr_op_build_green.argtypes = [C.c_void_p]
+        L.xtr_op_set_green.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.xtr_op_get_green.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.xtr_op_prepare.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.xtr_op_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.xtr_op_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.xtr_fgmres_solve.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p, C.POINTER(_Info),
+                                       ITER_FN, C.c_void_p]
+        L.xtr_fgmres_solve_device.argtypes = L.xtr_fgmres_solve.argtypes
+        L.xtr_fgmres_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_int,
+                                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                                    C.POINTER(_Info), ITER_FN, C.c_void_p]
+        L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        buf = C.create_string_buffer(4096)
+        lib().xtr_last_error(buf, 4096)
+        raise XtrError(buf.value.decode(errors="replace"))
+
+
+def hankel_table():
+    """The 431-tap ndec=40 J0/J1 filter (HankelCoefficients.LoadN40, HankelCoefficients.cs:37,40-65)."""
+    t = np.loadtxt(os.path.join(_HERE, "data", "hankel_ndec40.txt"))
+    return np.ascontiguousarray(t[:, 0]), np.ascontiguousarray(t[:, 1])
+
+
+@dataclass
+class ForwardSettings:
+    """ForwardSettings (Forward/Project/ForwardSettings.cs:6-18); the defaults are the project
+    reader's for a project that lists all five values."""
+    residual: float = 1e-8
+    inner_buffer_length: int = 10
+    outer_buffer_length: int = 30
+    max_repeats_number: int = 100
+    number_of_hankels: int = 10
+
+
+@dataclass
+class ResultInfo:
+    """ResultInfo (Extreme.Fgmres/ResultInfo.cs:4-36) with operation counts and wall time."""
+    convergence_achieved: bool
+    arnoldi_backward_error: float
+    backward_error: float
+    number_of_iterations: int
+    matvecs: int
+    dot_products: int
+    seconds: float
+
+    @staticmethod
+    def _from(i: _Info) -> "ResultInfo":
+        return ResultInfo(bool(i.converged), i.arnoldi_backward_error, i.backward_error, i.number_of_iterations,
+                          i.matvecs, i.dot_products, i.seconds)
+
+
+def _dptr(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor or host pointer of a numpy array."""
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    return C.c_void_p(t.ctypes.data)
+
+
+class ConvolutionOperator:
+    """The CIE operator A x = x - R x - b IFFT(G FFT(f R x)) and the Chi0 operator on the GPU
+    (ConvolutionOperator.cs:45-327), for one model at one frequency."""
+
+    COMPONENTS = ("xx", "xy", "xz", "yy", "yz", "zz")
+
+    def __init__(self, model: cm.CartesianModel, omega: float, settings: ForwardSettings | None = None,
+                 precision: str = "fp64", max_rhs: int = 2):
+        self.model = model
+        self.omega = float(omega)
+        self.settings = settings or ForwardSettings()
+        self.precision = precision
+        self.max_rhs = max_rhs
+        h0, h1 = hankel_table()
+        keep = []
+        arrs = {}
+        for name, a in (("depth", model.depth), ("thick", model.thick), ("lt", model.section.thickness),
+                        ("ls", model.section.sigma), ("h0", h0), ("h1", h1)):
+            arr, p = _dptr(a)
+            keep.append(arr)
+            arrs[name] = p
+        d = _Desc(model.nx, model.ny, model.nz, float(model.dx), float(model.dy), arrs["depth"], arrs["thick"],
+                  model.section.nlayers, float(model.section.zero_level), arrs["lt"], arrs["ls"], self.omega,
+                  int(self.settings.number_of_hankels), arrs["h0"], arrs["h1"], len(h0),
+                  XTR_FP64 if precision == "fp64" else XTR_FP32, int(max_rhs))
+        h = C.c_void_p()
+        _check(lib().xtr_op_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.n = model.n
+        self.operator_type = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xtr_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- Green tensor -------------------------------------------------------------------------
+    def calculate_green_tensor(self):
+        """AtoAGreenTensorCalculatorComponent.CalculateGreenTensor on the device (FP64)."""
+        _check(lib().xtr_op_build_green(self._h))
+
+    def set_green_tensor(self, green: dict):
+        """Uploads a Green tensor in the reference layout (component -> [2Nx*2Ny, P] complex128)."""
+        arrs = [np.ascontiguousarray(green[c], dtype=np.complex128) for c in self.COMPONENTS]
+        _check(lib().xtr_op_set_green(self._h, *[C.c_void_p(a.ctypes.data) for a in arrs]))
+
+    def green_tensor(self) -> dict:
+        m = self.model
+        L = 4 * m.nx * m.ny
+        S = m.nz * (m.nz + 1) // 2
+        out = {c: np.zeros((L, m.nz * m.nz if c in ("xz", "yz") else S), dtype=np.complex128)
+               for c in self.COMPONENTS}
+        _check(lib().xtr_op_get_green(self._h, *[C.c_void_p(out[c].ctypes.data) for c in self.COMPONENTS]))
+        return out
+
+    # --- operator -----------------------------------------------------------------------------
+    def prepare_operator(self, operator_type: str, zeta=None):
+        """PrepareOperator(greenTensor, OperatorType.A | OperatorType.Chi0) (ConvolutionOperator.cs:64-78)."""
+        if zeta is None:
+            zeta = self.model.zeta(self.omega)
+        z = np.ascontiguousarray(np.asarray(zeta, dtype=np.complex128).reshape(-1))
+        if z.size != self.model.nx * self.model.ny * self.model.nz:
+            raise ValueError("zeta must hold Nx*Ny*Nz values")
+        t = {"A": XTR_OP_A, "Chi0": XTR_OP_CHI0}[operator_type]
+        _check(lib().xtr_op_prepare(self._h, t, C.c_void_p(z.ctypes.data)))
+        self.operator_type = operator_type
+
+    def apply(self, x, y=None):
+        """Apply(input, output): numpy arrays (host) or torch GPU tensors (device), n or [nrhs, n]."""
+        nrhs = 1 if x.ndim == 1 else x.shape[0]
+        if hasattr(x, "data_ptr"):
+            if y is None:
+                y = x.new_empty(x.shape)
+            _check(lib().xtr_op_apply_device(self._h, _ptr(x), _ptr(y), nrhs))
+            return y
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if y is None:
+            y = np.empty_like(x)
+        _check(lib().xtr_op_apply(self._h, _ptr(x), _ptr(y), nrhs))
+        return y
+
+    def set_profiling(self, on: bool):
+        _check(lib().xtr_op_set_profiling(self._h, 1 if on else 0))
+
+    def stage_times(self):
+        out = (C.c_double * 5)()
+        _check(lib().xtr_op_stage_times(self._h, out))
+        return list(out)
+
+    def sizes(self):
+        n, gb, bb = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().xtr_op_sizes(self._h, C.byref(n), C.byref(gb), C.byref(bb)))
+        return n.value, gb.value, bb.value
+
+
+class AnomalyCurrentFgmresSolver:
+    """FGMRES(outer) with an inner GMRES right preconditioner on the GPU
+    (AnomalyCurrentFgmresSolver.cs:31-85 + FlexibleGmresWithGmresPreconditioner.cs)."""
+
+    def __init__(self, operator: ConvolutionOperator, settings: ForwardSettings | None = None,
+                 backward_error_checking: int = 1):
+        self.op = operator
+        s = settings or operator.settings
+        self.params = _Params(int(s.outer_buffer_length), int(s.inner_buffer_length), int(s.max_repeats_number),
+                              float(s.residual), int(backward_error_checking))
+        self.history = []
+
+    def _cb(self):
+        hist = self.history
+
+        def f(user, rhs, it, bea):
+            hist.append((rhs, it, bea))
+
+        return ITER_FN(f)
+
+    def solve(self, b, x) -> ResultInfo:
+        """Solves A x = b; x holds the initial guess (the reference passes x0 = chi0)."""
+        info = _Info()
+        cb = self._cb()
+        if hasattr(b, "data_ptr"):
+            _check(lib().xtr_fgmres_solve_device(self.op._h, C.byref(self.params), _ptr(b), _ptr(x),
+                                                 C.byref(info), cb, None))
+        else:
+            if not (b.flags["C_CONTIGUOUS"] and x.flags["C_CONTIGUOUS"]):
+                raise ValueError("host vectors must be C-contiguous")
+            _check(lib().xtr_fgmres_solve(self.op._h, C.byref(self.params), _ptr(b), _ptr(x), C.byref(info),
+                                          cb, None))
+        return ResultInfo._from(info)
+
+    def solve_batch(self, bs, xs) -> list:
+        """Side-by-side solves sharing every operator apply (device tensors)."""
+        n = len(bs)
+        infos = (_Info * n)()
+        bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
+        xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        cb = self._cb()
+        _check(lib().xtr_fgmres_solve_batch_device(self.op._h, C.byref(self.params), n, bp, xp, infos, cb, None))
+        return [ResultInfo._from(infos[i]) for i in range(n)]

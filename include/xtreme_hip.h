@@ -1,0 +1,166 @@
+/*
+ * xtreme_hip.h -- C ABI of libxtreme_hip.so, the MI355X (gfx950) engine for extrEMe's
+ * integral-equation operator apply and the FGMRES solve around it.
+ *
+ * Plain C: pointers, sizes, int status codes (0 = OK). No C++ types cross this boundary and
+ * no exception unwinds through it; on failure the message is kept per thread and returned by
+ * xtr_last_error().
+ *
+ * Complex numbers are interleaved {double re, im} (the reference's complex16,
+ * Native/Native.Matrix/algebra.h:29, = System.Numerics.Complex).
+ *
+ * Vector layout is AnomalyCurrent's: index ((i*Ny + j)*3 + c)*Nz + k, i.e. [Nx][Ny][3][Nz]
+ * (Extreme.Cartesian/Core/LayerAccessor.cs:46-47,82-86).
+ *
+ * Thread model: one host thread drives one handle at a time; host-pointer calls return after
+ * the handle's stream has finished the work. xtr_fgmres_solve_batch runs several right-hand
+ * sides on their own host threads and streams internally.
+ */
+#ifndef XTREME_HIP_H
+#define XTREME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } xtr_complex;
+
+typedef struct xtr_op xtr_op;
+
+/* Arithmetic of the operator apply. */
+enum {
+    XTR_FP64 = 0,   /* complex128 Green tensor, FFT and contraction (the reference's arithmetic) */
+    XTR_FP32 = 1    /* complex64 Green tensor, FFT and contraction; Krylov vectors stay FP64 */
+};
+
+/* Operator type (Extreme.Cartesian/Forward/AtoA/OperatorType.cs). */
+enum { XTR_OP_A = 0, XTR_OP_CHI0 = 1 };
+
+/* Model and discretisation: the fields of OmegaModel that the operator and the AtoA Green
+ * tensor read (Extreme.Cartesian/Model/Omega/OmegaModel.cs, Extreme.Core/Model/Section1D.cs,
+ * LateralDimensions.cs, AnomalyLayer.cs). Arrays are copied by xtr_op_create. */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy;
+    const double *depth;          /* [nz] top of each anomaly layer, m */
+    const double *thick;          /* [nz] thickness of each anomaly layer, m */
+    int nlayers;                  /* background layers incl. both half-spaces */
+    double zero_level;            /* depth of the top of layer 0 */
+    const double *layer_thickness;/* [nlayers], 0 for the half-spaces */
+    const double *layer_sigma;    /* [nlayers], S/m */
+    double omega;                 /* 2 pi f */
+    int number_of_hankels;        /* ForwardSettings.NumberOfHankels (> 0) */
+    const double *hank0;          /* [ntaps] J0 filter weights (HankelCoefficients.LoadN40) */
+    const double *hank1;          /* [ntaps] J1 filter weights */
+    int ntaps;                    /* 431 */
+    int precision;                /* XTR_FP64 or XTR_FP32 */
+    int max_rhs;                  /* right-hand sides one apply may carry (>= 1) */
+} xtr_op_desc;
+
+/* GmresParams as AnomalyCurrentFgmresSolver fills it (Extreme.Cartesian/Forward/
+ * AnomalyCurrentFgmresSolver.cs:40-50, Extreme.Fgmres/GmresParams.cs:4-23). */
+typedef struct {
+    int outer_buffer_length;      /* FGMRES restart m (OuterBufferLength) */
+    int inner_buffer_length;      /* inner GMRES m (InnerBufferLength) */
+    int max_repeat_number;        /* MaxRepeatsNumber */
+    double tolerance;             /* Residual */
+    int backward_error_checking;  /* 0 Arnoldi only, 1 warning, 2 error (BackwardErrorChecking) */
+} xtr_gmres_params;
+
+/* ResultInfo (Extreme.Fgmres/ResultInfo.cs:4-36) plus operation counts. */
+typedef struct {
+    int converged;
+    double arnoldi_backward_error;   /* bea */
+    double backward_error;           /* be = |b - A x| / |b|, -1 when not computed */
+    int number_of_iterations;        /* outer FGMRES iterations */
+    int64_t matvecs;                 /* operator applies */
+    int64_t dot_products;
+    double seconds;                  /* wall time of the solve */
+} xtr_result_info;
+
+/* IterationComplete (Extreme.Fgmres/Gmres.cs:64-66, AnomalyCurrentFgmresSolver.cs:132-139):
+ * rhs = index of the right-hand side in a batch, iteration = outer counter, bea. */
+typedef void (*xtr_iteration_fn)(void *user, int rhs, int iteration, double bea);
+
+/* Replaces: ConvolutionOperator ctor + FftBuffersPool.PrepareBuffersForModel
+ * (Extreme.Cartesian/Forward/AtoA/ConvolutionOperator.cs:33-43, Fft/FftBufferPool.cs:64-96),
+ * the GIEM2G-style handle pattern of Forward/AtoA/GIEM2G/Giem2gGreenTensor.cs:272-298. */
+int xtr_op_create(const xtr_op_desc *desc, xtr_op **op);
+
+/* Replaces: AtoAGreenTensorCalculatorComponent.CalculateGreenTensor
+ * (Extreme.Cartesian/Forward/AtoA/AtoAGreenTensorCalculatorComponent.cs:69-182):
+ * the FP64 AtoA Green tensor assembled, mirrored and FFT'd on the device. */
+int xtr_op_build_green(xtr_op *op);
+
+/* Replaces: handing a precomputed GreenTensor to ConvolutionOperator.PrepareOperator
+ * (ConvolutionOperator.cs:64-68; TensorCache reuse, Forward/TensorCache.cs:8-35).
+ * Host arrays in the reference layout: each component [2Nx*2Ny][P], P = Nz*Nz for xz, yz and
+ * Nz(Nz+1)/2 (packed upper, element (rc,tr) at rc + tr(tr+1)/2) for xx, xy, yy, zz
+ * (Green/Tensor/GreenTensor.cs:77-99, Model/TranceiverUtils.cs:10-16,55-74). */
+int xtr_op_set_green(xtr_op *op, const xtr_complex *xx, const xtr_complex *xy, const xtr_complex *xz,
+                     const xtr_complex *yy, const xtr_complex *yz, const xtr_complex *zz);
+
+/* Copies the device Green tensor back in the reference layout above (test and export hook;
+ * the reference's GreenTensor component pointers, GreenTensor.cs:128-139). */
+int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *xz,
+                     xtr_complex *yy, xtr_complex *yz, xtr_complex *zz);
+
+/* Replaces: ConvolutionOperator.PrepareOperator (ConvolutionOperator.cs:64-143):
+ * R, forward and backward factors for type XTR_OP_A or XTR_OP_CHI0.
+ * zeta: host [Nx][Ny][Nz] complex conductivity of the anomaly (OmegaAnomaly.Zeta). */
+int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta);
+
+/* Replaces: ConvolutionOperator.Apply (ConvolutionOperator.cs:45-62,162-327).
+ * x, y: host arrays of nrhs consecutive vectors of n = 3*Nx*Ny*Nz complex values. */
+int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs);
+
+/* Same on device-resident vectors (HIP device pointers), ordered on the handle's stream
+ * and complete on return. */
+int xtr_op_apply_device(xtr_op *op, const void *x, void *y, int nrhs);
+
+/* Replaces: AnomalyCurrentFgmresSolver.Solve -> FlexibleGmresWithGmresPreconditioner.Solve
+ * (Extreme.Cartesian/Forward/AnomalyCurrentFgmresSolver.cs:59-85,
+ *  Extreme.Fgmres/FlexibleGmresWithGmresPreconditioner.cs:29-63): FGMRES with an inner GMRES
+ * right preconditioner, iterated classical Gram-Schmidt, true residual at restarts.
+ * The operator must be prepared as XTR_OP_A. x holds the initial guess on entry (the
+ * reference passes x0 = chi0, ForwardSolver.cs:300-307) and the solution on return.
+ * Host pointers. */
+int xtr_fgmres_solve(xtr_op *op, const xtr_gmres_params *params, const xtr_complex *b, xtr_complex *x,
+                     xtr_result_info *info, xtr_iteration_fn on_iteration, void *user);
+
+/* Same with device pointers. */
+int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const void *b, void *x,
+                            xtr_result_info *info, xtr_iteration_fn on_iteration, void *user);
+
+/* nrhs independent solves with the same operator (the two polarisations of
+ * Mt3DForwardSolver.SolvePrivate, Extreme.Cartesian.Magnetotellurics/Solver/Mt3DForwardSolver.cs:95-129),
+ * run side by side so that every operator apply streams the Green tensor once for all of them.
+ * Each right-hand side follows exactly the iteration of a separate xtr_fgmres_solve.
+ * b[r], x[r]: device pointers. info: [nrhs]. */
+int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, int nrhs,
+                                  const void *const *b, void *const *x, xtr_result_info *info,
+                                  xtr_iteration_fn on_iteration, void *user);
+
+/* Device time of the last xtr_op_apply* call split by stage, in milliseconds:
+ * out[0] forward pass along y (with R and f fused), out[1] forward pass along x,
+ * out[2] contraction, out[3] inverse pass along x, out[4] inverse pass along y (with b and the
+ * last step fused). Timing is recorded only after xtr_op_set_profiling(op, 1). */
+int xtr_op_set_profiling(xtr_op *op, int enable);
+int xtr_op_stage_times(xtr_op *op, double *out5);
+
+/* Sizes: n = 3 Nx Ny Nz, the Green tensor's device bytes, the spectrum buffer bytes. */
+int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes);
+
+/* Replaces: ConvolutionOperator.Dispose / GreenTensor.Dispose (ConvolutionOperator.cs:338-344). */
+void xtr_op_destroy(xtr_op *op);
+
+/* Message of the last failed call on this thread; returns its length. */
+int xtr_last_error(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,315 @@
+"""ctypes binding of the FP64 CPU restatement (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path. Only the tests, the smoke
+check and the CPU-baseline leg of bench.py import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+c128p = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class CDouble(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class Section(C.Structure):
+    _fields_ = [("nlayers", C.c_int), ("zero_level", C.c_double),
+                ("thickness", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double))]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("depth", C.POINTER(C.c_double)), ("thick", C.POINTER(C.c_double))]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m_outer", C.c_int), ("m_inner", C.c_int), ("max_repeat", C.c_int),
+                ("tol", C.c_double), ("backward_check", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("converged", C.c_int), ("bea", C.c_double), ("be", C.c_double), ("iterations", C.c_int),
+                ("inner_iterations", C.c_int), ("matvecs", C.c_int64), ("dots", C.c_int64)]
+
+
+class Op(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("xx", "xy", "xz", "yy", "yz", "zz", "r", "f", "b")] + \
+               [("type", C.c_int), ("buf1", C.c_void_p), ("buf2", C.c_void_p), ("rx", C.c_void_p)]
+
+
+MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
+ITER = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.xo_zeta.restype = CDouble
+        L.xo_zeta.argtypes = [C.c_double, C.c_double]
+        L.xo_plane_wave_e.restype = CDouble
+        L.xo_plane_wave_e.argtypes = [C.POINTER(Section), C.c_double, C.c_double, C.c_double]
+        L.xo_plane_wave_h.restype = CDouble
+        L.xo_plane_wave_h.argtypes = [C.POINTER(Section), C.c_double, C.c_double]
+        L.xo_fft2_batch.argtypes = [c128p, C.c_int, C.c_int, C.c_int64, C.c_int]
+        L.xo_fft1.argtypes = [c128p, C.c_int, C.c_int]
+        L.xo_green_atoa.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, C.c_int, f64p, f64p,
+                                    C.c_int] + [c128p] * 6
+        L.xo_green_scalars.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, f64p, f64p,
+                                       C.c_void_p, C.c_int, c128p]
+        L.xo_plan_create.argtypes = [C.POINTER(Grid), C.c_int, C.c_void_p]
+        L.xo_plan_free.argtypes = [C.c_void_p]
+        L.xo_contract.argtypes = [C.c_int, C.c_int64] + [c128p] * 8
+        L.xo_op_init.argtypes = [C.POINTER(Op), C.c_int, C.c_int, C.c_int] + [c128p] * 6
+        L.xo_op_prepare.argtypes = [C.POINTER(Op), C.c_int, c128p, c128p, f64p]
+        L.xo_op_apply.argtypes = [C.POINTER(Op), C.c_void_p, C.c_void_p]
+        L.xo_op_free.argtypes = [C.POINTER(Op)]
+        L.xo_fgmres_solve.argtypes = [C.POINTER(GmresParams), MATVEC, C.c_void_p, ITER, C.c_void_p,
+                                      c128p, c128p, C.POINTER(Result)]
+        L.xo_gmres_inner.argtypes = [C.POINTER(GmresParams), MATVEC, C.c_void_p, c128p, c128p,
+                                     C.POINTER(Result)]
+        L.xo_zrotg.argtypes = [CDouble, CDouble, C.POINTER(C.c_double), C.POINTER(CDouble)]
+        L.xo_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int):
+    lib().xo_set_threads(int(n))
+
+
+def _cplx(v):
+    return complex(v.re, v.im)
+
+
+class _Keep:
+    """Holds numpy arrays alive for ctypes structures that point into them."""
+
+    def __init__(self):
+        self.refs = []
+
+    def d(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.refs.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_section(sec, keep):
+    return Section(sec.nlayers, float(sec.zero_level), keep.d(sec.thickness), keep.d(sec.sigma))
+
+
+def make_grid(model, keep):
+    return Grid(model.nx, model.ny, model.nz, float(model.dx), float(model.dy),
+                keep.d(model.depth), keep.d(model.thick))
+
+
+def plane_wave_e(section, omega, depth, thick=0.0):
+    k = _Keep()
+    s = make_section(section, k)
+    return _cplx(lib().xo_plane_wave_e(C.byref(s), omega, depth, thick))
+
+
+def plane_wave_h(section, omega, depth):
+    k = _Keep()
+    s = make_section(section, k)
+    return _cplx(lib().xo_plane_wave_h(C.byref(s), omega, depth))
+
+
+def fft2_batch(buf, n1, n2, nc, sign):
+    out = np.ascontiguousarray(buf, dtype=np.complex128).copy()
+    lib().xo_fft2_batch(out.reshape(-1), n1, n2, nc, sign)
+    return out
+
+
+def fft1(x, sign=-1):
+    out = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().xo_fft1(out, len(out), sign)
+    return out
+
+
+def hankel_table():
+    """The ndec=40 J0/J1 filter weights (data copied from the reference's hankelcoeff.xml)."""
+    path = os.path.join(os.path.dirname(_HERE), "extreme_amd", "data", "hankel_ndec40.txt")
+    t = np.loadtxt(path)
+    return np.ascontiguousarray(t[:, 0]), np.ascontiguousarray(t[:, 1])
+
+
+class _Plan(C.Structure):
+    _fields_ = [("nitems", C.c_int), ("nrho", C.POINTER(C.c_int)), ("rho", C.POINTER(C.POINTER(C.c_double))),
+                ("nlam", C.POINTER(C.c_int)), ("lam", C.POINTER(C.POINTER(C.c_double))),
+                ("nrho_merged", C.c_int), ("rho_merged", C.POINTER(C.c_double)),
+                ("merge_item", C.POINTER(C.c_int)), ("merge_index", C.POINTER(C.c_int))]
+
+
+def scalar_plan(model, number_of_hankels=10):
+    """ScalarPlansCreater.CreateForAnomalyToAnomaly: returns (list of rho, list of lambda, merged rho)."""
+    k = _Keep()
+    g = make_grid(model, k)
+    p = _Plan()
+    rc = lib().xo_plan_create(C.byref(g), number_of_hankels, C.byref(p))
+    if rc:
+        raise RuntimeError(f"xo_plan_create failed: {rc}")
+    rho = [np.ctypeslib.as_array(p.rho[i], (p.nrho[i],)).copy() for i in range(p.nitems)]
+    lam = [np.ctypeslib.as_array(p.lam[i], (p.nlam[i],)).copy() for i in range(p.nitems)]
+    merged = np.ctypeslib.as_array(p.rho_merged, (p.nrho_merged,)).copy()
+    item = np.ctypeslib.as_array(p.merge_item, (p.nrho_merged,)).copy()
+    index = np.ctypeslib.as_array(p.merge_index, (p.nrho_merged,)).copy()
+    lib().xo_plan_free(C.byref(p))
+    return {"rho": rho, "lam": lam, "rho_merged": merged, "merge_item": item, "merge_index": index}
+
+
+def green_scalars(model, omega, number_of_hankels=10, asym=True):
+    """Green scalars on the merged rho grid: asym -> I3 [pairs][rho]; symm -> I1, I2, I5 [3][pairs][rho]."""
+    h0, h1 = hankel_table()
+    k = _Keep()
+    s = make_section(model.section, k)
+    g = make_grid(model, k)
+    p = _Plan()
+    rc = lib().xo_plan_create(C.byref(g), number_of_hankels, C.byref(p))
+    if rc:
+        raise RuntimeError(f"xo_plan_create failed: {rc}")
+    nz = model.nz
+    npairs = nz * nz if asym else nz * (nz + 1) // 2
+    ncomp = 1 if asym else 3
+    out = np.zeros(ncomp * npairs * p.nrho_merged, dtype=np.complex128)
+    rc = lib().xo_green_scalars(C.byref(s), C.byref(g), omega, h0, h1, C.addressof(p), 1 if asym else 0, out)
+    nr = p.nrho_merged
+    lib().xo_plan_free(C.byref(p))
+    if rc:
+        raise RuntimeError("xo_green_scalars failed")
+    return out.reshape(ncomp, npairs, nr)
+
+
+COMPONENTS = ("xx", "xy", "xz", "yy", "yz", "zz")
+
+
+def green_atoa(model, omega, number_of_hankels=10, fft=True):
+    """AtoA Green tensor, reference layout: dict comp -> [2Nx*2Ny, P] complex128."""
+    h0, h1 = hankel_table()
+    k = _Keep()
+    s = make_section(model.section, k)
+    g = make_grid(model, k)
+    nz = model.nz
+    L = 4 * model.nx * model.ny
+    S = nz * (nz + 1) // 2
+    out = {c: np.zeros(L * (nz * nz if c in ("xz", "yz") else S), dtype=np.complex128) for c in COMPONENTS}
+    rc = lib().xo_green_atoa(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1, 1 if fft else 0,
+                             *[out[c] for c in COMPONENTS])
+    if rc:
+        raise RuntimeError(f"xo_green_atoa failed: {rc}")
+    return {c: v.reshape(L, -1) for c, v in out.items()}
+
+
+def contract(nz, green, src):
+    src = np.ascontiguousarray(src, dtype=np.complex128).reshape(-1)
+    length = src.size // (3 * nz)
+    dst = np.zeros_like(src)
+    lib().xo_contract(nz, length, *[np.ascontiguousarray(green[c]).reshape(-1) for c in COMPONENTS], src, dst)
+    return dst
+
+
+class Operator:
+    """ConvolutionOperator restated (Forward/AtoA/ConvolutionOperator.cs)."""
+
+    def __init__(self, model, green):
+        self.model = model
+        self._g = {c: np.ascontiguousarray(green[c]).reshape(-1) for c in COMPONENTS}
+        self.op = Op()
+        rc = lib().xo_op_init(C.byref(self.op), model.nx, model.ny, model.nz, *[self._g[c] for c in COMPONENTS])
+        if rc:
+            raise MemoryError("xo_op_init")
+
+    def prepare(self, op_type, zeta, zeta0):
+        """op_type 'A' or 'Chi0'."""
+        self._zeta = np.ascontiguousarray(zeta, dtype=np.complex128).reshape(-1)
+        self._zeta0 = np.ascontiguousarray(zeta0, dtype=np.complex128)
+        self._dz = np.ascontiguousarray(self.model.thick, dtype=np.float64)
+        lib().xo_op_prepare(C.byref(self.op), 0 if op_type == "A" else 1, self._zeta, self._zeta0, self._dz)
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        y = np.zeros_like(x)
+        lib().xo_op_apply(C.byref(self.op), x.ctypes.data, y.ctypes.data)
+        return y
+
+    def raw_apply(self, xptr, yptr):
+        lib().xo_op_apply(C.byref(self.op), xptr, yptr)
+
+    def __del__(self):
+        try:
+            lib().xo_op_free(C.byref(self.op))
+        except Exception:
+            pass
+
+
+def fgmres(matvec, b, x0, m_outer=30, m_inner=10, max_repeat=100, tol=1e-8, backward_check=1):
+    """FlexibleGmresWithGmresPreconditioner.Solve(b, x0, x0). matvec(xptr, yptr) works on raw pointers.
+    Returns (x, info dict, list of (outer?, iteration, bea))."""
+    b = np.ascontiguousarray(b, dtype=np.complex128).reshape(-1)
+    x = np.ascontiguousarray(x0, dtype=np.complex128).reshape(-1).copy()
+    hist = []
+
+    def _mv(ctx, xp, yp):
+        matvec(xp, yp)
+
+    def _it(ctx, outer, it, bea):
+        hist.append((outer, it, bea))
+
+    mv = MATVEC(_mv)
+    itf = ITER(_it)
+    p = GmresParams(b.size, m_outer, m_inner, max_repeat, tol, backward_check)
+    r = Result()
+    lib().xo_fgmres_solve(C.byref(p), mv, None, itf, None, b, x, C.byref(r))
+    info = {"converged": bool(r.converged), "bea": r.bea, "be": r.be, "iterations": r.iterations,
+            "inner_iterations": r.inner_iterations, "matvecs": r.matvecs, "dots": r.dots}
+    return x, info, hist
+
+
+def zrotg(a, b):
+    c = C.c_double()
+    s = CDouble()
+    lib().xo_zrotg(CDouble(a.real, a.imag), CDouble(b.real, b.imag), C.byref(c), C.byref(s))
+    return c.value, complex(s.re, s.im)
+
+
+def normal_field(model, omega):
+    """E0 per anomaly layer (Mt3DForwardSolver.CalculateNormalFieldFromSource)."""
+    return np.array([plane_wave_e(model.section, omega, float(d + t / 2), float(t))
+                     for d, t in zip(model.depth, model.thick)], dtype=np.complex128)
+
+
+def jscattered(model, pol, zeta, zeta0, e0):
+    """j_s = (zeta - zeta0) E0 in the [Nx][Ny][3][Nz] layout (ForwardSolver.cs:225-237)."""
+    nx, ny, nz = model.nx, model.ny, model.nz
+    js = np.zeros((nx, ny, 3, nz), dtype=np.complex128)
+    js[:, :, pol, :] = (zeta.reshape(nx, ny, nz) - zeta0[None, None, :]) * e0[None, None, :]
+    return js.reshape(-1)
+
+
+def escattered_jq(model, zeta, zeta0, chi, js):
+    nx, ny, nz = model.nx, model.ny, model.nz
+    z = zeta.reshape(nx, ny, 1, nz)
+    z0 = zeta0.reshape(1, 1, 1, nz)
+    chi = chi.reshape(nx, ny, 3, nz)
+    js = js.reshape(nx, ny, 3, nz)
+    es = (2 * np.sqrt(z0.real + 0j) * chi - js) / (z + np.conj(z0))
+    jq = es * (z - z0) + js
+    return es.reshape(-1), jq.reshape(-1)

@@ -1,0 +1,753 @@
+/*
+ * xo_green.c -- oracle: anomaly-to-anomaly (AtoA) electric Green tensor.
+ * TEST INFRASTRUCTURE ONLY (see xtr_oracle.h).
+ *
+ * Restates, in order:
+ *   scalar plans (ScalarPlansCreater.cs, ScalarPlan.cs, HankelCoefficients.cs:67-76),
+ *   Green scalars (GreenScalarCalculator.cs:54-131, PlanCalculatorFast.cs:80-318,
+ *                  VolumeToVolumeCalculatorUFast.cs:34-337, green_scalar.c:30-386,
+ *                  ScalarMathUtils.cs:99-263, InnerResultsCombiner.cs:19-62),
+ *   segments (ScalarSegments.cs:86-128),
+ *   lateral cell integration (GreenTensorCalculator.cs:166-552, IntegralFactors.cs:17-30,
+ *                  QSign.cs:64-67, AtoAGreenTensorCalculator.cs:229-321),
+ *   mirroring (GreenTensorFftPopulator.cs:37-148) and the forward FFT
+ *                 (AtoAGreenTensorCalculatorComponent.cs:197-279).
+ */
+#include "xtr_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define XO_PI 3.14159265358979323846
+#define NTAP 431
+#define HANK_L2 130 /* HankelCoefficients._l2; n1 = M1 - l2 = -130 */
+
+/* ---------------------------------------------------------------------- */
+/* plans                                                                   */
+/* ---------------------------------------------------------------------- */
+
+/* HankelCoefficients.GetLog10RhoStep (:67): (float)Math.Pow(10, 1/_ndec), _ndec float 40 */
+static double log10_rho_step(void)
+{
+    float inv = 1.0f / 40.0f;
+    return (double)(float)pow(10.0, (double)inv);
+}
+
+typedef struct { double *v; int n, cap; } dvec;
+
+static void dpush(dvec *a, double x)
+{
+    if (a->n == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 64;
+        a->v = realloc(a->v, sizeof(double) * a->cap);
+    }
+    a->v[a->n++] = x;
+}
+
+/* CreateRhoFromMin (ScalarPlansCreater.cs:142-157) */
+static dvec rho_from_min(double rmin, double rmax, double step)
+{
+    dvec r = {0};
+    double next = rmin;
+    while (next <= rmax) { dpush(&r, next); next *= step; }
+    dpush(&r, next);
+    return r;
+}
+
+/* CreateRhoFromMax (ScalarPlansCreater.cs:159-176) */
+static dvec rho_from_max(double rmin, double rmax, double step)
+{
+    dvec r = {0};
+    double next = rmax;
+    while (next >= rmin) { dpush(&r, next); next /= step; }
+    dpush(&r, next);
+    for (int i = 0; i < r.n / 2; i++) { double t = r.v[i]; r.v[i] = r.v[r.n - 1 - i]; r.v[r.n - 1 - i] = t; }
+    return r;
+}
+
+/* ScalarPlan.CalculateLambdasForLog10 (ScalarPlan.cs:76-97) */
+static double *lambdas_for(const double *rho, int nrho, int *nlam_out)
+{
+    double step = log10_rho_step();
+    int nlam = NTAP + nrho;
+    double *k = malloc(sizeof(double) * nlam);
+    k[HANK_L2] = 1 / rho[0];
+    for (int i = HANK_L2 + 1; i <= nlam - 1; i++) k[i] = k[i - 1] / step;
+    for (int i = HANK_L2 - 1; i >= 0; i--) k[i] = k[i + 1] * step;
+    *nlam_out = nlam;
+    return k;
+}
+
+static int cmp_rho(const void *a, const void *b)
+{
+    const double *x = a, *y = b;
+    if (x[0] < y[0]) return -1;
+    if (x[0] > y[0]) return 1;
+    /* first occurrence wins: order by (item, index) */
+    if (x[1] < y[1]) return -1;
+    if (x[1] > y[1]) return 1;
+    return (x[2] < y[2]) ? -1 : (x[2] > y[2]);
+}
+
+int xo_plan_create(const xo_grid *g, int number_of_hankels, xo_plan *plan)
+{
+    memset(plan, 0, sizeof(*plan));
+    int nh = abs(number_of_hankels);
+    if (nh < 1) return -1;
+    /* FindRhoMin / FindRhoMax for the anomaly (ScalarPlansCreater.cs:360-380) */
+    double h1 = g->dx / 2, h2 = g->dy / 2;
+    double rmin = h1 < h2 ? h1 : h2;
+    double mx = g->nx * g->dx - h1, my = g->ny * g->dy - h2;
+    double rmax = sqrt(mx * mx + my * my);
+    double step = log10_rho_step();
+    /* AddLog10Plans (:123-140) */
+    plan->nitems = nh;
+    plan->nrho = calloc(nh, sizeof(int));
+    plan->rho = calloc(nh, sizeof(double *));
+    plan->nlam = calloc(nh, sizeof(int));
+    plan->lam = calloc(nh, sizeof(double *));
+    dvec r1 = rho_from_min(rmin, rmax, step);
+    double rr2 = r1.v[r1.n - 1], rr1 = r1.v[r1.n - 2];
+    double st = (rr2 - rr1) / nh;
+    plan->rho[0] = r1.v;
+    plan->nrho[0] = r1.n;
+    for (int i = 1; i < nh; i++) {
+        dvec rn = rho_from_max(rmin, rr2 - i * st, step);
+        plan->rho[i] = rn.v;
+        plan->nrho[i] = rn.n;
+    }
+    for (int i = 0; i < nh; i++) plan->lam[i] = lambdas_for(plan->rho[i], plan->nrho[i], &plan->nlam[i]);
+    /* InnerResultsCombiner.CreateFromSample (:19-33): sorted unique rho, first wins */
+    int tot = 0;
+    for (int i = 0; i < nh; i++) tot += plan->nrho[i];
+    double *all = malloc(sizeof(double) * 3 * tot);
+    int c = 0;
+    for (int i = 0; i < nh; i++)
+        for (int j = 0; j < plan->nrho[i]; j++) {
+            all[3 * c] = plan->rho[i][j]; all[3 * c + 1] = i; all[3 * c + 2] = j; c++;
+        }
+    qsort(all, tot, 3 * sizeof(double), cmp_rho);
+    plan->rho_merged = malloc(sizeof(double) * tot);
+    plan->merge_item = malloc(sizeof(int) * tot);
+    plan->merge_index = malloc(sizeof(int) * tot);
+    int u = 0;
+    for (int t = 0; t < tot; t++) {
+        if (u > 0 && all[3 * t] == plan->rho_merged[u - 1]) continue;
+        plan->rho_merged[u] = all[3 * t];
+        plan->merge_item[u] = (int)all[3 * t + 1];
+        plan->merge_index[u] = (int)all[3 * t + 2];
+        u++;
+    }
+    plan->nrho_merged = u;
+    free(all);
+    /* ScalarSegments takes its radii from ScalarPlan.GetSortedRho (all values, duplicates kept);
+     * the two grids coincide only without duplicates, which is what the reference assumes. */
+    if (u != tot) {
+        fprintf(stderr, "xo_plan_create: duplicate rho values (%d of %d unique)\n", u, tot);
+        return -2;
+    }
+    return 0;
+}
+
+void xo_plan_free(xo_plan *plan)
+{
+    for (int i = 0; i < plan->nitems; i++) { free(plan->rho[i]); free(plan->lam[i]); }
+    free(plan->rho); free(plan->lam); free(plan->nrho); free(plan->nlam);
+    free(plan->rho_merged); free(plan->merge_item); free(plan->merge_index);
+    memset(plan, 0, sizeof(*plan));
+}
+
+/* ---------------------------------------------------------------------- */
+/* scalars                                                                 */
+/* ---------------------------------------------------------------------- */
+
+static double layer_depth(const xo_section *s, int index)
+{
+    double d = s->zero_level;
+    for (int i = 0; i < index; i++) d += s->thickness[i];
+    return d;
+}
+
+/* Per-item, per-lambda layer quantities (PlanCalculatorFast.cs:203-318) */
+typedef struct {
+    int nl, nlam;
+    xo_c *eta, *ex, *p1, *q1, *ps, *qs; /* [layer][lam] */
+} item_aux;
+
+static void item_aux_build(const xo_section *s, double omega, const double *lam, int nlam, item_aux *a)
+{
+    int nl = s->nlayers;
+    double mu0 = xo_mu0();
+    a->nl = nl; a->nlam = nlam;
+    size_t sz = (size_t)nl * nlam;
+    a->eta = malloc(sizeof(xo_c) * sz); a->ex = malloc(sizeof(xo_c) * sz);
+    a->p1 = calloc(sz, sizeof(xo_c)); a->q1 = calloc(sz, sizeof(xo_c));
+    a->ps = calloc(sz, sizeof(xo_c)); a->qs = calloc(sz, sizeof(xo_c));
+    xo_c *zeta = malloc(sizeof(xo_c) * nl);
+    for (int k = 0; k < nl; k++) zeta[k] = xo_zeta(s->sigma[k], omega);
+    for (int k = 0; k < nl; k++) {
+        xo_c value = (I * omega) * zeta[k] * mu0;          /* iOmega * zeta * Mu0 (:222-229) */
+        for (int m = 0; m < nlam; m++) {
+            a->eta[k * nlam + m] = csqrt(lam[m] * lam[m] - value);          /* CalcEta */
+            a->ex[k * nlam + m] = cexp(a->eta[k * nlam + m] * (-2 * s->thickness[k])); /* CalcExp */
+        }
+    }
+#define E(k, m) a->eta[(k) * nlam + (m)]
+#define X(k, m) a->ex[(k) * nlam + (m)]
+    for (int m = 0; m < nlam; m++) {
+        /* CalculateP1 (:236-249) + CalculatePBelow1 (:266-280), alpha1 = 1 */
+        a->p1[1 * nlam + m] = (E(1, m) - E(0, m)) / (E(1, m) + E(0, m));
+        /* CalculatePSigma (:251-264), alphaSigma[k] = zeta[k+1]/zeta[k] (AlphaBeta.cs:24-42) */
+        a->ps[1 * nlam + m] = (zeta[0] * E(1, m) - zeta[1] * E(0, m)) / (zeta[0] * E(1, m) + zeta[1] * E(0, m));
+        for (int k = 1; k < nl - 1; k++) {
+            xo_c pe = a->p1[k * nlam + m] * X(k, m);
+            xo_c part = 1.0 * (E(k, m) / E(k + 1, m)) * ((pe - 1) / (pe + 1));
+            a->p1[(k + 1) * nlam + m] = (1 + part) / (1 - part);
+            pe = a->ps[k * nlam + m] * X(k, m);
+            part = (zeta[k + 1] / zeta[k]) * (E(k, m) / E(k + 1, m)) * ((pe - 1) / (pe + 1));
+            a->ps[(k + 1) * nlam + m] = (1 + part) / (1 - part);
+        }
+        /* CalculateQ (:292-318), beta1 = 1, betaSigma[k] = zeta[k-1]/zeta[k] */
+        int n = nl - 1;
+        a->q1[n * nlam + m] = 0;
+        a->qs[n * nlam + m] = 0;
+        xo_c part = 1.0 * (E(n, m) / E(n - 1, m));
+        a->q1[(n - 1) * nlam + m] = (1 - part) / (1 + part);
+        part = (zeta[n - 1] / zeta[n]) * (E(n, m) / E(n - 1, m));
+        a->qs[(n - 1) * nlam + m] = (1 - part) / (1 + part);
+        for (int k = n - 1; k > 1; k--) {
+            xo_c qe = a->q1[k * nlam + m] * X(k, m);
+            part = 1.0 * (E(k, m) / E(k - 1, m)) * ((qe - 1) / (qe + 1));
+            a->q1[(k - 1) * nlam + m] = (1 + part) / (1 - part);
+            qe = a->qs[k * nlam + m] * X(k, m);
+            part = (zeta[k - 1] / zeta[k]) * (E(k, m) / E(k - 1, m)) * ((qe - 1) / (qe + 1));
+            a->qs[(k - 1) * nlam + m] = (1 + part) / (1 - part);
+        }
+    }
+#undef E
+#undef X
+    free(zeta);
+}
+
+static void item_aux_free(item_aux *a)
+{
+    free(a->eta); free(a->ex); free(a->p1); free(a->q1); free(a->ps); free(a->qs);
+}
+
+/* A[rc][tr] at lambda m for the P/Q set given (CalculateA :160-200, CalculateASigma :138-158) */
+static xo_c ann_at(const xo_section *s, double omega, const item_aux *a, const xo_c *P, const xo_c *Q,
+                   int rc, int tr, int m, int sigma)
+{
+    int nlam = a->nlam;
+    int smax = rc > tr ? rc : tr, rmin = rc < tr ? rc : tr;
+    int n = smax;
+    xo_c prev = 1.0 / (a->eta[n * nlam + m] * (1 - P[n * nlam + m] * Q[n * nlam + m] * a->ex[n * nlam + m]));
+    for (int k = n - 1; k >= rmin; k--) {
+        double dk1 = layer_depth(s, k + 1);
+        xo_c q = ((1 + P[(k + 1) * nlam + m]) / (1 + P[k * nlam + m] * a->ex[k * nlam + m])) *
+                 cexp((a->eta[(k + 1) * nlam + m] - a->eta[k * nlam + m]) * dk1);
+        prev = q * prev;
+    }
+    /* prev = A[rmin][smax] = A[smax][rmin] before the Sigma rescaling */
+    if (sigma && rc > tr) {
+        xo_c zrc = xo_zeta(s->sigma[rc], omega), ztr = xo_zeta(s->sigma[tr], omega);
+        /* a[n,k] = (zeta_n / zeta_k) a[k,n] with n = rc > k = tr */
+        return (zrc / ztr) * prev;
+    }
+    return prev;
+}
+
+typedef struct {
+    int r, s, t, b;
+    double dt, db1, r1, r2, s1, s2;
+} envelop; /* NativeEnvelop (green_scalar.c:13-28) */
+
+static xo_c e4(xo_c a, xo_c b, xo_c c, xo_c d) { return cexp(a) - cexp(b) - cexp(c) + cexp(d); }
+
+/* U1..U5 for one transceiver pair at one lambda (VolumeToVolumeCalculatorUFast.cs:34-337).
+ * which: bit0 U1, bit1 U2, bit2 U3, bit3 U4, bit4 U5. */
+static void pair_u(const xo_section *sec, double omega, const item_aux *a, const envelop *ne,
+                   double lambda, int m, int which, int kase, xo_c *u)
+{
+    int nlam = a->nlam;
+    double mu0 = xo_mu0();
+    xo_c er = a->eta[ne->r * nlam + m], es = a->eta[ne->s * nlam + m];
+    xo_c et = a->eta[ne->t * nlam + m], eb = a->eta[ne->b * nlam + m];
+    /* PreCalculate: T, R, Ctop, Cbot, Dtop, Dbot (green_scalar.c:86-386) */
+    xo_c T = e4(-er * ne->r1 - es * ne->s1 + 2 * ne->dt * et, -er * ne->r1 - es * ne->s2 + 2 * ne->dt * et,
+                -er * ne->r2 - es * ne->s1 + 2 * ne->dt * et, -er * ne->r2 - es * ne->s2 + 2 * ne->dt * et);
+    xo_c R = e4(-2 * ne->db1 * eb + ne->s2 * es + ne->r2 * er, -2 * ne->db1 * eb + ne->s1 * es + ne->r2 * er,
+                -2 * ne->db1 * eb + ne->s2 * es + ne->r1 * er, -2 * ne->db1 * eb + ne->s1 * es + ne->r1 * er);
+    xo_c Ctop = 0, Cbot = 0, Dtop = 0, Dbot = 0;
+    xo_c F = -2 * ne->db1 * eb + 2 * ne->dt * et;
+    if (kase == 2) { /* rc above tr */
+        Ctop = e4(-ne->s1 * es + ne->r2 * er, -ne->s2 * es + ne->r2 * er,
+                  -ne->s1 * es + ne->r1 * er, -ne->s2 * es + ne->r1 * er);
+        Dtop = e4(F + ne->s2 * es - ne->r1 * er, F + ne->s1 * es - ne->r1 * er,
+                  F + ne->s2 * es - ne->r2 * er, F + ne->s1 * es - ne->r2 * er);
+    } else if (kase == 1) { /* rc below tr */
+        Cbot = e4(-ne->r1 * er + ne->s2 * es, -ne->r1 * er + ne->s1 * es,
+                  -ne->r2 * er + ne->s2 * es, -ne->r2 * er + ne->s1 * es);
+        Dbot = e4(F - ne->s1 * es + ne->r2 * er, F - ne->s2 * es + ne->r2 * er,
+                  F - ne->s1 * es + ne->r1 * er, F - ne->s2 * es + ne->r1 * er);
+    }
+    for (int set = 0; set < 2; set++) {
+        /* set 0: P1/Q1/A1 (U1); set 1: PSigma/QSigma/ASigma (U2..U5) */
+        if (set == 0 && !(which & 1)) continue;
+        if (set == 1 && !(which & 30)) continue;
+        const xo_c *P = set ? a->ps : a->p1;
+        const xo_c *Q = set ? a->qs : a->q1;
+        xo_c Pt = P[ne->t * nlam + m], Qb = Q[ne->b * nlam + m];
+        xo_c ann = ann_at(sec, omega, a, P, Q, ne->r, ne->s, m, set);
+        /* psi for U1/U2/U5 (Psi1And5...) and U3/U4 (Psi3And4...) */
+        xo_c psi1, psi3;
+        if (kase == 0) {
+            int n = ne->r;
+            double tau = ne->r2 - ne->r1;
+            double dn1 = layer_depth(sec, n + 1) - layer_depth(sec, n);
+            xo_c en = a->eta[n * nlam + m];
+            xo_c Pn = P[n * nlam + m], Qn = Q[n * nlam + m];
+            /* CalcPsi1RcAndTrMatch (:313-332) */
+            xo_c part1 = tau * en - (1 - cexp(-en * tau));
+            xo_c part2 = Pn * Qn * ((tau * en + 1) * cexp(-2 * en * dn1) - cexp(-2 * en * dn1 + en * tau));
+            psi1 = 2 * (part1 - part2);
+            psi3 = 0; /* CalcPsi3And4RcAndTrMatch (:255-260) */
+        } else if (kase == 1) {
+            psi1 = Cbot + Pt * Qb * Dbot;   /* CalcPsi1And5RcBelowTr (:242-253) */
+            psi3 = -Cbot + Pt * Qb * Dbot;  /* CalcPsi3And4RcBelowTr (:275-286) */
+        } else {
+            psi1 = Ctop + Pt * Qb * Dtop;   /* CalcPsi1And5RcAboveTr (:229-240) */
+            psi3 = Ctop - Pt * Qb * Dtop;   /* CalcPsi3And4RcAboveTr (:262-273) */
+        }
+        xo_c fac = 1 / (er * es);
+        if (set == 0) {
+            /* CalculateU1And5Special (:144-159) */
+            u[0] = ann * fac * (1 * Pt * T + Qb * R + 1 * psi1);
+            continue;
+        }
+        if (which & 2) u[1] = ann * (1 * Pt * T + Qb * R - 1 * psi1);          /* U2Special (:183-193) */
+        if (which & 4) u[2] = ann * (1 / es) * (-1 * Pt * T + Qb * R + 1 * psi3); /* U3Special (:195-210) */
+        if (which & 8) u[3] = ann * (1 / er) * (-1 * Pt * T + Qb * R - 1 * psi3); /* U4Special (:212-227) */
+        if (which & 16) {
+            if (kase == 0) {
+                /* CalculateU5SpecialForTrAndRcMatch (:161-181) + CalcPsi5RcAndTrMatch (:290-311) */
+                int n = ne->r;
+                double tau = ne->r2 - ne->r1;
+                double ln = sec->thickness[n];
+                xo_c kk = (I * omega) * xo_zeta(sec->sigma[n], omega) * mu0;
+                xo_c en = a->eta[n * nlam + m];
+                xo_c Pn = P[n * nlam + m], Qn = Q[n * nlam + m];
+                xo_c part1 = -(1 - cexp(-en * tau));
+                xo_c part2 = Pn * Qn * (cexp(-en * (2 * ln - tau)) - cexp(-2 * en * ln));
+                xo_c part3 = (kk / (lambda * lambda)) * (1 - Pn * Qn * cexp(-2 * en * ln)) * tau * en;
+                xo_c psi5 = 2 * (part1 + part2 + part3);
+                u[4] = (ann * fac) * (Pt * T + Qb * R + psi5);
+            } else {
+                u[4] = ann * fac * (1 * Pt * T + Qb * R + 1 * psi1);
+            }
+        }
+    }
+}
+
+/* HankelCoefficients.ConvoluteWithFast (:103-118): fr[i] = sum_j revHank[j] fk[i + j] */
+static xo_c hankel_dot(const double *hank, const xo_c *fk, int i)
+{
+    double re = 0, im = 0;
+    for (int j = 0; j < NTAP; j++) {
+        double h = hank[NTAP - 1 - j];
+        re += creal(fk[i + j]) * h;   /* CalcDotProduct (green_scalar.c:43-55) */
+        im += cimag(fk[i + j]) * h;
+    }
+    return re + I * im;
+}
+
+int xo_green_scalars(const xo_section *sec, const xo_grid *g, double omega,
+                     const double *hank0, const double *hank1, const xo_plan *plan,
+                     int asym, xo_c *out)
+{
+    int nz = g->nz;
+    int npairs = asym ? nz * nz : nz * (nz + 1) / 2;
+    int nrho = plan->nrho_merged;
+    int ncomp = asym ? 1 : 3;
+    double mu0 = xo_mu0();
+    int *pr = malloc(sizeof(int) * npairs), *pt = malloc(sizeof(int) * npairs);
+    if (asym) {
+        for (int rc = 0; rc < nz; rc++)
+            for (int tr = 0; tr < nz; tr++) { pr[rc * nz + tr] = rc; pt[rc * nz + tr] = tr; }
+    } else {
+        int c = 0;
+        for (int tr = 0; tr < nz; tr++)
+            for (int rc = 0; rc <= tr; rc++) { pr[c] = rc; pt[c] = tr; c++; }
+    }
+    item_aux *aux = malloc(sizeof(item_aux) * plan->nitems);
+    for (int it = 0; it < plan->nitems; it++)
+        item_aux_build(sec, omega, plan->lam[it], plan->nlam[it], &aux[it]);
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < npairs; p++) {
+        int rc = pr[p], tr = pt[p];
+        double r1 = g->depth[rc], r2 = g->depth[rc] + g->thick[rc];
+        double s1 = g->depth[tr], s2 = g->depth[tr] + g->thick[tr];
+        envelop ne;
+        ne.r = xo_layer_at_depth(sec, r1 + (r2 - r1) / 2);  /* AuxContainerFast.CreateContainer */
+        ne.s = xo_layer_at_depth(sec, s1 + (s2 - s1) / 2);
+        if (ne.r < 0 || ne.s < 0) { err = 1; continue; }
+        ne.r1 = r1; ne.r2 = r2; ne.s1 = s1; ne.s2 = s2;
+        ne.t = ne.r < ne.s ? ne.r : ne.s;
+        ne.b = ne.r > ne.s ? ne.r : ne.s;
+        ne.dt = layer_depth(sec, ne.t);
+        ne.db1 = layer_depth(sec, ne.b + 1);
+        /* CalcUGeneral (:90-103) */
+        double zw = r1 + (r2 - r1) / 2;
+        int kase = (zw > s1 && zw < s2) ? 0 : (zw >= s2 ? 1 : 2);
+        xo_c zrc2 = 2 * xo_zeta(sec->sigma[ne.r], omega);
+        xo_c iwm = I * omega * mu0;
+        xo_c **items = malloc(sizeof(xo_c *) * plan->nitems);
+        for (int it = 0; it < plan->nitems; it++) {
+            int nlam = plan->nlam[it], nr = plan->nrho[it];
+            const double *lam = plan->lam[it];
+            xo_c *U = malloc(sizeof(xo_c) * nlam * ncomp);  /* [comp][lam] */
+            for (int m = 0; m < nlam; m++) {
+                xo_c u[5];
+                if (asym) {
+                    pair_u(sec, omega, &aux[it], &ne, lam[m], m, 4, kase, u);
+                    U[m] = u[2] * (lam[m] / zrc2);                         /* NormalizeBeforeHankelForE */
+                } else {
+                    pair_u(sec, omega, &aux[it], &ne, lam[m], m, 1 | 2 | 16, kase, u);
+                    U[0 * nlam + m] = u[0] * (iwm / 2);
+                    U[1 * nlam + m] = u[1] / zrc2;
+                    U[2 * nlam + m] = u[4] * ((lam[m] * lam[m]) / zrc2);
+                }
+            }
+            xo_c *Ir = malloc(sizeof(xo_c) * nr * ncomp);
+            for (int i = 0; i < nr; i++) {
+                double rho = plan->rho[it][i];
+                if (asym) {
+                    Ir[i] = hankel_dot(hank0, U, i) / (2 * XO_PI * rho);                 /* I3: J0, +1/(2 pi rho) */
+                } else {
+                    Ir[0 * nr + i] = -hankel_dot(hank1, U + 0 * nlam, i) / (2 * XO_PI * rho); /* I1: J1, -1/(2 pi rho) */
+                    Ir[1 * nr + i] = hankel_dot(hank1, U + 1 * nlam, i) / (2 * XO_PI * rho);  /* I2: J1 */
+                    Ir[2 * nr + i] = hankel_dot(hank1, U + 2 * nlam, i) / (2 * XO_PI * rho);  /* I5: J1 */
+                }
+            }
+            free(U);
+            items[it] = Ir;
+        }
+        for (int k = 0; k < nrho; k++) {
+            int it = plan->merge_item[k], idx = plan->merge_index[k], nr = plan->nrho[it];
+            for (int c = 0; c < ncomp; c++)
+                out[((int64_t)c * npairs + p) * nrho + k] = items[it][c * nr + idx];
+        }
+        for (int it = 0; it < plan->nitems; it++) free(items[it]);
+        free(items);
+    }
+    for (int it = 0; it < plan->nitems; it++) item_aux_free(&aux[it]);
+    free(aux); free(pr); free(pt);
+    return err ? -1 : 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* lateral integration                                                     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct { int index; double radius; } knot;
+
+/* PrepareKnot (GreenTensorCalculator.cs:207-228) */
+static knot make_knot(const double *radii, int nr, double x, double y)
+{
+    double r = sqrt(x * x + y * y);
+    int lo = 0, hi = nr - 1, idx = -1;
+    /* Array.BinarySearch: exact index or (insertion point - 1) */
+    while (lo <= hi) {
+        int mid = lo + (hi - lo) / 2;
+        if (radii[mid] == r) { idx = mid; break; }
+        if (radii[mid] < r) lo = mid + 1; else hi = mid - 1;
+    }
+    if (idx < 0) idx = lo - 1;
+    if (idx == nr - 1) idx--;
+    if (idx < 0) idx = 0;
+    knot k = {idx, r};
+    return k;
+}
+
+typedef struct { double d1a, d1b, d2a, d2b, d3a, d3b; } factors;
+
+/* IntegralFactors.Prepare (IntegralFactors.cs:17-30) */
+static factors prep_factors(double s, double u, double rs, double ru, double v)
+{
+    factors f;
+    f.d1a = v == 0 ? (u * ru - s * rs) / 2 : (u * ru - s * rs + v * v * (log((u + ru) / (s + rs)))) / 2;
+    f.d1b = u - s;
+    f.d2a = v == 0 ? 0 : -(u - s) * v;
+    f.d2b = v == 0 ? 0 : -(v * log((u + ru) / (s + rs)));
+    f.d3a = -(u * u - s * s) / 2;
+    f.d3b = -(ru - rs);
+    return f;
+}
+
+/* One pair-chunk integration context */
+typedef struct {
+    int asym;
+    int npairs, p0, pc;          /* chunk [p0, p0 + pc) */
+    int nseg;
+    const xo_c *segA, *segB;     /* [seg][comp][pair] */
+    int ncomp;
+    const double *radii;
+    int nr;
+    xo_c *q[4];                  /* asym: q[0]=xz q[1]=yz; symm: xx yy zz xy */
+    xo_c *G[4];                  /* component tensors [2Nx][2Ny][P] */
+    int ny2;
+} ictx;
+
+/* AddToQBuffAlong{X,Y}Electric (GreenTensorCalculator.cs:505-528) */
+static void add_seg(ictx *c, const factors *f, int seg, int alongY)
+{
+    int np = c->npairs;
+    for (int p = c->p0; p < c->p0 + c->pc; p++) {
+        int lp = p - c->p0;
+        if (c->asym) {
+            xo_c a3 = c->segA[((int64_t)seg * 1 + 0) * np + p], b3 = c->segB[((int64_t)seg * 1 + 0) * np + p];
+            if (alongY) { c->q[0][lp] += f->d1a * a3; c->q[0][lp] += f->d1b * b3; }  /* Qxz */
+            else { c->q[1][lp] += f->d1a * a3; c->q[1][lp] += f->d1b * b3; }          /* Qyz */
+        } else {
+            xo_c a1 = c->segA[((int64_t)seg * 3 + 0) * np + p], b1 = c->segB[((int64_t)seg * 3 + 0) * np + p];
+            xo_c a2 = c->segA[((int64_t)seg * 3 + 1) * np + p], b2 = c->segB[((int64_t)seg * 3 + 1) * np + p];
+            xo_c a5 = c->segA[((int64_t)seg * 3 + 2) * np + p], b5 = c->segB[((int64_t)seg * 3 + 2) * np + p];
+            if (!alongY) {
+                c->q[0][lp] += f->d2a * a1; c->q[0][lp] += f->d2b * b1;   /* Qxx <- I1, D2 */
+                c->q[1][lp] += f->d2a * a2; c->q[1][lp] += f->d2b * b2;   /* Qyy <- I2, D2 */
+                c->q[2][lp] += f->d2a * a5; c->q[2][lp] += f->d2b * b5;   /* Qzz <- I5, D2 */
+                c->q[3][lp] += f->d3a * a2; c->q[3][lp] += f->d3b * b2;   /* Qxy <- I2, D3 */
+            } else {
+                c->q[0][lp] += f->d2a * a2; c->q[0][lp] += f->d2b * b2;   /* Qxx <- I2, D2 */
+                c->q[1][lp] += f->d2a * a1; c->q[1][lp] += f->d2b * b1;   /* Qyy <- I1, D2 */
+                c->q[2][lp] += f->d2a * a5; c->q[2][lp] += f->d2b * b5;   /* Qzz <- I5, D2 */
+                c->q[3][lp] += f->d3a * a1; c->q[3][lp] += f->d3b * b1;   /* Qxy <- I1, D3 */
+            }
+        }
+    }
+}
+
+/* Integrate (GreenTensorCalculator.cs:367-411) */
+static void integrate(ictx *c, knot left, knot right, double s, double u, double v, int alongY)
+{
+    int nq = c->asym ? 2 : 4;
+    for (int t = 0; t < nq; t++) memset(c->q[t], 0, sizeof(xo_c) * c->pc);
+    int i1 = left.index, i2 = right.index;
+    if (i1 == i2) {
+        factors f = prep_factors(s, u, left.radius, right.radius, v);
+        add_seg(c, &f, i1, alongY);
+    } else {
+        double r1 = left.radius, r2 = c->radii[i1 + 1];
+        double sj = s, uj = sqrt(r2 * r2 - v * v);
+        factors f = prep_factors(sj, uj, r1, r2, v);
+        add_seg(c, &f, i1, alongY);
+        for (int i = i1 + 1; i < i2; i++) {
+            r1 = r2; r2 = c->radii[i + 1];
+            sj = uj; uj = sqrt(r2 * r2 - v * v);
+            f = prep_factors(sj, uj, r1, r2, v);
+            add_seg(c, &f, i, alongY);
+        }
+        sj = uj; uj = u; r1 = r2; r2 = right.radius;
+        f = prep_factors(sj, uj, r1, r2, v);
+        add_seg(c, &f, i2, alongY);
+    }
+}
+
+/* QSigns (QSign.cs:64-67), per component in q-order */
+static const int QS_ASYM[4][2] = {   /* xz, yz */
+    {1, 1},   /* AlongX */
+    {1, 1},   /* AlongY */
+    {-1, 1},  /* AlongXSymm */
+    {1, -1},  /* AlongYSymm */
+};
+static const int QS_SYMM[4][4] = {   /* xx, yy, zz, xy */
+    {1, -1, -1, -1},
+    {-1, 1, -1, -1},
+    {1, -1, -1, 1},
+    {-1, 1, -1, 1},
+};
+
+/* FillGreen (GreenTensorCalculator.cs:460-497) */
+static void fill_green(ictx *c, int i, int j, int sign, int qsel)
+{
+    int nq = c->asym ? 2 : 4;
+    int64_t shift = ((int64_t)i * c->ny2 + j) * c->npairs;
+    for (int t = 0; t < nq; t++) {
+        int qs = c->asym ? QS_ASYM[qsel][t] : QS_SYMM[qsel][t];
+        double al = (double)(sign * qs);
+        xo_c *dst = c->G[t] + shift + c->p0;
+        for (int lp = 0; lp < c->pc; lp++) dst[lp] += al * c->q[t][lp];
+    }
+}
+
+static void fill_along_x(ictx *c, int i, int j, int left, int right, int qsel)
+{
+    if (j == left) fill_green(c, i, left, -1, qsel);
+    else if (j == right) fill_green(c, i, right - 1, 1, qsel);
+    else { fill_green(c, i, j - 1, 1, qsel); fill_green(c, i, j, -1, qsel); }
+}
+
+static void fill_along_y(ictx *c, int i, int j, int left, int right, int qsel)
+{
+    if (i == left) fill_green(c, left, j, -1, qsel);
+    else if (i == right) fill_green(c, right - 1, j, 1, qsel);
+    else { fill_green(c, i - 1, j, 1, qsel); fill_green(c, i, j, -1, qsel); }
+}
+
+static void run_chunk(ictx *c, const xo_grid *g, const knot *K, const knot *ZX, const knot *ZY)
+{
+    int nx = g->nx, ny = g->ny;
+    double dx = g->dx, dy = g->dy;
+#define KN(i, j) K[(i) * (ny + 1) + (j)]
+    /* RunAlongX (GreenTensorCalculator.cs:254-298) */
+    for (int j = 0; j <= ny; j++) {
+        double y = dy * (j - 0.5);
+        for (int i = 0; i <= nx - 1; i++) {
+            double x1 = dx * (i - 0.5), x2 = dx * (i + 0.5);
+            if (x1 > 0 && x2 > 0) {
+                integrate(c, KN(i, j), KN(i + 1, j), x1, x2, y, 0);
+                fill_along_x(c, i, j, 0, ny, 0);
+            } else if (x1 < 0 && x2 < 0) {
+                integrate(c, KN(i + 1, j), KN(i, j), -x2, -x1, y, 0);
+                fill_along_x(c, i, j, 0, ny, 2);
+            } else {
+                integrate(c, ZX[j], KN(i, j), 0, -x1, y, 0);
+                fill_along_x(c, i, j, 0, ny, 2);
+                integrate(c, ZX[j], KN(i + 1, j), 0, x2, y, 0);
+                fill_along_x(c, i, j, 0, ny, 0);
+            }
+        }
+    }
+    /* RunAlongY (GreenTensorCalculator.cs:318-361) */
+    for (int i = 0; i <= nx; i++) {
+        double x = dx * (i - 0.5);
+        for (int j = 0; j <= ny - 1; j++) {
+            double y1 = dy * (j - 0.5), y2 = dy * (j + 0.5);
+            if (y1 > 0 && y2 > 0) {
+                integrate(c, KN(i, j), KN(i, j + 1), y1, y2, x, 1);
+                fill_along_y(c, i, j, 0, nx, 1);
+            } else if (y1 < 0 && y2 < 0) {
+                integrate(c, KN(i, j + 1), KN(i, j), -y2, -y1, x, 1);
+                fill_along_y(c, i, j, 0, nx, 3);
+            } else {
+                integrate(c, ZY[i], KN(i, j), 0, -y1, x, 1);
+                fill_along_y(c, i, j, 0, nx, 3);
+                integrate(c, ZY[i], KN(i, j + 1), 0, y2, x, 1);
+                fill_along_y(c, i, j, 0, nx, 1);
+            }
+        }
+    }
+#undef KN
+}
+
+/* GreenTensorFftPopulator.PopulateForFft (:37-52, 88-148) */
+static void mirror(xo_c *G, int nx, int ny, int64_t len, int xs, int ys)
+{
+    int64_t nx2 = 2 * nx, ny2 = 2 * ny;
+    for (int i = 1; i < nx; i++)
+        for (int j = 1; j < ny; j++) {
+            const xo_c *src = G + len * (i * ny2 + j);
+            xo_c *dx_ = G + len * ((nx2 - i) * ny2 + j);
+            xo_c *dy_ = G + len * (i * ny2 + ny2 - j);
+            xo_c *dxy = G + len * ((nx2 - i) * ny2 + ny2 - j);
+            for (int64_t p = 0; p < len; p++) {
+                dx_[p] += xs * src[p];
+                dy_[p] += ys * src[p];
+                dxy[p] += (xs * ys) * src[p];
+            }
+        }
+    for (int i = 1; i < nx; i++) {
+        const xo_c *src = G + len * (i * ny2);
+        xo_c *dst = G + len * ((nx2 - i) * ny2);
+        for (int64_t p = 0; p < len; p++) dst[p] += xs * src[p];
+    }
+    for (int j = 1; j < ny; j++) {
+        const xo_c *src = G + len * j;
+        xo_c *dst = G + len * (ny2 - j);
+        for (int64_t p = 0; p < len; p++) dst[p] += ys * src[p];
+    }
+}
+
+static int tensor_group(const xo_section *sec, const xo_grid *g, double omega, const double *hank0,
+                        const double *hank1, const xo_plan *plan, int asym, int fft, xo_c **G)
+{
+    int nz = g->nz, nx = g->nx, ny = g->ny;
+    int npairs = asym ? nz * nz : nz * (nz + 1) / 2;
+    int ncomp = asym ? 1 : 3;
+    int nrho = plan->nrho_merged;
+    int nseg = nrho - 1;
+    xo_c *Isc = malloc(sizeof(xo_c) * (size_t)ncomp * npairs * nrho);
+    if (xo_green_scalars(sec, g, omega, hank0, hank1, plan, asym, Isc)) { free(Isc); return -1; }
+    /* ScalarSegments.CalculateScalarSegment (ScalarSegments.cs:86-128) */
+    xo_c *segA = malloc(sizeof(xo_c) * (size_t)nseg * ncomp * npairs);
+    xo_c *segB = malloc(sizeof(xo_c) * (size_t)nseg * ncomp * npairs);
+    const double *radii = plan->rho_merged;
+    for (int k = 0; k < nseg; k++) {
+        double s = radii[k], u = radii[k + 1], div = u - s;
+        for (int c = 0; c < ncomp; c++)
+            for (int p = 0; p < npairs; p++) {
+                xo_c i0 = Isc[((int64_t)c * npairs + p) * nrho + k], i1 = Isc[((int64_t)c * npairs + p) * nrho + k + 1];
+                xo_c a = (i1 - i0) / div;
+                segA[((int64_t)k * ncomp + c) * npairs + p] = a;
+                segB[((int64_t)k * ncomp + c) * npairs + p] = i0 - a * s;
+            }
+    }
+    free(Isc);
+    /* PrepareKnots (GreenTensorCalculator.cs:166-205) */
+    knot *K = malloc(sizeof(knot) * (nx + 1) * (ny + 1));
+    knot *ZX = malloc(sizeof(knot) * (ny + 1)), *ZY = malloc(sizeof(knot) * (nx + 1));
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j <= ny; j++)
+            K[i * (ny + 1) + j] = make_knot(radii, nrho, g->dx * (i - 0.5), g->dy * (j - 0.5));
+    for (int j = 0; j <= ny; j++) ZX[j] = make_knot(radii, nrho, 0, g->dy * (j - 0.5));
+    for (int i = 0; i <= nx; i++) ZY[i] = make_knot(radii, nrho, g->dx * (i - 0.5), 0);
+    int nq = asym ? 2 : 4;
+    int64_t compsize = (int64_t)4 * nx * ny * npairs;
+    for (int t = 0; t < nq; t++) memset(G[t], 0, sizeof(xo_c) * compsize);
+    const int CH = 32;
+    int nchunks = (npairs + CH - 1) / CH;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ch = 0; ch < nchunks; ch++) {
+        ictx c;
+        memset(&c, 0, sizeof(c));
+        c.asym = asym; c.npairs = npairs; c.p0 = ch * CH;
+        c.pc = (c.p0 + CH <= npairs) ? CH : npairs - c.p0;
+        c.nseg = nseg; c.segA = segA; c.segB = segB; c.ncomp = ncomp;
+        c.radii = radii; c.nr = nrho; c.ny2 = 2 * ny;
+        for (int t = 0; t < nq; t++) { c.q[t] = calloc(CH, sizeof(xo_c)); c.G[t] = G[t]; }
+        run_chunk(&c, g, K, ZX, ZY);
+        for (int t = 0; t < nq; t++) free(c.q[t]);
+    }
+    free(K); free(ZX); free(ZY); free(segA); free(segB);
+    /* mirror with parities; q-order asym {xz, yz}, symm {xx, yy, zz, xy} */
+    if (asym) {
+        mirror(G[0], nx, ny, npairs, -1, 1);   /* XZ */
+        mirror(G[1], nx, ny, npairs, 1, -1);   /* YZ */
+    } else {
+        mirror(G[0], nx, ny, npairs, 1, 1);    /* XX */
+        mirror(G[1], nx, ny, npairs, 1, 1);    /* YY */
+        mirror(G[2], nx, ny, npairs, 1, 1);    /* ZZ */
+        mirror(G[3], nx, ny, npairs, -1, -1);  /* XY */
+    }
+    if (fft)
+        for (int t = 0; t < nq; t++) xo_fft2_batch(G[t], 2 * nx, 2 * ny, npairs, -1);
+    return 0;
+}
+
+int xo_green_atoa(const xo_section *s, const xo_grid *g, double omega, int number_of_hankels,
+                  const double *hank0, const double *hank1, int fft,
+                  xo_c *xx, xo_c *xy, xo_c *xz, xo_c *yy, xo_c *yz, xo_c *zz)
+{
+    xo_plan plan;
+    int rc = xo_plan_create(g, number_of_hankels, &plan);
+    if (rc) return rc;
+    xo_c *ga[2] = {xz, yz};
+    xo_c *gs[4] = {xx, yy, zz, xy};
+    rc = tensor_group(s, g, omega, hank0, hank1, &plan, 1, fft, ga);
+    if (!rc) rc = tensor_group(s, g, omega, hank0, hank1, &plan, 0, fft, gs);
+    xo_plan_free(&plan);
+    return rc;
+}

@@ -1,0 +1,94 @@
+"""GPU FGMRES (K1-K13) against the oracle's restatement of Extreme.Fgmres on the same operator."""
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import xtreme as X
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_SOLUTION = 1e-9   # north_star: FP64 GPU path within 1e-9 relative of the FP64 oracle
+
+
+def _rhs(oracle, m, w, G, pol):
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    oop = oracle.Operator(m, G)
+    e0 = oracle.normal_field(m, w)
+    js = oracle.jscattered(m, pol, z, z0, e0)
+    oop.prepare("Chi0", z, z0)
+    chi0 = oop.apply(js)
+    oop.prepare("A", z, z0)
+    return oop, chi0
+
+
+@pytest.mark.parametrize("m_outer,m_inner,max_repeat", [(30, 10, 100), (4, 3, 50), (2, 2, 3)])
+def test_fgmres_matches_oracle_c1(oracle, torch_cuda, m_outer, m_inner, max_repeat):
+    m = cm.config_c1()
+    w = cm.omega_of(m.frequencies[0])
+    G = oracle.green_atoa(m, w)
+    oop, chi0 = _rhs(oracle, m, w, G, 0)
+    x_ref, info_ref, hist_ref = oracle.fgmres(oop.raw_apply, chi0, chi0, m_outer, m_inner, max_repeat, 1e-8)
+    s = X.ForwardSettings(residual=1e-8, inner_buffer_length=m_inner, outer_buffer_length=m_outer,
+                          max_repeats_number=max_repeat)
+    op = X.ConvolutionOperator(m, w, s)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    solver = X.AnomalyCurrentFgmresSolver(op, s)
+    x = chi0.copy()
+    info = solver.solve(chi0, x)
+    print(f"m={m_outer}/{m_inner}/{max_repeat}: gpu it {info.number_of_iterations} mv {info.matvecs} "
+          f"bea {info.arnoldi_backward_error:.3e} be {info.backward_error:.3e}; oracle it {info_ref['iterations']} "
+          f"mv {info_ref['matvecs']}; x rel {rel(x, x_ref):.3e}", flush=True)
+    assert info.number_of_iterations == info_ref["iterations"]
+    assert info.matvecs == info_ref["matvecs"]
+    assert info.convergence_achieved == info_ref["converged"]
+    beas = [h[2] for h in solver.history]
+    beas_ref = [h[2] for h in hist_ref if h[0] == 1]
+    assert len(beas) == len(beas_ref)
+    for a, b in zip(beas, beas_ref):
+        assert abs(a - b) <= 1e-9 * max(b, 1e-12) + 1e-13
+    assert rel(x, x_ref) < TOL_SOLUTION
+
+
+def test_batch_solve_matches_single(oracle, torch_cuda):
+    torch = torch_cuda
+    m = cm.config_c1()
+    w = cm.omega_of(m.frequencies[0])
+    G = oracle.green_atoa(m, w)
+    s = X.ForwardSettings()
+    op = X.ConvolutionOperator(m, w, s, max_rhs=2)
+    op.set_green_tensor(G)
+    rhs = [_rhs(oracle, m, w, G, pol)[1] for pol in (0, 1)]
+    op.prepare_operator("A", m.zeta(w))
+    solver = X.AnomalyCurrentFgmresSolver(op, s)
+    singles = []
+    for b in rhs:
+        x = b.copy()
+        info = solver.solve(b, x)
+        singles.append((x, info))
+    bs = [torch.from_numpy(b).to("cuda") for b in rhs]
+    xs = [b.clone() for b in bs]
+    infos = solver.solve_batch(bs, xs)
+    for (x1, i1), xb, ib in zip(singles, xs, infos):
+        print(f"batch: it {ib.number_of_iterations} vs {i1.number_of_iterations}, rel {rel(xb.cpu().numpy(), x1):.3e}",
+              flush=True)
+        assert ib.number_of_iterations == i1.number_of_iterations
+        assert ib.matvecs == i1.matvecs
+        assert rel(xb.cpu().numpy(), x1) < 1e-12
+
+
+def test_zero_rhs_returns_immediately(oracle, torch_cuda):
+    """bn == 0 -> ResultInfo.Converged(0, 0, 0) without touching x (Gmres.cs:80-83)."""
+    m = cm.small_model(4, 4, 2)
+    w = cm.omega_of(1.0)
+    G = oracle.green_atoa(m, w)
+    op = X.ConvolutionOperator(m, w)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    solver = X.AnomalyCurrentFgmresSolver(op)
+    b = np.zeros(m.n, dtype=np.complex128)
+    x = np.ones(m.n, dtype=np.complex128)
+    info = solver.solve(b, x)
+    assert info.convergence_achieved and info.number_of_iterations == 0 and info.matvecs == 0
+    assert np.all(x == 1)

@@ -1,0 +1,82 @@
+"""GPU parity of the operator apply (A9) against the FP64 oracle, on oracle-built Green tensors."""
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import xtreme as X
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12   # FP64 apply vs FP64 oracle, relative 2-norm
+TOL_FP32 = 2e-6    # complex64 Green tensor and FFT
+
+
+def _setup(oracle, model, precision="fp64", max_rhs=2):
+    w = cm.omega_of(model.frequencies[0])
+    G = oracle.green_atoa(model, w)
+    op = X.ConvolutionOperator(model, w, precision=precision, max_rhs=max_rhs)
+    op.set_green_tensor(G)
+    oop = oracle.Operator(model, G)
+    return w, G, op, oop
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 2), (8, 12, 3), (16, 8, 5)])
+@pytest.mark.parametrize("kind", ["A", "Chi0"])
+def test_apply_small_grids(oracle, torch_cuda, shape, kind):
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nx * 100 + ny * 10 + nz)
+    w, G, op, oop = _setup(oracle, m)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator(kind, z)
+    oop.prepare(kind, z, z0)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    y = op.apply(x)
+    print(f"apply {kind} {shape}: rel {rel(y, oop.apply(x)):.3e}", flush=True)
+    assert rel(y, oop.apply(x)) < TOL_FP64
+
+
+def test_apply_c1_host_and_device(oracle, torch_cuda):
+    torch = torch_cuda
+    m = cm.config_c1()
+    w, G, op, oop = _setup(oracle, m)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    rng = np.random.default_rng(3)
+    xs = rng.standard_normal((2, m.n)) + 1j * rng.standard_normal((2, m.n))
+    for kind in ("A", "Chi0"):
+        op.prepare_operator(kind, z)
+        oop.prepare(kind, z, z0)
+        ref = np.stack([oop.apply(xs[0]), oop.apply(xs[1])])
+        yh = op.apply(xs)                                         # host, nrhs = 2
+        xd = torch.from_numpy(xs).to("cuda")
+        yd = op.apply(xd).cpu().numpy()                           # device, nrhs = 2
+        y1 = op.apply(xs[1])                                      # nrhs = 1
+        print(f"C1 {kind}: host {rel(yh, ref):.3e} device {rel(yd, ref):.3e}", flush=True)
+        assert rel(yh, ref) < TOL_FP64
+        assert rel(yd, ref) < TOL_FP64
+        assert rel(y1, ref[1]) < TOL_FP64
+
+
+def test_identity_when_zeta_equals_background(oracle, torch_cuda):
+    """zeta == zeta0 makes R = 0, so A = I exactly and Chi0 = 0 (ConvolutionOperator.cs:138-141)."""
+    m = cm.config_c1()
+    w, G, op, oop = _setup(oracle, m)
+    zb = np.repeat(m.zeta0(w)[None, :], m.nx * m.ny, axis=0).reshape(-1)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    op.prepare_operator("A", zb)
+    assert np.array_equal(op.apply(x), x)
+
+
+def test_apply_fp32_error_bound(oracle, torch_cuda):
+    m = cm.config_c1()
+    w, G, op, oop = _setup(oracle, m, precision="fp32")
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator("A", z)
+    oop.prepare("A", z, z0)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    e = rel(op.apply(x), oop.apply(x))
+    print(f"C1 fp32 apply rel error {e:.3e}", flush=True)
+    assert e < TOL_FP32
