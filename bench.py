@@ -1,0 +1,188 @@
+"""Benchmark: FGMRES solve time per frequency (both polarisations) to a 1e-8 residual, and the
+operator-apply HBM bandwidth, on one MI355X.
+
+Workload (default): the 128x128x32 model with 20 seeded random boxes over the 3-layer background
+(dx = dy = 200 m, dz 50 -> 400 m), its 8 frequencies 1e-3 .. 10 Hz, FP64 (complex128) throughout,
+FGMRES(30) with the inner GMRES(10) right preconditioner, tolerance 1e-8 (the reference's stopping
+test on the Arnoldi residual; the true residual is reported beside it).
+
+A step is the solve of one frequency: chi <- chi0 for both polarisations and the side-by-side
+FGMRES solve of A chi = chi0 (the reference's SolveCie span, pol X + pol Y). Green tensors and
+chi0 of every frequency are built on the GPU before the timed region (the reference accounts
+the Green tensor separately, as AtoA_Green). Step s solves frequency s mod 8.
+
+The JSON line also carries the live roofline of the dominant kernel (the Green-tensor
+contraction, timed with HIP events on its own stream over the timed region) and a CPU baseline:
+the project's FP64 C restatement (the oracle) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-applies", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(frequencies shard across GPUs with no exchange, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from extreme_amd import cartesian_model as cm
+    from extreme_amd import forward as F
+    from extreme_amd import xtreme as X
+
+    model = cm.config_c2() if args.config == "c2" else cm.config_c1()
+    freqs = model.frequencies
+    settings = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30,
+                                 max_repeats_number=100, number_of_hankels=10)
+    nsolve = min(len(freqs), max(args.steps, 1))
+    t0 = time.perf_counter()
+    problems = [F.FrequencySolve(model, freqs[i], settings) for i in range(nsolve)]
+    setup_s = time.perf_counter() - t0
+    green_s = [p.green_seconds for p in problems]
+    print(f"# setup {setup_s:.1f}s (Green tensors {sum(green_s):.1f}s for {nsolve} frequencies)", file=sys.stderr,
+          flush=True)
+
+    for w in range(args.warmup):
+        infos = problems[w % nsolve].solve()
+        print(f"# warmup {w}: f={problems[w % nsolve].frequency:.4g} its {[i.number_of_iterations for i in infos]}",
+              file=sys.stderr, flush=True)
+
+    for p in problems:
+        p.op.set_profiling(True)
+    results = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        p = problems[s % nsolve]
+        infos = p.solve()
+        results.append((p, infos))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+
+    # ---- per-stage device time and algorithmic bytes (SURVEY 8(d) formula) -----------------
+    m = model
+    nz, L = m.nz, 4 * m.nx * m.ny
+    gblk = 4 * (nz * (nz + 1) // 2) + 2 * nz * nz
+    n = m.n
+    N = m.nx * m.ny * nz
+    M = L * 3 * nz
+    SG = SV = SR = 16
+    stage_ms = {k: 0.0 for k in X.ConvolutionOperator.STAGES}
+    rounds = 0
+    rhs_applies = 0
+    for p in problems:
+        st, cnt = p.op.stage_times()
+        for k in st:
+            stage_ms[k] += st[k]
+        rounds += cnt
+        p.op.set_profiling(False)
+    for p, infos in results:
+        rhs_applies += sum(i.matvecs for i in infos)
+    green_bytes = SG * L * gblk
+    contract_bytes = rounds * green_bytes + rhs_applies * 2 * M * SV
+    apply_alg_bytes = rounds * green_bytes + rhs_applies * 2 * n * SV + rounds * N * SR
+    apply_ms = sum(stage_ms.values())
+    contract_avg_ms = stage_ms["contract"] / max(rounds, 1)
+    achieved_gbs = contract_bytes / (stage_ms["contract"] * 1e-3) / 1e9
+    apply_gbs = apply_alg_bytes / (apply_ms * 1e-3) / 1e9
+
+    iters = [[i.number_of_iterations for i in infos] for _, infos in results]
+    matvecs = [[i.matvecs for i in infos] for _, infos in results]
+    be = [[i.backward_error for i in infos] for _, infos in results]
+    bea = [[i.arnoldi_backward_error for i in infos] for _, infos in results]
+    conv = all(i.convergence_achieved for _, infos in results for i in infos)
+
+    # ---- CPU baseline: the oracle's FP64 operator apply on this host ----------------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(problems[0], args.cpu_applies, sum(sum(mv) for mv in matvecs) / args.steps)
+
+    line = {
+        "metric": METRIC,
+        "value": elapsed / args.steps,
+        "unit": "s/freq",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded random-box model, SplitMix64)",
+        "config": {"workload": f"{m.name}: {m.nx}x{m.ny}x{m.nz} cells, 3-layer background, 20 boxes, "
+                               f"{nsolve} of {len(freqs)} frequencies, both polarisations, FGMRES(30)/GMRES(10), "
+                               "tol 1e-8", "nx": m.nx, "ny": m.ny, "nz": m.nz,
+                   "frequencies_hz": [round(p.frequency, 6) for p in problems], "precision": args.precision},
+        "solves_per_s": 2 * args.steps / elapsed,
+        "apply_gbs": apply_gbs,
+        "apply_frac_of_hbm_peak": apply_gbs / HBM_PEAK_GBS,
+        "stage_ms_per_apply": {k: v / max(rounds, 1) for k, v in stage_ms.items()},
+        "applies": rounds,
+        "rhs_applies": rhs_applies,
+        "iterations": iters,
+        "matvecs": matvecs,
+        "arnoldi_backward_error": bea,
+        "true_backward_error": be,
+        "converged": conv,
+        "green_seconds_per_freq": float(np.mean(green_s)),
+        "roofline": {"kernel": "contract_kernel<double>", "bound": "hbm", "achieved": achieved_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": contract_avg_ms,
+                     "algorithmic_bytes_per_launch": contract_bytes / max(rounds, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(problem, applies, mean_rhs_applies_per_freq):
+    """Times the oracle's FP64 operator apply (the FFT convolution + packed-block contraction the
+    reference runs per matvec) on the host cores, on a bounded sample of `applies` applies, and
+    scales to seconds per frequency with the GPU run's operator-apply count."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    m = problem.model
+    G = problem.op.green_tensor()
+    oop = O.Operator(m, G)
+    oop.prepare("A", m.zeta(problem.omega).reshape(-1), m.zeta0(problem.omega))
+    x = problem.chi0[0].cpu().numpy()
+    oop.apply(x)  # first touch
+    t0 = time.perf_counter()
+    for _ in range(applies):
+        oop.apply(x)
+    per_apply = (time.perf_counter() - t0) / applies
+    return {"value": per_apply * mean_rhs_applies_per_freq, "unit": "s/freq", "cores": threads,
+            "kind": "port",
+            "sample": f"{applies} FP64 operator applies of the oracle (C restatement, OpenMP, self-written FFT) "
+                      f"at {m.nx}x{m.ny}x{m.nz}, {per_apply:.3f} s each, times the GPU run's "
+                      f"{mean_rhs_applies_per_freq:.0f} operator applies per frequency (both pols); "
+                      "Krylov vector work not included"}
+
+
+if __name__ == "__main__":
+    main()

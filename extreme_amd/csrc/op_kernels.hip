@@ -237,7 +237,10 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
                            (const cplx<T> *)tw);
     };
     switch (P) {
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 6: go(std::integral_constant<int, 6>{}); break;
     case 8: go(std::integral_constant<int, 8>{}); break;
+    case 12: go(std::integral_constant<int, 12>{}); break;
     case 16: go(std::integral_constant<int, 16>{}); break;
     case 24: go(std::integral_constant<int, 24>{}); break;
     case 32: go(std::integral_constant<int, 32>{}); break;
@@ -257,7 +260,8 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
 bool fft_length_supported(int p)
 {
     switch (p) {
-    case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512:
+    case 4: case 6: case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
+    case 256: case 384: case 512:
         return true;
     default:
         return false;

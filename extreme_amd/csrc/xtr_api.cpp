@@ -12,6 +12,7 @@
 #include "xtr_common.h"
 #include "../../include/xtreme_hip.h"
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -114,16 +115,28 @@ struct xtr_op {
     DevBuf<unsigned char> buf;    // max_rhs * M spectrum
     DevBuf<double2> stage_x, stage_y;   // host-pointer apply staging
 
+    // Per-stage timing: one set of six events per apply while profiling is on, summed on demand.
     bool profiling = false;
-    hipEvent_t ev[6] = {};
-    double stage_ms[5] = {0, 0, 0, 0, 0};
+    std::vector<std::array<hipEvent_t, 6>> ev_sets;
+    size_t ev_used = 0;
     std::mutex apply_mu;
 
     ~xtr_op()
     {
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto &set : ev_sets)
+            for (auto &e : set)
+                if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t *next_events()
+    {
+        if (ev_used == ev_sets.size()) {
+            std::array<hipEvent_t, 6> set{};
+            for (auto &e : set) XTR_CHECK(hipEventCreate(&e));
+            ev_sets.push_back(set);
+        }
+        return ev_sets[ev_used++].data();
     }
 
     size_t elem() const { return precision == XTR_FP64 ? sizeof(double2) : sizeof(float2); }
@@ -149,17 +162,18 @@ struct xtr_op {
         a.tw_x = precision == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
         a.tw_y = precision == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
         a.v = v;
-        if (profiling) XTR_CHECK(hipEventRecord(ev[0], s));
+        hipEvent_t *ev = profiling ? next_events() : nullptr;
+        if (ev) XTR_CHECK(hipEventRecord(ev[0], s));
         xtr::launch_fwd_y(a, s);
-        if (profiling) XTR_CHECK(hipEventRecord(ev[1], s));
+        if (ev) XTR_CHECK(hipEventRecord(ev[1], s));
         xtr::launch_fwd_x(a, s);
-        if (profiling) XTR_CHECK(hipEventRecord(ev[2], s));
+        if (ev) XTR_CHECK(hipEventRecord(ev[2], s));
         xtr::launch_contract(a, s);
-        if (profiling) XTR_CHECK(hipEventRecord(ev[3], s));
+        if (ev) XTR_CHECK(hipEventRecord(ev[3], s));
         xtr::launch_inv_x(a, s);
-        if (profiling) XTR_CHECK(hipEventRecord(ev[4], s));
+        if (ev) XTR_CHECK(hipEventRecord(ev[4], s));
         xtr::launch_inv_y(a, s);
-        if (profiling) XTR_CHECK(hipEventRecord(ev[5], s));
+        if (ev) XTR_CHECK(hipEventRecord(ev[5], s));
     }
 };
 
@@ -565,7 +579,7 @@ int xtr_op_create(const xtr_op_desc *d, xtr_op **out)
         require(d->nx > 0 && d->ny > 0 && d->nz > 0, "grid sizes must be positive");
         require(3 * d->nz * 1 <= 1024, "Nz too large for the contraction kernel");
         require(xtr::fft_length_supported(2 * d->nx) && xtr::fft_length_supported(2 * d->ny),
-                "2*Nx and 2*Ny must be one of 8,16,24,32,48,64,96,128,192,256,384,512");
+                "2*Nx and 2*Ny must be one of 4,6,8,12,16,24,32,48,64,96,128,192,256,384,512");
         require(d->precision == XTR_FP64 || d->precision == XTR_FP32, "precision must be XTR_FP64 or XTR_FP32");
         require(d->max_rhs >= 1 && d->max_rhs <= xtr::kMaxRhs, "max_rhs must be 1..8");
         require(3 * d->nz * d->max_rhs <= 1024, "3*Nz*max_rhs must not exceed 1024");
@@ -589,7 +603,6 @@ int xtr_op_create(const xtr_op_desc *d, xtr_op **out)
         }
         for (int k = 0; k < d->nz; ++k) background_layer(op->section, op->depth[k], op->depth[k] + op->thick[k]);
         XTR_CHECK(hipStreamCreateWithFlags(&op->stream, hipStreamDefault));
-        for (auto &e : op->ev) XTR_CHECK(hipEventCreate(&e));
         const xtr::Geom &g = op->g;
         // packed (rc, tr) table in CombineSymm order (TranceiverUtils.cs:65-74)
         std::vector<uint16_t> prc(2 * g.S());
@@ -795,21 +808,28 @@ int xtr_op_set_profiling(xtr_op *op, int enable)
 {
     return guarded([&] {
         require(op, "null argument");
+        std::lock_guard<std::mutex> lk(op->apply_mu);
         op->profiling = enable != 0;
+        op->ev_used = 0;
     });
 }
 
-int xtr_op_stage_times(xtr_op *op, double *out5)
+int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies)
 {
     return guarded([&] {
         require(op && out5, "null argument");
-        require(op->profiling, "profiling is off");
-        XTR_CHECK(hipEventSynchronize(op->ev[5]));
-        for (int i = 0; i < 5; ++i) {
-            float ms = 0;
-            XTR_CHECK(hipEventElapsedTime(&ms, op->ev[i], op->ev[i + 1]));
-            out5[i] = ms;
+        std::lock_guard<std::mutex> lk(op->apply_mu);
+        for (int i = 0; i < 5; ++i) out5[i] = 0;
+        for (size_t k = 0; k < op->ev_used; ++k) {
+            auto &ev = op->ev_sets[k];
+            XTR_CHECK(hipEventSynchronize(ev[5]));
+            for (int i = 0; i < 5; ++i) {
+                float ms = 0;
+                XTR_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                out5[i] += ms;
+            }
         }
+        if (applies) *applies = (int64_t)op->ev_used;
     });
 }
 

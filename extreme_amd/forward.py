@@ -1,0 +1,150 @@
+"""Host side of the MT forward solve around the GPU operator: the caller logic the reference's C# keeps.
+
+Restates, in host FP64 (a few values per layer or per cell, not the hot path):
+  PlaneWaveCalculator.CalculateFieldE/H  (Extreme.Cartesian.Magnetotellurics/PlaneWaveCalculator.cs:11-83)
+  Mt3DForwardSolver.CalculateNormalFieldFromSource  (Solver/Mt3DForwardSolver.cs:251-275)
+  ForwardSolver.CalculateJScattered / CalculateEScatteredFrom / CalculateJqFrom
+                                          (Extreme.Cartesian/Forward/ForwardSolver.cs:186-449)
+and drives the GPU for CalculateChi0From and SolveEquationFor (ForwardSolver.cs:253-307) with both
+polarisations solved side by side (Mt3DForwardSolver.cs:95-129).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import cartesian_model as cm
+from . import xtreme as X
+
+
+def _pw_container(section: cm.Section1D, omega: float, rc: int, tr: int):
+    """eta, exp, P1, Q1 and A at lambda = 0 (PlanCalculator.cs:195-314)."""
+    nl = section.nlayers
+    zeta = section.zeta(omega)
+    eta = np.array([np.sqrt(complex(0.0 - (1j * omega * zeta[k] * cm.MU0).real,
+                                    0.0 - (1j * omega * zeta[k] * cm.MU0).imag)) for k in range(nl)])
+    ex = np.exp(-2 * eta * np.asarray(section.thickness, dtype=float))
+    q = np.zeros(nl, dtype=complex)
+    n = nl - 1
+    part = eta[n] / eta[n - 1]
+    q[n - 1] = (1 - part) / (1 + part)
+    for k in range(n - 1, 1, -1):
+        qe = q[k] * ex[k]
+        part = (eta[k] / eta[k - 1]) * ((qe - 1) / (qe + 1))
+        q[k - 1] = (1 + part) / (1 - part)
+    p = np.zeros(nl, dtype=complex)
+    p[1] = (eta[1] - eta[0]) / (eta[1] + eta[0])
+    for k in range(1, nl - 1):
+        pe = p[k] * ex[k]
+        part = (eta[k] / eta[k + 1]) * ((pe - 1) / (pe + 1))
+        p[k + 1] = (1 + part) / (1 - part)
+    a = np.zeros((nl, nl), dtype=complex)
+    s, r = max(rc, tr), min(rc, tr)
+    for nn in range(s, r - 1, -1):
+        prev = 1.0 / (eta[nn] * (1 - p[nn] * q[nn] * ex[nn]))
+        a[nn, nn] = prev
+        for k in range(nn - 1, r - 1, -1):
+            dk1 = section.layer_depth(k + 1)
+            qq = ((1 + p[k + 1]) / (1 + p[k] * ex[k])) * np.exp((eta[k + 1] - eta[k]) * dk1)
+            prev = qq * prev
+            a[k, nn] = prev
+            a[nn, k] = prev
+    return eta, q, a
+
+
+def plane_wave_e(section: cm.Section1D, omega: float, depth: float, thick: float = 0.0) -> complex:
+    """PlaneWaveCalculator.CalculateFieldE (:11-40): normal E at depth (averaged over thick)."""
+    tr = section.layer_at_depth(0.0)
+    rc = section.layer_at_depth(depth)
+    eta, q, a = _pw_container(section, omega, rc, tr)
+    zeta = section.zeta(omega)
+    ikr = 1j * np.sqrt(1j * omega * cm.MU0 * zeta[rc])
+    ik1 = 1j * np.sqrt(1j * omega * cm.MU0 * zeta[tr])
+    dr1 = section.layer_depth(rc + 1)
+    d2 = section.layer_depth(tr + 1)
+    aa = a[rc, tr] / a[tr, tr]
+    if thick == 0:
+        upper = np.exp(ikr * depth) + q[rc] * np.exp(ikr * (2 * dr1 - depth))
+    else:
+        z1, z2 = depth - thick / 2, depth + thick / 2
+        u1 = np.exp(ikr * z1) - q[rc] * np.exp(ikr * (2 * dr1 - z1))
+        u2 = np.exp(ikr * z2) - q[rc] * np.exp(ikr * (2 * dr1 - z2))
+        upper = (u2 - u1) / (z2 - z1) / ikr
+    lower = 1 + q[tr] * np.exp(ik1 * (2 * d2))
+    return complex(aa * (upper / lower))
+
+
+def plane_wave_h(section: cm.Section1D, omega: float, depth: float) -> complex:
+    """PlaneWaveCalculator.CalculateFieldH (:42-64)."""
+    tr = section.layer_at_depth(0.0)
+    rc = section.layer_at_depth(depth)
+    eta, q, a = _pw_container(section, omega, rc, tr)
+    zeta = section.zeta(omega)
+    kkr = 1j * omega * cm.MU0 * zeta[rc]
+    ikr = 1j * np.sqrt(kkr)
+    ik1 = 1j * np.sqrt(1j * omega * cm.MU0 * zeta[tr])
+    dr1 = section.layer_depth(rc + 1)
+    d2 = section.layer_depth(tr + 1)
+    aa = a[rc, tr] / a[tr, tr]
+    upper = np.exp(ikr * depth) - q[rc] * np.exp(ikr * (2 * dr1 - depth))
+    lower = 1 + q[tr] * np.exp(ik1 * (2 * d2))
+    return complex((np.sqrt(kkr) / (omega * cm.MU0)) * aa * (upper / lower))
+
+
+def normal_field(model: cm.CartesianModel, omega: float) -> np.ndarray:
+    """E0 per anomaly layer, averaged over the layer (Mt3DForwardSolver.cs:271-275)."""
+    return np.array([plane_wave_e(model.section, omega, float(d + t / 2), float(t))
+                     for d, t in zip(model.depth, model.thick)], dtype=np.complex128)
+
+
+def scattered_current(model: cm.CartesianModel, omega: float, pol: int, e0: np.ndarray) -> np.ndarray:
+    """j_s = (zeta - zeta0) E0 for polarisation pol (0 = X, 1 = Y), [Nx][Ny][3][Nz] (ForwardSolver.cs:225-237)."""
+    nx, ny, nz = model.nx, model.ny, model.nz
+    js = np.zeros((nx, ny, 3, nz), dtype=np.complex128)
+    js[:, :, pol, :] = (model.zeta(omega) - model.zeta0(omega)[None, None, :]) * e0[None, None, :]
+    return js.reshape(-1)
+
+
+def scattered_field_and_jq(model: cm.CartesianModel, omega: float, chi: np.ndarray, js: np.ndarray):
+    """E_s and jQ (ForwardSolver.cs:391-406, 424-436)."""
+    nx, ny, nz = model.nx, model.ny, model.nz
+    z = model.zeta(omega).reshape(nx, ny, 1, nz)
+    z0 = model.zeta0(omega).reshape(1, 1, 1, nz)
+    chi = np.asarray(chi).reshape(nx, ny, 3, nz)
+    js = np.asarray(js).reshape(nx, ny, 3, nz)
+    es = (2 * np.sqrt(z0.real + 0j) * chi - js) / (z + np.conj(z0))
+    jq = es * (z - z0) + js
+    return es.reshape(-1), jq.reshape(-1)
+
+
+class FrequencySolve:
+    """One frequency of Mt3DForwardSolver.SolvePrivate on the GPU: Green tensor, chi0 for both
+    polarisations, and the side-by-side FGMRES solve of A chi = chi0."""
+
+    def __init__(self, model: cm.CartesianModel, frequency: float, settings: X.ForwardSettings | None = None,
+                 precision: str = "fp64"):
+        import torch
+        self.torch = torch
+        self.model = model
+        self.frequency = frequency
+        self.omega = cm.omega_of(frequency)
+        self.settings = settings or X.ForwardSettings()
+        self.op = X.ConvolutionOperator(model, self.omega, self.settings, precision=precision, max_rhs=2)
+        t0 = time.perf_counter()
+        self.op.calculate_green_tensor()
+        self.green_seconds = time.perf_counter() - t0
+        e0 = normal_field(model, self.omega)
+        self.js = [scattered_current(model, self.omega, pol, e0) for pol in (0, 1)]
+        self.op.prepare_operator("Chi0")
+        js_dev = torch.from_numpy(np.stack(self.js)).to("cuda")
+        self.chi0 = self.op.apply(js_dev)                       # CalculateChi0From, both pols in one apply
+        self.op.prepare_operator("A")
+        self.solver = X.AnomalyCurrentFgmresSolver(self.op, self.settings)
+        self.chi = self.chi0.clone()
+
+    def solve(self):
+        """SolveEquationFor for pol X and pol Y: chi <- chi0, then FGMRES (ForwardSolver.cs:300-307)."""
+        self.chi.copy_(self.chi0)
+        self.solver.history.clear()
+        return self.solver.solve_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]])

@@ -78,7 +78,7 @@ def lib():
                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                     C.POINTER(_Info), ITER_FN, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
-        L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
         _lib = L
@@ -226,12 +226,17 @@ class ConvolutionOperator:
         return y
 
     def set_profiling(self, on: bool):
+        """Starts (and resets) or stops per-stage HIP-event timing of every apply."""
         _check(lib().xtr_op_set_profiling(self._h, 1 if on else 0))
 
+    STAGES = ("fwd_y", "fwd_x", "contract", "inv_x", "inv_y")
+
     def stage_times(self):
+        """(summed milliseconds per stage, number of applies) since profiling was switched on."""
         out = (C.c_double * 5)()
-        _check(lib().xtr_op_stage_times(self._h, out))
-        return list(out)
+        n = C.c_int64()
+        _check(lib().xtr_op_stage_times(self._h, out, C.byref(n)))
+        return dict(zip(self.STAGES, list(out))), n.value
 
     def sizes(self):
         n, gb, bb = C.c_int64(), C.c_int64(), C.c_int64()

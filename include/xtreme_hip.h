@@ -143,12 +143,13 @@ int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, in
                                   const void *const *b, void *const *x, xtr_result_info *info,
                                   xtr_iteration_fn on_iteration, void *user);
 
-/* Device time of the last xtr_op_apply* call split by stage, in milliseconds:
- * out[0] forward pass along y (with R and f fused), out[1] forward pass along x,
+/* Device time of the operator applies since xtr_op_set_profiling(op, 1), summed by stage, in
+ * milliseconds: out[0] forward pass along y (with R and f fused), out[1] forward pass along x,
  * out[2] contraction, out[3] inverse pass along x, out[4] inverse pass along y (with b and the
- * last step fused). Timing is recorded only after xtr_op_set_profiling(op, 1). */
+ * last step fused); *applies = number of applies summed. Recorded with HIP events on the stream
+ * the kernels run on (the reference's OperatorA* profiler events, ProfilerEvent.cs:40-49). */
 int xtr_op_set_profiling(xtr_op *op, int enable);
-int xtr_op_stage_times(xtr_op *op, double *out5);
+int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
 
 /* Sizes: n = 3 Nx Ny Nz, the Green tensor's device bytes, the spectrum buffer bytes. */
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes);

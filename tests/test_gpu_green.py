@@ -1,0 +1,59 @@
+"""Device assembly of the AtoA Green tensor (G1-G13) against the FP64 oracle's restatement."""
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import xtreme as X
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_GREEN = 1e-10   # per component, relative 2-norm, FP64 device vs FP64 oracle
+
+
+def _three_layer_small(nx, ny, nz):
+    """A small grid straddling a background interface (layers 0-2 km and 2-10 km)."""
+    dz = 4000.0 / nz
+    depth = np.arange(nz) * dz
+    thick = np.full(nz, dz)
+    sec = cm.Section1D(0.0, [0.0, 2000.0, 8000.0, 0.0], [0.0, 1e-2, 1e-1, 1e-3])
+    sigma = np.full((nx, ny, nz), 0.05)
+    return cm.CartesianModel(nx, ny, 300.0, 250.0, depth, thick, sigma, sec, "three-layer", [0.3])
+
+
+@pytest.mark.parametrize("which", ["small", "three-layer", "c1"])
+def test_green_matches_oracle(oracle, torch_cuda, which):
+    if which == "small":
+        m = cm.small_model(4, 4, 3, dx=150.0, dz=80.0)
+    elif which == "three-layer":
+        m = _three_layer_small(8, 6, 4)
+    else:
+        m = cm.config_c1()
+    w = cm.omega_of(m.frequencies[0])
+    ref = oracle.green_atoa(m, w)
+    op = X.ConvolutionOperator(m, w)
+    op.calculate_green_tensor()
+    got = op.green_tensor()
+    for c in X.ConvolutionOperator.COMPONENTS:
+        e = rel(got[c], ref[c])
+        print(f"{which} {c}: rel {e:.3e}", flush=True)
+        assert e < TOL_GREEN, (c, e)
+
+
+def test_green_spatial_symmetries(oracle):
+    """The mirrored spatial tensor has the parities of GreenTensorFftPopulator.cs:41-51 and zero
+    row Nx / column Ny (CPU check of the oracle's structure)."""
+    m = cm.small_model(4, 6, 2)
+    w = cm.omega_of(1.0)
+    g = oracle.green_atoa(m, w, fft=False)
+    nx2, ny2 = 2 * m.nx, 2 * m.ny
+    par = {"xx": (1, 1), "yy": (1, 1), "zz": (1, 1), "xy": (-1, -1), "xz": (-1, 1), "yz": (1, -1)}
+    for c, (xs, ys) in par.items():
+        a = g[c].reshape(nx2, ny2, -1)
+        assert np.all(a[m.nx, :, :] == 0) and np.all(a[:, m.ny, :] == 0)
+        for i in range(1, m.nx):
+            for j in range(0, m.ny):
+                assert np.array_equal(a[nx2 - i, j], xs * a[i, j])
+        for i in range(0, m.nx):
+            for j in range(1, m.ny):
+                assert np.array_equal(a[i, ny2 - j], ys * a[i, j])
