@@ -31,6 +31,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s"
 
+# HBM bytes per launch of the contraction kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one counter per run) and scaled by the same counters on a 1 GiB device copy
+# (FETCH x 2.0000, WRITE x 1.0000); summary in profiles/contract_kernel_c2_hbm_traffic.md.
+# Keyed by (workload, nz, right-hand sides per launch); a workload without an entry reports null.
+MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -41,7 +47,14 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (the known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -151,12 +164,28 @@ def main():
         "converged": conv,
         "green_seconds_per_freq": float(np.mean(green_s)),
         "roofline": {"kernel": "contract_kernel<double>", "bound": "hbm", "achieved": achieved_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
                      "avg_launch_ms": contract_avg_ms,
                      "algorithmic_bytes_per_launch": contract_bytes / max(rounds, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain elementwise copy of nbytes (read nbytes, write nbytes) per launch."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "ms_per_copy": dt * 1e3, "gbs_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
 
 
 def cpu_baseline(problem, applies, mean_rhs_applies_per_freq):
