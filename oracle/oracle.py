@@ -88,6 +88,10 @@ def lib():
                                      C.POINTER(Result)]
         L.xo_zrotg.argtypes = [CDouble, CDouble, C.POINTER(C.c_double), C.POINTER(CDouble)]
         L.xo_set_threads.argtypes = [C.c_int]
+        L.xo_log10_rho_step.restype = C.c_double
+        L.xo_hankel_lambdas.restype = C.c_int
+        L.xo_hankel_lambdas.argtypes = [C.c_double, C.c_int, f64p]
+        L.xo_hankel_filter.argtypes = [f64p, c128p, C.c_int, c128p]
         _lib = L
     return _lib
 
@@ -124,13 +128,19 @@ def make_grid(model, keep):
 def plane_wave_e(section, omega, depth, thick=0.0):
     k = _Keep()
     s = make_section(section, k)
-    return _cplx(lib().xo_plane_wave_e(C.byref(s), omega, depth, thick))
+    v = _cplx(lib().xo_plane_wave_e(C.byref(s), omega, depth, thick))
+    if np.isnan(v.real):
+        raise ValueError("depth is not within the 1-D section")
+    return v
 
 
 def plane_wave_h(section, omega, depth):
     k = _Keep()
     s = make_section(section, k)
-    return _cplx(lib().xo_plane_wave_h(C.byref(s), omega, depth))
+    v = _cplx(lib().xo_plane_wave_h(C.byref(s), omega, depth))
+    if np.isnan(v.real):
+        raise ValueError("depth is not within the 1-D section")
+    return v
 
 
 def fft2_batch(buf, n1, n2, nc, sign):
@@ -150,6 +160,26 @@ def hankel_table():
     path = os.path.join(os.path.dirname(_HERE), "extreme_amd", "data", "hankel_ndec40.txt")
     t = np.loadtxt(path)
     return np.ascontiguousarray(t[:, 0]), np.ascontiguousarray(t[:, 1])
+
+
+def hankel_rho_step():
+    """(float)10^(1/40) as the reference rounds it (HankelCoefficients.cs:67)."""
+    return lib().xo_log10_rho_step()
+
+
+def hankel_lambdas(rho0, nrho):
+    """lambda grid of the output grid rho[i] = rho0 * step^i (ScalarPlan.cs:76-97)."""
+    lam = np.zeros(431 + nrho)
+    n = lib().xo_hankel_lambdas(rho0, nrho, lam)
+    return lam[:n]
+
+
+def hankel_filter(hank, fk, nrho):
+    """out[i] = sum_j hank[430 - j] fk[i + j] (HankelCoefficients.ConvoluteWithFast :103-118)."""
+    fk = np.ascontiguousarray(fk, dtype=np.complex128)
+    out = np.zeros(nrho, dtype=np.complex128)
+    lib().xo_hankel_filter(np.ascontiguousarray(hank, dtype=np.float64), fk, nrho, out)
+    return out
 
 
 class _Plan(C.Structure):

@@ -141,6 +141,7 @@ xo_c xo_plane_wave_e(const xo_section *s, double omega, double depth, double thi
     double mu0 = xo_mu0();
     int tr = xo_layer_at_depth(s, 0.0);
     int rc = xo_layer_at_depth(s, depth);
+    if (rc < 0 || tr < 0) return NAN;   /* the reference throws: not within the 1-D section */
     pw_aux c;
     pw_prepare(s, omega, rc, tr, &c);
     int nl = c.nl;
@@ -172,6 +173,7 @@ xo_c xo_plane_wave_h(const xo_section *s, double omega, double depth)
     double mu0 = xo_mu0();
     int tr = xo_layer_at_depth(s, 0.0);
     int rc = xo_layer_at_depth(s, depth);
+    if (rc < 0 || tr < 0) return NAN;   /* the reference throws: not within the 1-D section */
     pw_aux c;
     pw_prepare(s, omega, rc, tr, &c);
     int nl = c.nl;

@@ -363,6 +363,24 @@ static xo_c hankel_dot(const double *hank, const xo_c *fk, int i)
     return re + I * im;
 }
 
+/* Test hooks for the filter alone: the lambda grid of an output grid rho[i] = rho0 * step^i
+ * (ScalarPlan.CalculateLambdasForLog10) and the filter sums on it (ConvoluteWithFast). */
+double xo_log10_rho_step(void) { return log10_rho_step(); }
+
+int xo_hankel_lambdas(double rho0, int nrho, double *lam)
+{
+    int nlam;
+    double *k = lambdas_for(&rho0, nrho, &nlam);
+    memcpy(lam, k, sizeof(double) * nlam);
+    free(k);
+    return nlam;
+}
+
+void xo_hankel_filter(const double *hank, const xo_c *fk, int nrho, xo_c *out)
+{
+    for (int i = 0; i < nrho; i++) out[i] = hankel_dot(hank, fk, i);
+}
+
 int xo_green_scalars(const xo_section *sec, const xo_grid *g, double omega,
                      const double *hank0, const double *hank1, const xo_plan *plan,
                      int asym, xo_c *out)

@@ -86,6 +86,15 @@ int xo_green_scalars(const xo_section *s, const xo_grid *g, double omega,
                      const double *hank0, const double *hank1, const xo_plan *plan,
                      int asym, xo_c *out);
 
+/* Filter alone (test hooks): xo_hankel_lambdas fills lam[0 .. 431 + nrho - 1] for the output
+ * grid rho[i] = rho0 * step^i, step = xo_log10_rho_step() (ScalarPlan.cs:76-97,
+ * HankelCoefficients.cs:67), and returns the count; xo_hankel_filter gives
+ * out[i] = sum_j hank[430 - j] fk[i + j] (HankelCoefficients.ConvoluteWithFast :103-118), so that
+ * out[i] / rho[i] approximates the integral of fk(lambda) J(lambda rho[i]) over lambda. */
+double xo_log10_rho_step(void);
+int xo_hankel_lambdas(double rho0, int nrho, double *lam);
+void xo_hankel_filter(const double *hank, const xo_c *fk, int nrho, xo_c *out);
+
 /* Whole AtoA tensor after mirroring and forward FFT, reference layout:
  * each component [2Nx][2Ny][P] with P = nz*nz (xz, yz) or nz(nz+1)/2 (xx, xy, yy, zz).
  * fft = 0 stops before the FFT (the mirrored spatial tensor). */
