@@ -129,6 +129,9 @@ def main():
     bea = [[i.arnoldi_backward_error for i in infos] for _, infos in results]
     conv = all(i.convergence_achieved for _, infos in results for i in infos)
 
+    # ---- the host-pointer boundary: one apply of both polarisations with PCIe copies ---------
+    pcie = host_boundary_apply(problems[0])
+
     # ---- CPU baseline: the oracle's FP64 operator apply on this host ----------------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -169,8 +172,34 @@ def main():
                      "avg_launch_ms": contract_avg_ms,
                      "algorithmic_bytes_per_launch": contract_bytes / max(rounds, 1)},
         "cpu_baseline": cpu,
+        "host_boundary": pcie,
     }
     print(json.dumps(line), flush=True)
+
+
+def host_boundary_apply(problem, reps=3):
+    """Apply of both polarisations through the host-pointer entry point (H2D, apply, D2H) beside
+    the same apply on device-resident vectors."""
+    import torch
+    op = problem.op
+    xd = problem.chi0
+    xh = xd.cpu().numpy()
+    yd = torch.empty_like(xd)
+    yh = np.empty_like(xh)
+    op.apply(xd, yd)
+    op.apply(xh, yh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.apply(xd, yd)
+    torch.cuda.synchronize()
+    dev = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.apply(xh, yh)
+    host = (time.perf_counter() - t0) / reps
+    return {"apply_device_ms": dev * 1e3, "apply_host_pointers_ms": host * 1e3,
+            "bytes_over_pcie_per_apply": int(2 * xh.nbytes)}
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
