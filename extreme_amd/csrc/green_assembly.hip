@@ -340,7 +340,7 @@ __device__ inline Factors factors(double s, double u, double rs, double ru, doub
 }
 
 struct LatArgs {
-    int nx, ny, npairs, asym, nr;
+    int nx, ny, nz, npairs, asym, nr;
     double dx, dy;
     const double2 *segA, *segB;   // [seg][comp][pair]
     const double *radii;
@@ -462,8 +462,10 @@ __global__ __launch_bounds__(256) void lateral_kernel(LatArgs a)
     // write G(i,j) and its mirror images (GreenTensorFftPopulator.cs:88-148)
     const int nq = a.asym ? 2 : 4;
     const int64_t nx2 = 2LL * a.nx, ny2 = 2LL * a.ny;
+    // asym pairs are rc-major (p = rc*nz + tr); the device block stores them swizzled
+    const int pd = a.asym ? asym_pos(p / a.nz, p % a.nz, a.nz, asym_swizzle_mask(a.nz)) : p;
     for (int t = 0; t < nq; ++t) {
-        const int64_t o = a.off[t] + p;
+        const int64_t o = a.off[t] + pd;
         const double2 g = G.q[t];
         a.G[((int64_t)i * ny2 + j) * a.gblk + o] = g;
         if (i >= 1) a.G[((nx2 - i) * ny2 + j) * a.gblk + o] = (double)a.xs[t] * g;
@@ -660,6 +662,7 @@ void build_green_atoa(const GreenInput &in, double2 *G, const double2 *tw_x, con
         LatArgs la{};
         la.nx = g.nx;
         la.ny = g.ny;
+        la.nz = nz;
         la.npairs = npairs;
         la.asym = asym;
         la.nr = nmerged;

@@ -13,6 +13,10 @@
 // butterfly inputs into registers before a barrier and writing after it, and writes
 // the tile back. The zero padding of the forward passes and the cropping of the inverse
 // passes are applied at load/store time, so padded rows are never read or written twice.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
 #include "xtr_common.h"
 #include "xtr_kernels.h"
 
@@ -346,6 +350,7 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
     const int tid = threadIdx.x, nth = blockDim.x;
     const int64_t l0 = (int64_t)blockIdx.x * W;
     const int per_w = nrhs * C;
+    const int swz = asym_swizzle_mask(nz);
 
     for (int e = tid; e < W * per_w; e += nth) {
         const int w = e / per_w, rest = e % per_w;
@@ -379,7 +384,8 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
                 bb[rc * ld + tr] = g;
                 bb[tr * ld + rc] = g;
             } else {
-                bb[(o / nz) * ld + (o % nz)] = g;
+                const int rc = o / nz;
+                bb[rc * ld + ((o % nz) ^ (rc & swz))] = g;
             }
         }
         __syncthreads();
@@ -415,9 +421,216 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
     }
 }
 
+// ---- streaming contraction (FP64, nz = 32) ------------------------------------------------------
+// Persistent: one 256-thread workgroup per CU walks wavenumbers l = blockIdx.x, += gridDim.x.
+// Each wavenumber's whole Green block (4 packed + 2 swizzled asymmetric blocks, 4160 elements) and
+// its NRHS spectrum columns are copied straight from HBM into LDS by global_load_lds_dwordx4
+// (64 lanes x 16 B = 1 KB per wave-load; 68 wave-loads, 17 per wave), double-buffered: the copy
+// of wavenumber l + gridDim.x is in flight while l is contracted, so the HBM stream never waits
+// for the arithmetic. The copies are issued from inline asm so that the only waits on them are the
+// explicit vmcnt(17) / vmcnt(0) below: vector-memory operations of one wave complete in order, so
+// after issuing the 17 copies of the next wavenumber, vmcnt(17) means the current one has landed.
+// Contraction: wave w owns columns [8w, 8w+8); lane (s, r) takes row r and columns 8w + 4s + j
+// (j < 4) of all six blocks, for all three components and both right-hand sides, with its u slice
+// in registers (36 FMAs per 8 LDS reads). Partial rows are summed across s by a lane shuffle and
+// across the four waves through LDS.
+namespace stream {
+constexpr int NZ = 32, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
+constexpr int NCHUNK = 68;                  // ceil((GB + 2C) / 64), a multiple of the 4 waves
+constexpr int PER_WAVE = NCHUNK / 4;
+constexpr int BUF = NCHUNK * 64;            // elements per LDS buffer
+constexpr int MASK = 15;                    // asym_swizzle_mask(32)
+static_assert((GB + 2 * C + 63) / 64 <= NCHUNK && NCHUNK % 4 == 0 && PER_WAVE == 17, "chunking");
+}  // namespace stream
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+template <int NRHS>
+__device__ __forceinline__ void stream_issue(const double2 *__restrict__ G, const double2 *buf, int64_t M, int64_t l,
+                                             unsigned dst, int wave, int lane)
+{
+    using namespace stream;
+#pragma unroll
+    for (int t = 0; t < PER_WAVE; ++t) {
+        const int k = wave + 4 * t;
+        const double2 *src;
+        if (k * 64 < GB) {
+            src = G + l * GB + k * 64 + lane;                   // GB = 65 * 64: whole chunks
+        } else {
+            int q = (k - GB / 64) * 64 + lane;                  // spectrum column q of this wavenumber
+            if (q >= NRHS * C) q = NRHS * C - 1;                // padding lanes re-read a valid element
+            const int rhs = q / C;
+            src = buf + rhs * M + l * C + (q - rhs * C);
+        }
+        const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+    }
+}
+#pragma clang diagnostic pop
+#define XTR_WAIT_VM(n) asm volatile("s_waitcnt vmcnt(" #n ")" ::: "memory")
+// LDS writes of this wave (partial sums) must be complete before other waves pass the barrier and
+// read them: s_barrier alone does not wait for them.
+#define XTR_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
+
+template <int NRHS>
+__device__ __forceinline__ void stream_compute(const double2 *__restrict__ blk, double2 *__restrict__ part, int wave,
+                                               int lane)
+{
+    using namespace stream;
+    const double2 *u = blk + GB;                                 // [NRHS][C]
+    const int s = lane >> 5, r = lane & 31, r15 = r & MASK;
+    const int cb = wave * 8 + s * 4;
+    const int trir = r * (r + 1) / 2;
+    double2 ux[NRHS][4], uy[NRHS][4], uz[NRHS][4];
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            ux[h][j] = u[h * C + cb + j];
+            uy[h][j] = u[h * C + NZ + cb + j];
+            uz[h][j] = u[h * C + 2 * NZ + cb + j];
+        }
+    double2 yx[NRHS], yy[NRHS], yz[NRHS];
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+        yx[h] = mk<double>(0, 0);
+        yy[h] = yx[h];
+        yz[h] = yx[h];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = cb + j;
+        const int pidx = c >= r ? r + c * (c + 1) / 2 : c + trir;    // packed (min, max)
+        const double2 gxx = blk[pidx], gxy = blk[S + pidx], gyy = blk[2 * S + pidx], gzz = blk[3 * S + pidx];
+        const int ro = r * NZ + (c ^ r15), co = c * NZ + (r ^ (c & MASK));   // (r, c) and (c, r), swizzled
+        const double2 gxz = blk[4 * S + ro], gyz = blk[4 * S + A + ro];
+        const double2 gzx = blk[4 * S + co], gzy = blk[4 * S + A + co];
+#pragma unroll
+        for (int h = 0; h < NRHS; ++h) {
+            yx[h] = cfma<double>(yx[h], gxx, ux[h][j]);
+            yx[h] = cfma<double>(yx[h], gxy, uy[h][j]);
+            yx[h] = cfma<double>(yx[h], gxz, uz[h][j]);
+            yy[h] = cfma<double>(yy[h], gxy, ux[h][j]);
+            yy[h] = cfma<double>(yy[h], gyy, uy[h][j]);
+            yy[h] = cfma<double>(yy[h], gyz, uz[h][j]);
+            yz[h] = cfms<double>(yz[h], gzx, ux[h][j]);        // -XZ^T u_x
+            yz[h] = cfms<double>(yz[h], gzy, uy[h][j]);        // -YZ^T u_y
+            yz[h] = cfma<double>(yz[h], gzz, uz[h][j]);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+        yx[h].x += __shfl_xor(yx[h].x, 32, 64);
+        yx[h].y += __shfl_xor(yx[h].y, 32, 64);
+        yy[h].x += __shfl_xor(yy[h].x, 32, 64);
+        yy[h].y += __shfl_xor(yy[h].y, 32, 64);
+        yz[h].x += __shfl_xor(yz[h].x, 32, 64);
+        yz[h].y += __shfl_xor(yz[h].y, 32, 64);
+    }
+    // both halves now hold the wave's row sums; half s stores right-hand side s
+    if (NRHS == 1 && s) return;
+    const int h = NRHS == 1 ? 0 : s;
+    double2 *pw = part + (wave * NRHS + h) * C;
+    // component-wise selects (a select of whole double2 values would go through scratch)
+    const double vxx = h ? yx[NRHS - 1].x : yx[0].x, vxy = h ? yx[NRHS - 1].y : yx[0].y;
+    const double vyx = h ? yy[NRHS - 1].x : yy[0].x, vyy = h ? yy[NRHS - 1].y : yy[0].y;
+    const double vzx = h ? yz[NRHS - 1].x : yz[0].x, vzy = h ? yz[NRHS - 1].y : yz[0].y;
+    pw[r] = mk<double>(vxx, vxy);
+    pw[NZ + r] = mk<double>(vyx, vyy);
+    pw[2 * NZ + r] = mk<double>(vzx, vzy);
+}
+
+template <int NRHS>
+__global__ __launch_bounds__(256) void contract_stream_kernel(const double2 *__restrict__ G, double2 *buf, int64_t L,
+                                                              int64_t M)
+{
+    using namespace stream;
+    __shared__ double2 bufA[BUF];
+    __shared__ double2 bufB[BUF];
+    __shared__ double2 part[4 * NRHS * C];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
+    const unsigned la = (unsigned)(size_t)(__attribute__((address_space(3))) double2 *)bufA;
+    const unsigned lb = (unsigned)(size_t)(__attribute__((address_space(3))) double2 *)bufB;
+    int64_t l = blockIdx.x;                                      // the launch keeps gridDim.x <= L
+    auto finish = [&](int64_t lw) __attribute__((always_inline)) {
+        if (t < NRHS * C) {
+            double2 acc = part[t];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) acc = cadd<double>(acc, part[w * NRHS * C + t]);
+            const int h = t / C;
+            buf[h * M + lw * C + (t - h * C)] = acc;
+        }
+    };
+    stream_issue<NRHS>(G, buf, M, l, la, wave, lane);
+    while (true) {
+        int64_t ln = l + gridDim.x;
+        if (ln < L) {
+            stream_issue<NRHS>(G, buf, M, ln, lb, wave, lane);
+            XTR_WAIT_VM(17);
+        } else {
+            XTR_WAIT_VM(0);
+        }
+        XTR_BARRIER();                                           // wavenumber l is in bufA
+        stream_compute<NRHS>(bufA, part, wave, lane);
+        XTR_BARRIER();                                           // partials complete; bufA free
+        finish(l);
+        if (ln >= L) break;
+        l = ln;
+        ln = l + gridDim.x;
+        if (ln < L) {
+            stream_issue<NRHS>(G, buf, M, ln, la, wave, lane);
+            XTR_WAIT_VM(17);
+        } else {
+            XTR_WAIT_VM(0);
+        }
+        XTR_BARRIER();
+        stream_compute<NRHS>(bufB, part, wave, lane);
+        XTR_BARRIER();
+        finish(l);
+        if (ln >= L) break;
+        l = ln;
+    }
+    XTR_WAIT_VM(0);                                              // no copy may land after exit
+}
+
+// XTR_CONTRACT_KERNEL=generic selects contract_kernel for every shape (to compare the two kernels).
+static bool contract_force_generic()
+{
+    const char *e = std::getenv("XTR_CONTRACT_KERNEL");
+    return e && std::strcmp(e, "generic") == 0;
+}
+
+static int device_cus()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    XTR_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw hip_error("device index out of range");
+    if (!cus[dev]) XTR_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev];
+}
+
+// The streaming kernel covers FP64 at nz = 32 (two 68 KB LDS buffers); other shapes use contract_kernel.
+static bool contract_stream(const ApplyArgs &a, hipStream_t s)
+{
+    if (a.precision != 0 || a.g.nz != stream::NZ || (a.nrhs != 1 && a.nrhs != 2) || contract_force_generic())
+        return false;
+    const int64_t L = a.g.L();
+    const unsigned grid = (unsigned)std::min<int64_t>(L, device_cus());
+    if (a.nrhs == 2)
+        hipLaunchKernelGGL(contract_stream_kernel<2>, dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
+                           (double2 *)a.buf, L, a.g.M());
+    else
+        hipLaunchKernelGGL(contract_stream_kernel<1>, dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
+                           (double2 *)a.buf, L, a.g.M());
+    XTR_LAUNCH_CHECK("contract_stream_kernel");
+    return true;
+}
+
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
+    if (contract_stream(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;
@@ -454,8 +667,10 @@ __global__ void green_interleave_kernel(const double2 *xx, const double2 *xy, co
             si = l * S + (o % S);
         } else {
             const int b = (int)((o - 4LL * S) / A);
+            const int pos = (int)((o - 4LL * S) % A);          // device (swizzled) position
+            const int rc = pos / nz, tr = (pos % nz) ^ (rc & asym_swizzle_mask(nz));
             src = b == 0 ? xz : yz;
-            si = l * A + ((o - 4LL * S) % A);
+            si = l * A + (int64_t)rc * nz + tr;
         }
         if (dir == 0) dev[e] = src[si];
         else const_cast<double2 *>(src)[si] = dev[e];

@@ -32,6 +32,15 @@ template <typename T> __host__ __device__ inline cplx<T> cfma(cplx<T> acc, cplx<
     acc.y = fma(a.y, b.x, acc.y);
     return acc;
 }
+// acc - a*b
+template <typename T> __host__ __device__ inline cplx<T> cfms(cplx<T> acc, cplx<T> a, cplx<T> b)
+{
+    acc.x = fma(-a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+    return acc;
+}
 // acc + conj(a)*b
 template <typename T> __host__ __device__ inline cplx<T> cfma_conj(cplx<T> acc, cplx<T> a, cplx<T> b)
 {

@@ -21,6 +21,13 @@ struct Geom {
     int64_t gblk() const { return 4 * S() + 2LL * nz * nz; }  // Green elements per wavenumber
 };
 
+// Device position of the asymmetric-block element (rc, tr) (xz and yz, nz*nz each): rows are
+// XOR-swizzled, tr ^ (rc & mask), so that a block copied linearly into LDS can be read along rows
+// and along columns without bank conflicts. mask = 15 when 16 | nz, 7 when 8 | nz, else 0
+// (plain row-major). The reference layout at the set/get boundary is plain row-major [rc][tr].
+__host__ __device__ inline int asym_swizzle_mask(int nz) { return nz % 16 == 0 ? 15 : (nz % 8 == 0 ? 7 : 0); }
+__host__ __device__ inline int asym_pos(int rc, int tr, int nz, int mask) { return rc * nz + (tr ^ (rc & mask)); }
+
 // Per-call vector pointers (device, FP64) for up to kMaxRhs right-hand sides.
 struct RhsPtrs {
     const double2 *x[kMaxRhs];

@@ -80,3 +80,67 @@ def test_apply_fp32_error_bound(oracle, torch_cuda):
     e = rel(op.apply(x), oop.apply(x))
     print(f"C1 fp32 apply rel error {e:.3e}", flush=True)
     assert e < TOL_FP32
+
+
+def _random_green(oracle, model, seed):
+    """A seeded random Green tensor in the reference layout (exercises every element position)."""
+    rng = np.random.default_rng(seed)
+    L = 4 * model.nx * model.ny
+    nz = model.nz
+    S = nz * (nz + 1) // 2
+    return {c: rng.standard_normal((L, nz * nz if c in ("xz", "yz") else S))
+            + 1j * rng.standard_normal((L, nz * nz if c in ("xz", "yz") else S)) for c in oracle.COMPONENTS}
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 32), (12, 6, 32), (24, 12, 32)])
+def test_apply_nz32_random_green(oracle, torch_cuda, shape):
+    """nz = 32 runs the streaming contraction: fewer, equal and more wavenumbers than CUs, an uneven
+    split over the persistent workgroups, one and two right-hand sides."""
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nx + ny)
+    w = cm.omega_of(m.frequencies[0])
+    G = _random_green(oracle, m, seed=nx * ny)
+    op = X.ConvolutionOperator(m, w, max_rhs=2)
+    op.set_green_tensor(G)
+    oop = oracle.Operator(m, G)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator("A", z)
+    oop.prepare("A", z, z0)
+    rng = np.random.default_rng(11)
+    xs = rng.standard_normal((2, m.n)) + 1j * rng.standard_normal((2, m.n))
+    ref = np.stack([oop.apply(xs[0]), oop.apply(xs[1])])
+    y2 = op.apply(xs)
+    y1 = op.apply(xs[0])
+    print(f"nz32 {shape}: nrhs2 {rel(y2, ref):.3e} nrhs1 {rel(y1, ref[0]):.3e}", flush=True)
+    assert rel(y2, ref) < TOL_FP64
+    assert rel(y1, ref[0]) < TOL_FP64
+
+
+def test_streaming_contraction_matches_generic(oracle, torch_cuda, monkeypatch):
+    """The two contraction kernels agree on the same operator and input (summation order only)."""
+    m = cm.small_model(24, 12, 32, seed=5)
+    w = cm.omega_of(m.frequencies[0])
+    G = _random_green(oracle, m, seed=3)
+    op = X.ConvolutionOperator(m, w, max_rhs=2)
+    op.set_green_tensor(G)
+    op.prepare_operator("A")
+    xs = np.random.default_rng(2).standard_normal((2, m.n)) + 0.5j
+    y_stream = op.apply(xs)
+    monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
+    y_generic = op.apply(xs)
+    monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+    print(f"stream vs generic: {rel(y_stream, y_generic):.3e}", flush=True)
+    assert rel(y_stream, y_generic) < 1e-14
+
+
+@pytest.mark.parametrize("nz", [5, 8, 24, 32])
+def test_green_set_get_roundtrip(oracle, torch_cuda, nz):
+    """Reference layout -> device layout (swizzled asymmetric blocks) -> reference layout, bit-exact."""
+    m = cm.small_model(3, 2, nz, seed=nz)
+    w = cm.omega_of(m.frequencies[0])
+    G = _random_green(oracle, m, seed=nz)
+    op = X.ConvolutionOperator(m, w)
+    op.set_green_tensor(G)
+    back = op.green_tensor()
+    for c in oracle.COMPONENTS:
+        assert np.array_equal(back[c].reshape(G[c].shape), G[c]), c
