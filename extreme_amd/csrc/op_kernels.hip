@@ -421,44 +421,69 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
     }
 }
 
-// ---- streaming contraction (FP64, nz = 32) ------------------------------------------------------
+// ---- streaming contraction (nz = 32) ---------------------------------------------------------------
 // Persistent: one 256-thread workgroup per CU walks wavenumbers l = blockIdx.x, += gridDim.x.
 // Each wavenumber's whole Green block (4 packed + 2 swizzled asymmetric blocks, 4160 elements) and
-// its NRHS spectrum columns are copied straight from HBM into LDS by global_load_lds_dwordx4
-// (64 lanes x 16 B = 1 KB per wave-load; 68 wave-loads, 17 per wave), double-buffered: the copy
-// of wavenumber l + gridDim.x is in flight while l is contracted, so the HBM stream never waits
-// for the arithmetic. The copies are issued from inline asm so that the only waits on them are the
-// explicit vmcnt(17) / vmcnt(0) below: vector-memory operations of one wave complete in order, so
-// after issuing the 17 copies of the next wavenumber, vmcnt(17) means the current one has landed.
+// its spectrum columns are copied straight from HBM into LDS by global_load_lds_dwordx4 (64 lanes x
+// 16 B = 1 KB per wave-load), into a ring of NBUF buffers: the copies of the next NBUF-1 wavenumbers
+// are in flight while one is contracted, so the HBM stream does not wait for the arithmetic
+// (complex128: 68 KB per wavenumber, 2 buffers; complex64: 34 KB, 4 buffers). The copies are issued
+// from inline asm so that the only waits on them are the explicit vmcnt waits below: vector-memory
+// operations of one wave complete in order, so with k wave-loads issued after those of wavenumber l,
+// vmcnt(k) means l has landed.
 // Contraction: wave w owns columns [8w, 8w+8); lane (s, r) takes row r and columns 8w + 4s + j
 // (j < 4) of all six blocks, for all three components and both right-hand sides, with its u slice
 // in registers (36 FMAs per 8 LDS reads). Partial rows are summed across s by a lane shuffle and
 // across the four waves through LDS.
 namespace stream {
 constexpr int NZ = 32, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
-constexpr int NCHUNK = 68;                  // ceil((GB + 2C) / 64), a multiple of the 4 waves
-constexpr int PER_WAVE = NCHUNK / 4;
-constexpr int BUF = NCHUNK * 64;            // elements per LDS buffer
 constexpr int MASK = 15;                    // asym_swizzle_mask(32)
-static_assert((GB + 2 * C + 63) / 64 <= NCHUNK && NCHUNK % 4 == 0 && PER_WAVE == 17, "chunking");
+template <typename T> struct Cfg {
+    static constexpr int E = (int)sizeof(cplx<T>);            // bytes per element
+    static constexpr int EPL = 16 / E;                        // elements per lane per wave-load
+    static constexpr int EPC = 64 * EPL;                      // elements per wave-load
+    static constexpr int RAW = (GB + 2 * C + EPC - 1) / EPC;  // wave-loads for Green + 2 rhs columns
+    static constexpr int NCHUNK = (RAW + 3) / 4 * 4;          // padded to the 4 waves
+    static constexpr int PER_WAVE = NCHUNK / 4;
+    static constexpr int BUF = NCHUNK * EPC;                  // elements per LDS buffer
+    static constexpr int NBUF = E == 16 ? 2 : 4;
+};
+static_assert(GB % 2 == 0 && C % 2 == 0, "element pairs never straddle the Green/spectrum boundary");
+static_assert(Cfg<double>::PER_WAVE * (Cfg<double>::NBUF - 1) <= 63, "vmcnt range");
+static_assert(Cfg<float>::PER_WAVE * (Cfg<float>::NBUF - 1) <= 63, "vmcnt range");
+static_assert(Cfg<double>::NBUF * Cfg<double>::BUF * 16 + 4 * 2 * C * 16 <= 160 * 1024, "LDS");
+static_assert(Cfg<float>::NBUF * Cfg<float>::BUF * 8 + 4 * 2 * C * 8 <= 160 * 1024, "LDS");
 }  // namespace stream
+
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// LDS writes of this wave (partial sums) must be complete before other waves pass the barrier and
+// read them: s_barrier alone does not wait for them.
+#define XTR_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
 
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-template <int NRHS>
-__device__ __forceinline__ void stream_issue(const double2 *__restrict__ G, const double2 *buf, int64_t M, int64_t l,
+// This wave's PER_WAVE wave-loads of wavenumber l into the LDS buffer at byte address dst.
+template <typename T, int NRHS>
+__device__ __forceinline__ void stream_issue(const cplx<T> *__restrict__ G, const cplx<T> *buf, int64_t M, int64_t l,
                                              unsigned dst, int wave, int lane)
 {
     using namespace stream;
+    using K = Cfg<T>;
 #pragma unroll
-    for (int t = 0; t < PER_WAVE; ++t) {
+    for (int t = 0; t < K::PER_WAVE; ++t) {
         const int k = wave + 4 * t;
-        const double2 *src;
-        if (k * 64 < GB) {
-            src = G + l * GB + k * 64 + lane;                   // GB = 65 * 64: whole chunks
+        const int e = k * K::EPC + lane * K::EPL;                  // first element this lane copies
+        const cplx<T> *src;
+        if (e < GB) {
+            src = G + l * GB + e;
         } else {
-            int q = (k - GB / 64) * 64 + lane;                  // spectrum column q of this wavenumber
-            if (q >= NRHS * C) q = NRHS * C - 1;                // padding lanes re-read a valid element
+            int q = e - GB;                                         // spectrum column of this wavenumber
+            if (q > NRHS * C - K::EPL) q = NRHS * C - K::EPL;       // padding lanes re-read valid data
             const int rhs = q / C;
             src = buf + rhs * M + l * C + (q - rhs * C);
         }
@@ -467,21 +492,17 @@ __device__ __forceinline__ void stream_issue(const double2 *__restrict__ G, cons
     }
 }
 #pragma clang diagnostic pop
-#define XTR_WAIT_VM(n) asm volatile("s_waitcnt vmcnt(" #n ")" ::: "memory")
-// LDS writes of this wave (partial sums) must be complete before other waves pass the barrier and
-// read them: s_barrier alone does not wait for them.
-#define XTR_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
 
-template <int NRHS>
-__device__ __forceinline__ void stream_compute(const double2 *__restrict__ blk, double2 *__restrict__ part, int wave,
+template <typename T, int NRHS>
+__device__ __forceinline__ void stream_compute(const cplx<T> *__restrict__ blk, cplx<T> *__restrict__ part, int wave,
                                                int lane)
 {
     using namespace stream;
-    const double2 *u = blk + GB;                                 // [NRHS][C]
+    const cplx<T> *u = blk + GB;                                 // [NRHS][C]
     const int s = lane >> 5, r = lane & 31, r15 = r & MASK;
     const int cb = wave * 8 + s * 4;
     const int trir = r * (r + 1) / 2;
-    double2 ux[NRHS][4], uy[NRHS][4], uz[NRHS][4];
+    cplx<T> ux[NRHS][4], uy[NRHS][4], uz[NRHS][4];
 #pragma unroll
     for (int h = 0; h < NRHS; ++h)
 #pragma unroll
@@ -490,10 +511,10 @@ __device__ __forceinline__ void stream_compute(const double2 *__restrict__ blk, 
             uy[h][j] = u[h * C + NZ + cb + j];
             uz[h][j] = u[h * C + 2 * NZ + cb + j];
         }
-    double2 yx[NRHS], yy[NRHS], yz[NRHS];
+    cplx<T> yx[NRHS], yy[NRHS], yz[NRHS];
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) {
-        yx[h] = mk<double>(0, 0);
+        yx[h] = mk<T>(0, 0);
         yy[h] = yx[h];
         yz[h] = yx[h];
     }
@@ -501,21 +522,21 @@ __device__ __forceinline__ void stream_compute(const double2 *__restrict__ blk, 
     for (int j = 0; j < 4; ++j) {
         const int c = cb + j;
         const int pidx = c >= r ? r + c * (c + 1) / 2 : c + trir;    // packed (min, max)
-        const double2 gxx = blk[pidx], gxy = blk[S + pidx], gyy = blk[2 * S + pidx], gzz = blk[3 * S + pidx];
+        const cplx<T> gxx = blk[pidx], gxy = blk[S + pidx], gyy = blk[2 * S + pidx], gzz = blk[3 * S + pidx];
         const int ro = r * NZ + (c ^ r15), co = c * NZ + (r ^ (c & MASK));   // (r, c) and (c, r), swizzled
-        const double2 gxz = blk[4 * S + ro], gyz = blk[4 * S + A + ro];
-        const double2 gzx = blk[4 * S + co], gzy = blk[4 * S + A + co];
+        const cplx<T> gxz = blk[4 * S + ro], gyz = blk[4 * S + A + ro];
+        const cplx<T> gzx = blk[4 * S + co], gzy = blk[4 * S + A + co];
 #pragma unroll
         for (int h = 0; h < NRHS; ++h) {
-            yx[h] = cfma<double>(yx[h], gxx, ux[h][j]);
-            yx[h] = cfma<double>(yx[h], gxy, uy[h][j]);
-            yx[h] = cfma<double>(yx[h], gxz, uz[h][j]);
-            yy[h] = cfma<double>(yy[h], gxy, ux[h][j]);
-            yy[h] = cfma<double>(yy[h], gyy, uy[h][j]);
-            yy[h] = cfma<double>(yy[h], gyz, uz[h][j]);
-            yz[h] = cfms<double>(yz[h], gzx, ux[h][j]);        // -XZ^T u_x
-            yz[h] = cfms<double>(yz[h], gzy, uy[h][j]);        // -YZ^T u_y
-            yz[h] = cfma<double>(yz[h], gzz, uz[h][j]);
+            yx[h] = cfma<T>(yx[h], gxx, ux[h][j]);
+            yx[h] = cfma<T>(yx[h], gxy, uy[h][j]);
+            yx[h] = cfma<T>(yx[h], gxz, uz[h][j]);
+            yy[h] = cfma<T>(yy[h], gxy, ux[h][j]);
+            yy[h] = cfma<T>(yy[h], gyy, uy[h][j]);
+            yy[h] = cfma<T>(yy[h], gyz, uz[h][j]);
+            yz[h] = cfms<T>(yz[h], gzx, ux[h][j]);             // -XZ^T u_x
+            yz[h] = cfms<T>(yz[h], gzy, uy[h][j]);             // -YZ^T u_y
+            yz[h] = cfma<T>(yz[h], gzz, uz[h][j]);
         }
     }
 #pragma unroll
@@ -530,67 +551,66 @@ __device__ __forceinline__ void stream_compute(const double2 *__restrict__ blk, 
     // both halves now hold the wave's row sums; half s stores right-hand side s
     if (NRHS == 1 && s) return;
     const int h = NRHS == 1 ? 0 : s;
-    double2 *pw = part + (wave * NRHS + h) * C;
-    // component-wise selects (a select of whole double2 values would go through scratch)
-    const double vxx = h ? yx[NRHS - 1].x : yx[0].x, vxy = h ? yx[NRHS - 1].y : yx[0].y;
-    const double vyx = h ? yy[NRHS - 1].x : yy[0].x, vyy = h ? yy[NRHS - 1].y : yy[0].y;
-    const double vzx = h ? yz[NRHS - 1].x : yz[0].x, vzy = h ? yz[NRHS - 1].y : yz[0].y;
-    pw[r] = mk<double>(vxx, vxy);
-    pw[NZ + r] = mk<double>(vyx, vyy);
-    pw[2 * NZ + r] = mk<double>(vzx, vzy);
+    cplx<T> *pw = part + (wave * NRHS + h) * C;
+    // component-wise selects (a select of whole vector values would go through scratch)
+    const T vxx = h ? yx[NRHS - 1].x : yx[0].x, vxy = h ? yx[NRHS - 1].y : yx[0].y;
+    const T vyx = h ? yy[NRHS - 1].x : yy[0].x, vyy = h ? yy[NRHS - 1].y : yy[0].y;
+    const T vzx = h ? yz[NRHS - 1].x : yz[0].x, vzy = h ? yz[NRHS - 1].y : yz[0].y;
+    pw[r] = mk<T>(vxx, vxy);
+    pw[NZ + r] = mk<T>(vyx, vyy);
+    pw[2 * NZ + r] = mk<T>(vzx, vzy);
 }
 
-template <int NRHS>
-__global__ __launch_bounds__(256) void contract_stream_kernel(const double2 *__restrict__ G, double2 *buf, int64_t L,
+// waits until at most `after` groups of PER_WAVE wave-loads are outstanding
+template <int PW>
+__device__ __forceinline__ void wait_groups(int after)
+{
+    switch (after) {
+    case 0: wait_vm<0>(); break;
+    case 1: wait_vm<PW>(); break;
+    case 2: wait_vm<2 * PW>(); break;
+    default: wait_vm<3 * PW>(); break;
+    }
+}
+
+template <typename T, int NRHS>
+__global__ __launch_bounds__(256) void contract_stream_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t L,
                                                               int64_t M)
 {
     using namespace stream;
-    __shared__ double2 bufA[BUF];
-    __shared__ double2 bufB[BUF];
-    __shared__ double2 part[4 * NRHS * C];
+    using K = Cfg<T>;
+    static_assert(K::NBUF <= 4, "wait_groups covers up to three groups in flight after the current one");
+    __shared__ cplx<T> ring[K::NBUF * K::BUF];
+    __shared__ cplx<T> part[4 * NRHS * C];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
-    const unsigned la = (unsigned)(size_t)(__attribute__((address_space(3))) double2 *)bufA;
-    const unsigned lb = (unsigned)(size_t)(__attribute__((address_space(3))) double2 *)bufB;
-    int64_t l = blockIdx.x;                                      // the launch keeps gridDim.x <= L
-    auto finish = [&](int64_t lw) __attribute__((always_inline)) {
-        if (t < NRHS * C) {
-            double2 acc = part[t];
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
+    const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= L
+    const int64_t l0 = blockIdx.x;
+    const int64_t cnt = (L - 1 - l0) / g + 1;                    // wavenumbers of this workgroup
 #pragma unroll
-            for (int w = 1; w < 4; ++w) acc = cadd<double>(acc, part[w * NRHS * C + t]);
+    for (int p = 0; p < K::NBUF - 1; ++p)
+        if (p < cnt) stream_issue<T, NRHS>(G, buf, M, l0 + p * g, base + (unsigned)(p * K::BUF * K::E), wave, lane);
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int64_t l = l0 + i * g;
+        const int64_t ahead = i + K::NBUF - 1;                   // the wavenumber index to prefetch now
+        if (ahead < cnt)
+            stream_issue<T, NRHS>(G, buf, M, l0 + ahead * g, base + (unsigned)((ahead % K::NBUF) * K::BUF * K::E),
+                                  wave, lane);
+        // groups issued after wavenumber i's: i+1 .. min(cnt-1, i+NBUF-1)
+        const int64_t after = (cnt - 1 - i) < (K::NBUF - 1) ? (cnt - 1 - i) : (K::NBUF - 1);
+        wait_groups<K::PER_WAVE>((int)after);
+        XTR_BARRIER();                                           // wavenumber i is in its buffer
+        stream_compute<T, NRHS>(ring + (i % K::NBUF) * K::BUF, part, wave, lane);
+        XTR_BARRIER();                                           // partials complete; buffer free
+        if (t < NRHS * C) {
+            cplx<T> acc = part[t];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) acc = cadd<T>(acc, part[w * NRHS * C + t]);
             const int h = t / C;
-            buf[h * M + lw * C + (t - h * C)] = acc;
+            buf[h * M + l * C + (t - h * C)] = acc;
         }
-    };
-    stream_issue<NRHS>(G, buf, M, l, la, wave, lane);
-    while (true) {
-        int64_t ln = l + gridDim.x;
-        if (ln < L) {
-            stream_issue<NRHS>(G, buf, M, ln, lb, wave, lane);
-            XTR_WAIT_VM(17);
-        } else {
-            XTR_WAIT_VM(0);
-        }
-        XTR_BARRIER();                                           // wavenumber l is in bufA
-        stream_compute<NRHS>(bufA, part, wave, lane);
-        XTR_BARRIER();                                           // partials complete; bufA free
-        finish(l);
-        if (ln >= L) break;
-        l = ln;
-        ln = l + gridDim.x;
-        if (ln < L) {
-            stream_issue<NRHS>(G, buf, M, ln, la, wave, lane);
-            XTR_WAIT_VM(17);
-        } else {
-            XTR_WAIT_VM(0);
-        }
-        XTR_BARRIER();
-        stream_compute<NRHS>(bufB, part, wave, lane);
-        XTR_BARRIER();
-        finish(l);
-        if (ln >= L) break;
-        l = ln;
     }
-    XTR_WAIT_VM(0);                                              // no copy may land after exit
+    wait_vm<0>();                                                // no copy may land after exit
 }
 
 // XTR_CONTRACT_KERNEL=generic selects contract_kernel for every shape (to compare the two kernels).
@@ -610,19 +630,19 @@ static int device_cus()
     return cus[dev];
 }
 
-// The streaming kernel covers FP64 at nz = 32 (two 68 KB LDS buffers); other shapes use contract_kernel.
+// The streaming kernel covers nz = 32 with one or two right-hand sides; other shapes use contract_kernel.
+template <typename T>
 static bool contract_stream(const ApplyArgs &a, hipStream_t s)
 {
-    if (a.precision != 0 || a.g.nz != stream::NZ || (a.nrhs != 1 && a.nrhs != 2) || contract_force_generic())
-        return false;
+    if (a.g.nz != stream::NZ || (a.nrhs != 1 && a.nrhs != 2) || contract_force_generic()) return false;
     const int64_t L = a.g.L();
     const unsigned grid = (unsigned)std::min<int64_t>(L, device_cus());
     if (a.nrhs == 2)
-        hipLaunchKernelGGL(contract_stream_kernel<2>, dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
-                           (double2 *)a.buf, L, a.g.M());
+        hipLaunchKernelGGL((contract_stream_kernel<T, 2>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
+                           (cplx<T> *)a.buf, L, a.g.M());
     else
-        hipLaunchKernelGGL(contract_stream_kernel<1>, dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
-                           (double2 *)a.buf, L, a.g.M());
+        hipLaunchKernelGGL((contract_stream_kernel<T, 1>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
+                           (cplx<T> *)a.buf, L, a.g.M());
     XTR_LAUNCH_CHECK("contract_stream_kernel");
     return true;
 }
@@ -630,7 +650,7 @@ static bool contract_stream(const ApplyArgs &a, hipStream_t s)
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
-    if (contract_stream(a, s)) return;
+    if (contract_stream<T>(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;

@@ -94,6 +94,7 @@ struct PinBuf {
 struct xtr_op {
     xtr::Geom g{};
     int precision = XTR_FP64;
+    int active = XTR_FP64;        // arithmetic of the next applies when precision == XTR_MIXED
     int max_rhs = 1;
     int type = -1;
     double omega = 0;
@@ -105,7 +106,7 @@ struct xtr_op {
 
     hipStream_t stream = nullptr;
     DevBuf<double2> G64;          // [L][gblk] FP64 Green tensor
-    DevBuf<float2> G32;           // [L][gblk] complex64 copy for XTR_FP32
+    DevBuf<float2> G32;           // [L][gblk] complex64 copy for XTR_FP32 and XTR_MIXED
     bool green_ready = false;
     DevBuf<double2> R;            // [nx*ny*nz]
     DevBuf<double> f, b;          // [nz]
@@ -139,7 +140,8 @@ struct xtr_op {
         return ev_sets[ev_used++].data();
     }
 
-    size_t elem() const { return precision == XTR_FP64 ? sizeof(double2) : sizeof(float2); }
+    size_t elem() const { return precision == XTR_FP32 ? sizeof(float2) : sizeof(double2); }
+    int arith() const { return precision == XTR_MIXED ? active : precision; }
 
     // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr).
     void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s)
@@ -148,19 +150,20 @@ struct xtr_op {
         require(type == XTR_OP_A || type == XTR_OP_CHI0, "the operator is not prepared");
         require(nrhs >= 1 && nrhs <= max_rhs, "nrhs out of range");
         std::lock_guard<std::mutex> lk(apply_mu);
+        const int p = arith();
         xtr::ApplyArgs a{};
         a.g = g;
-        a.precision = precision;
+        a.precision = p;
         a.nrhs = nrhs;
         a.type = type;
         a.R = R.p;
         a.f = f.p;
         a.b = b.p;
-        a.G = precision == XTR_FP64 ? (const void *)G64.p : (const void *)G32.p;
+        a.G = p == XTR_FP64 ? (const void *)G64.p : (const void *)G32.p;
         a.packed_rc = packed_rc.p;
         a.buf = buf.p;
-        a.tw_x = precision == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
-        a.tw_y = precision == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
+        a.tw_x = p == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
+        a.tw_y = p == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
         a.v = v;
         hipEvent_t *ev = profiling ? next_events() : nullptr;
         if (ev) XTR_CHECK(hipEventRecord(ev[0], s));
@@ -559,6 +562,72 @@ struct ApplyGate {
 // C ABI
 // =============================================================================================
 
+// Side-by-side FGMRES solves of nrhs right-hand sides with one ApplyGate (shared applies).
+static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, const void *const *b, void *const *x,
+                        xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
+{
+    ApplyGate gate(op, nrhs);
+    std::vector<std::unique_ptr<SolveCtx>> ctxs;
+    std::vector<hipStream_t> streams(nrhs, nullptr);
+    std::vector<hipEvent_t> ready(nrhs, nullptr);
+    std::vector<std::exception_ptr> errs(nrhs);
+    std::mutex cb_mu;
+    struct CbUser { xtr_iteration_fn fn; void *user; std::mutex *mu; };
+    CbUser cbu{on_iteration, user, &cb_mu};
+    auto cb = [](void *u, int r, int it, double bea) {
+        auto *c = (CbUser *)u;
+        std::lock_guard<std::mutex> lk(*c->mu);
+        c->fn(c->user, r, it, bea);
+    };
+    try {
+        for (int r = 0; r < nrhs; ++r) {
+            XTR_CHECK(hipStreamCreateWithFlags(&streams[r], hipStreamDefault));
+            XTR_CHECK(hipEventCreateWithFlags(&ready[r], hipEventDisableTiming));
+            auto c = std::make_unique<SolveCtx>();
+            c->stream = streams[r];
+            c->rhs_index = r;
+            if (on_iteration) {
+                c->on_iter = cb;
+                c->user = &cbu;
+            }
+            hipStream_t s = streams[r];
+            hipEvent_t ev = ready[r];
+            ApplyGate *gp = &gate;
+            c->apply = [gp, s, ev](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev); };
+            ctxs.push_back(std::move(c));
+        }
+        // inputs written on the caller's default stream must be visible to the solve streams
+        XTR_CHECK(hipDeviceSynchronize());
+        std::vector<std::thread> th;
+        for (int r = 0; r < nrhs; ++r) {
+            th.emplace_back([&, r] {
+                try {
+                    std::memset(&info[r], 0, sizeof(info[r]));
+                    run_fgmres(op->g.n(), *params, *ctxs[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
+                } catch (...) {
+                    errs[r] = std::current_exception();
+                }
+                gate.leave();
+            });
+        }
+        for (auto &t : th) t.join();
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    } catch (...) {
+        for (auto &s : streams)
+            if (s) (void)hipStreamSynchronize(s);
+        for (auto &e : ready)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &s : streams)
+            if (s) (void)hipStreamDestroy(s);
+        throw;
+    }
+    for (auto &s : streams) (void)hipStreamSynchronize(s);
+    for (auto &e : ready) (void)hipEventDestroy(e);
+    for (auto &s : streams) (void)hipStreamDestroy(s);
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
 extern "C" {
 
 int xtr_last_error(char *buf, int len)
@@ -580,7 +649,8 @@ int xtr_op_create(const xtr_op_desc *d, xtr_op **out)
         require(3 * d->nz * 1 <= 1024, "Nz too large for the contraction kernel");
         require(xtr::fft_length_supported(2 * d->nx) && xtr::fft_length_supported(2 * d->ny),
                 "2*Nx and 2*Ny must be one of 4,6,8,12,16,24,32,48,64,96,128,192,256,384,512");
-        require(d->precision == XTR_FP64 || d->precision == XTR_FP32, "precision must be XTR_FP64 or XTR_FP32");
+        require(d->precision == XTR_FP64 || d->precision == XTR_FP32 || d->precision == XTR_MIXED,
+                "precision must be XTR_FP64, XTR_FP32 or XTR_MIXED");
         require(d->max_rhs >= 1 && d->max_rhs <= xtr::kMaxRhs, "max_rhs must be 1..8");
         require(3 * d->nz * d->max_rhs <= 1024, "3*Nz*max_rhs must not exceed 1024");
         require(d->nlayers >= 3, "the section needs the air, at least one layer and the lower half-space");
@@ -635,11 +705,11 @@ void xtr_op_destroy(xtr_op *op)
 
 static void finish_green(xtr_op *op)
 {
-    if (op->precision == XTR_FP32) {
+    if (op->precision == XTR_FP32 || op->precision == XTR_MIXED) {
         op->G32.alloc(op->G64.count);
         xtr::launch_to_f32(op->G64.p, op->G32.p, (int64_t)op->G64.count, op->stream);
         XTR_CHECK(hipStreamSynchronize(op->stream));
-        op->G64.release();
+        if (op->precision == XTR_FP32) op->G64.release();
     }
     op->green_ready = true;
 }
@@ -838,7 +908,8 @@ int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_b
     return guarded([&] {
         require(op, "null argument");
         if (n) *n = op->g.n();
-        if (green_bytes) *green_bytes = (int64_t)(op->g.L() * op->g.gblk() * op->elem());
+        const int64_t per = op->precision == XTR_FP64 ? 16 : op->precision == XTR_FP32 ? 8 : 24;
+        if (green_bytes) *green_bytes = (int64_t)(op->g.L() * op->g.gblk() * per);
         if (buffer_bytes) *buffer_bytes = (int64_t)(op->g.M() * op->elem());
     });
 }
@@ -887,66 +958,100 @@ int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, in
         require(op && params && b && x && info, "null argument");
         require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
         require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs must be within 1..max_rhs");
-        ApplyGate gate(op, nrhs);
-        std::vector<std::unique_ptr<SolveCtx>> ctxs;
-        std::vector<hipStream_t> streams(nrhs, nullptr);
-        std::vector<hipEvent_t> ready(nrhs, nullptr);
-        std::vector<std::exception_ptr> errs(nrhs);
-        std::mutex cb_mu;
-        struct CbUser { xtr_iteration_fn fn; void *user; std::mutex *mu; };
-        CbUser cbu{on_iteration, user, &cb_mu};
-        auto cb = [](void *u, int r, int it, double bea) {
-            auto *c = (CbUser *)u;
-            std::lock_guard<std::mutex> lk(*c->mu);
-            c->fn(c->user, r, it, bea);
-        };
-        try {
-            for (int r = 0; r < nrhs; ++r) {
-                XTR_CHECK(hipStreamCreateWithFlags(&streams[r], hipStreamDefault));
-                XTR_CHECK(hipEventCreateWithFlags(&ready[r], hipEventDisableTiming));
-                auto c = std::make_unique<SolveCtx>();
-                c->stream = streams[r];
-                c->rhs_index = r;
-                if (on_iteration) {
-                    c->on_iter = cb;
-                    c->user = &cbu;
-                }
-                hipStream_t s = streams[r];
-                hipEvent_t ev = ready[r];
-                ApplyGate *gp = &gate;
-                c->apply = [gp, s, ev](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev); };
-                ctxs.push_back(std::move(c));
-            }
-            // inputs written on the caller's default stream must be visible to the solve streams
-            XTR_CHECK(hipDeviceSynchronize());
-            std::vector<std::thread> th;
-            for (int r = 0; r < nrhs; ++r) {
-                th.emplace_back([&, r] {
-                    try {
-                        std::memset(&info[r], 0, sizeof(info[r]));
-                        run_fgmres(op->g.n(), *params, *ctxs[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
-                    } catch (...) {
-                        errs[r] = std::current_exception();
-                    }
-                    gate.leave();
-                });
-            }
-            for (auto &t : th) t.join();
-            XTR_CHECK(hipStreamSynchronize(op->stream));
-        } catch (...) {
-            for (auto &s : streams)
-                if (s) (void)hipStreamSynchronize(s);
-            for (auto &e : ready)
-                if (e) (void)hipEventDestroy(e);
-            for (auto &s : streams)
-                if (s) (void)hipStreamDestroy(s);
-            throw;
+        solve_batch(op, params, nrhs, b, x, info, on_iteration, user);
+    });
+}
+
+
+int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const void *const *b,
+                                   void *const *x, xtr_refine_info *info)
+{
+    return guarded([&] {
+        require(op && params && b && x && info, "null argument");
+        require(op->precision == XTR_MIXED, "the refined solve needs an operator created with XTR_MIXED");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs must be within 1..max_rhs");
+        require(params->tolerance > 0 && params->max_refinements >= 1, "tolerance > 0 and max_refinements >= 1");
+        const int64_t n = op->g.n();
+        const auto t0 = std::chrono::steady_clock::now();
+        XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
+        SolveCtx ctx;                        // reduction scratch for the FP64 norms
+        ctx.stream = op->stream;
+        std::vector<DevBuf<double2>> r(nrhs), d(nrhs);
+        for (int k = 0; k < nrhs; ++k) {
+            r[k].alloc(n);
+            d[k].alloc(n);
         }
-        for (auto &s : streams) (void)hipStreamSynchronize(s);
-        for (auto &e : ready) (void)hipEventDestroy(e);
-        for (auto &s : streams) (void)hipStreamDestroy(s);
-        for (auto &e : errs)
-            if (e) std::rethrow_exception(e);
+        auto norm = [&](const double2 *v) {
+            xtr::launch_norm2(v, n, ctx.partial_r.p, ctx.stream);
+            xtr::launch_reduce_r(ctx.partial_r.p, ctx.norm_dev.p, ctx.stream);
+            XTR_CHECK(hipMemcpyAsync(&ctx.h_host.p[32], ctx.norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx.stream));
+            XTR_CHECK(hipStreamSynchronize(ctx.stream));
+            double nn = 0;
+            std::memcpy(&nn, &ctx.h_host.p[32], sizeof(double));
+            return std::sqrt(nn);
+        };
+        std::vector<double> bn(nrhs), rn(nrhs, 0.0);
+        std::vector<int> live;
+        for (int k = 0; k < nrhs; ++k) {
+            std::memset(&info[k], 0, sizeof(info[k]));
+            bn[k] = norm((const double2 *)b[k]);
+            live.push_back(k);
+        }
+        struct ActiveGuard {
+            xtr_op *o;
+            ~ActiveGuard() { o->active = XTR_FP64; }
+        } guard{op};
+        for (int step = 0;; ++step) {
+            // FP64 true residuals r = b - A x of the unconverged right-hand sides, one batched apply
+            op->active = XTR_FP64;
+            xtr::RhsPtrs v{};
+            for (size_t i = 0; i < live.size(); ++i) {
+                v.x[i] = (const double2 *)x[live[i]];
+                v.y[i] = r[live[i]].p;
+            }
+            op->apply_raw(v, (int)live.size(), op->stream);
+            std::vector<int> next;
+            for (int k : live) {
+                info[k].matvecs64++;
+                xtr::launch_sub((const double2 *)b[k], r[k].p, r[k].p, n, op->stream);
+                rn[k] = bn[k] > 0 ? norm(r[k].p) : 0.0;
+                info[k].residual = bn[k] > 0 ? rn[k] / bn[k] : 0.0;
+                if (info[k].residual <= params->tolerance) info[k].converged = 1;
+                else if (step < params->max_refinements) next.push_back(k);
+            }
+            live.swap(next);
+            if (live.empty()) break;
+            // complex64 FGMRES corrections A d = r, side by side, from d = 0
+            op->active = XTR_FP32;
+            std::vector<const void *> rb;
+            std::vector<void *> dx;
+            xtr_gmres_params ip = params->inner;
+            double tol_in = 1.0;
+            for (int k : live) {
+                XTR_CHECK(hipMemsetAsync(d[k].p, 0, n * sizeof(double2), op->stream));
+                rb.push_back(r[k].p);
+                dx.push_back(d[k].p);
+                tol_in = std::min(tol_in, std::max(params->inner_floor, 0.5 * params->tolerance * bn[k] / rn[k]));
+            }
+            ip.tolerance = tol_in;
+            XTR_CHECK(hipStreamSynchronize(op->stream));
+            std::vector<xtr_result_info> ii(live.size());
+            solve_batch(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
+            for (size_t i = 0; i < live.size(); ++i) {
+                const int k = live[i];
+                info[k].refinements++;
+                info[k].matvecs32 += ii[i].matvecs;
+                info[k].inner_iterations += ii[i].number_of_iterations;
+                xtr::Coef32 one{};
+                one.c[0] = make_double2(1.0, 0.0);
+                xtr::launch_update_solution(d[k].p, 1, one, (double2 *)x[k], n, op->stream);   // x += d
+            }
+        }
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int k = 0; k < nrhs; ++k) info[k].seconds = secs;
     });
 }
 

@@ -148,3 +148,10 @@ class FrequencySolve:
         self.chi.copy_(self.chi0)
         self.solver.history.clear()
         return self.solver.solve_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]])
+
+    def solve_mixed(self, inner_floor=1e-8, max_refinements=10):
+        """The same two solves in mixed precision (needs precision="mixed"): chi <- chi0, then FP64
+        refinement around complex64 FGMRES until the FP64 true residual reaches the tolerance."""
+        self.chi.copy_(self.chi0)
+        return self.solver.solve_refined_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]],
+                                               inner_floor=inner_floor, max_refinements=max_refinements)

@@ -24,7 +24,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxtreme_hip.so")
 _lib = None
 
-XTR_FP64, XTR_FP32 = 0, 1
+XTR_FP64, XTR_FP32, XTR_MIXED = 0, 1, 2
+_PRECISION = {"fp64": XTR_FP64, "fp32": XTR_FP32, "mixed": XTR_MIXED}
 XTR_OP_A, XTR_OP_CHI0 = 0, 1
 
 
@@ -49,6 +50,17 @@ class _Params(C.Structure):
 class _Info(C.Structure):
     _fields_ = [("converged", C.c_int), ("arnoldi_backward_error", C.c_double), ("backward_error", C.c_double),
                 ("number_of_iterations", C.c_int), ("matvecs", C.c_int64), ("dot_products", C.c_int64),
+                ("seconds", C.c_double)]
+
+
+class _RefineParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("inner_floor", C.c_double), ("max_refinements", C.c_int),
+                ("inner", _Params)]
+
+
+class _RefineInfo(C.Structure):
+    _fields_ = [("converged", C.c_int), ("residual", C.c_double), ("refinements", C.c_int),
+                ("matvecs64", C.c_int64), ("matvecs32", C.c_int64), ("inner_iterations", C.c_int),
                 ("seconds", C.c_double)]
 
 
@@ -77,6 +89,9 @@ def lib():
         L.xtr_fgmres_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_int,
                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                     C.POINTER(_Info), ITER_FN, C.c_void_p]
+        L.xtr_refined_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_RefineParams), C.c_int,
+                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                                     C.POINTER(_RefineInfo)]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -107,6 +122,24 @@ class ForwardSettings:
     outer_buffer_length: int = 30
     max_repeats_number: int = 100
     number_of_hankels: int = 10
+
+
+@dataclass
+class RefineInfo:
+    """Result of one mixed-precision solve: the FP64 true relative residual it stopped on and the
+    applies it spent in each precision."""
+    converged: bool
+    residual: float
+    refinements: int
+    matvecs64: int
+    matvecs32: int
+    inner_iterations: int
+    seconds: float
+
+    @staticmethod
+    def _from(i) -> "RefineInfo":
+        return RefineInfo(bool(i.converged), i.residual, i.refinements, i.matvecs64, i.matvecs32,
+                          i.inner_iterations, i.seconds)
 
 
 @dataclass
@@ -162,7 +195,7 @@ class ConvolutionOperator:
         d = _Desc(model.nx, model.ny, model.nz, float(model.dx), float(model.dy), arrs["depth"], arrs["thick"],
                   model.section.nlayers, float(model.section.zero_level), arrs["lt"], arrs["ls"], self.omega,
                   int(self.settings.number_of_hankels), arrs["h0"], arrs["h1"], len(h0),
-                  XTR_FP64 if precision == "fp64" else XTR_FP32, int(max_rhs))
+                  _PRECISION[precision], int(max_rhs))
         h = C.c_void_p()
         _check(lib().xtr_op_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -287,3 +320,15 @@ class AnomalyCurrentFgmresSolver:
         cb = self._cb()
         _check(lib().xtr_fgmres_solve_batch_device(self.op._h, C.byref(self.params), n, bp, xp, infos, cb, None))
         return [ResultInfo._from(infos[i]) for i in range(n)]
+
+    def solve_refined_batch(self, bs, xs, tolerance=None, inner_floor=1e-8, max_refinements=10) -> list:
+        """Mixed precision (operator created with precision="mixed"): FP64 iterative refinement
+        around complex64 FGMRES corrections, stopping on the FP64 true residual. Device tensors."""
+        n = len(bs)
+        infos = (_RefineInfo * n)()
+        bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
+        xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        tol = float(self.params.tolerance if tolerance is None else tolerance)
+        rp = _RefineParams(tol, float(inner_floor), int(max_refinements), self.params)
+        _check(lib().xtr_refined_solve_batch_device(self.op._h, C.byref(rp), n, bp, xp, infos))
+        return [RefineInfo._from(infos[i]) for i in range(n)]

@@ -32,7 +32,9 @@ typedef struct xtr_op xtr_op;
 /* Arithmetic of the operator apply. */
 enum {
     XTR_FP64 = 0,   /* complex128 Green tensor, FFT and contraction (the reference's arithmetic) */
-    XTR_FP32 = 1    /* complex64 Green tensor, FFT and contraction; Krylov vectors stay FP64 */
+    XTR_FP32 = 1,   /* complex64 Green tensor, FFT and contraction; Krylov vectors stay FP64 */
+    XTR_MIXED = 2   /* both tensors kept: FP64 applies by default, complex64 applies inside
+                       xtr_refined_solve_batch_device's correction solves */
 };
 
 /* Operator type (Extreme.Cartesian/Forward/AtoA/OperatorType.cs). */
@@ -142,6 +144,36 @@ int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const vo
 int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, int nrhs,
                                   const void *const *b, void *const *x, xtr_result_info *info,
                                   xtr_iteration_fn on_iteration, void *user);
+
+/* Mixed-precision solve: FP64 iterative refinement around complex64 FGMRES correction solves.
+ * No counterpart in the reference (its arithmetic is complex128 throughout); it solves the same
+ * system A x = b as xtr_fgmres_solve_batch_device and stops on the FP64 true residual. */
+typedef struct {
+    double tolerance;             /* stop when |b - A x| / |b| <= tolerance, A applied in FP64 */
+    double inner_floor;           /* smallest FGMRES tolerance asked of a correction solve */
+    int max_refinements;          /* correction solves per right-hand side (>= 1) */
+    xtr_gmres_params inner;       /* FGMRES(m) / GMRES(m_in) settings of the correction solves;
+                                     inner.tolerance is ignored (set per refinement) */
+} xtr_refine_params;
+
+typedef struct {
+    int converged;
+    double residual;              /* final FP64 true relative residual */
+    int refinements;              /* correction solves run */
+    int64_t matvecs64;            /* FP64 applies (true residuals) */
+    int64_t matvecs32;            /* complex64 applies (inside the correction solves) */
+    int inner_iterations;         /* outer FGMRES iterations summed over the correction solves */
+    double seconds;
+} xtr_refine_info;
+
+/* Solves A x[r] = b[r] for nrhs right-hand sides side by side, x holding the initial guess. The
+ * operator must be created with XTR_MIXED and prepared as XTR_OP_A. Each refinement computes
+ * r = b - A x with the FP64 tensor (one batched apply), solves A d = r by FGMRES with the complex64
+ * tensor to max(inner_floor, tolerance |b| / (2 |r|)), and adds d to x.
+ * Replaces, for callers that accept it: AnomalyCurrentFgmresSolver.Solve
+ * (Extreme.Cartesian/Forward/AnomalyCurrentFgmresSolver.cs:59-85) for both polarisations. */
+int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs,
+                                   const void *const *b, void *const *x, xtr_refine_info *info);
 
 /* Device time of the operator applies since xtr_op_set_profiling(op, 1), summed by stage, in
  * milliseconds: out[0] forward pass along y (with R and f fused), out[1] forward pass along x,

@@ -144,3 +144,31 @@ def test_green_set_get_roundtrip(oracle, torch_cuda, nz):
     back = op.green_tensor()
     for c in oracle.COMPONENTS:
         assert np.array_equal(back[c].reshape(G[c].shape), G[c]), c
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 32), (24, 12, 32)])
+def test_apply_nz32_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
+    """complex64 streaming contraction (4-buffer ring): within the FP32 bound of the FP64 oracle and
+    equal to the complex64 generic kernel up to summation order."""
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nx * ny)
+    w = cm.omega_of(m.frequencies[0])
+    G = _random_green(oracle, m, seed=nx + 7)
+    op = X.ConvolutionOperator(m, w, precision="fp32", max_rhs=2)
+    op.set_green_tensor(G)
+    oop = oracle.Operator(m, G)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator("A", z)
+    oop.prepare("A", z, z0)
+    xs = np.random.default_rng(8).standard_normal((2, m.n)) + 1j * np.random.default_rng(9).standard_normal((2, m.n))
+    ref = np.stack([oop.apply(xs[0]), oop.apply(xs[1])])
+    y2 = op.apply(xs)
+    y1 = op.apply(xs[1])
+    monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
+    yg = op.apply(xs)
+    monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+    print(f"fp32 nz32 {shape}: vs oracle {rel(y2, ref):.2e} nrhs1 {rel(y1, ref[1]):.2e} "
+          f"vs generic {rel(y2, yg):.2e}", flush=True)
+    assert rel(y2, ref) < TOL_FP32
+    assert rel(y1, ref[1]) < TOL_FP32
+    assert rel(y2, yg) < 1e-6
