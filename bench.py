@@ -33,9 +33,9 @@ METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s
 
 # HBM bytes per launch of the contraction kernel, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one counter per run) and scaled by the same counters on a 1 GiB device copy
-# (FETCH x 2.0000, WRITE x 1.0000); summary in profiles/contract_kernel_c2_hbm_traffic.md.
+# (FETCH x 2.0000, WRITE x 1.0000); summary in profiles/contract_stream_kernel_c2_hbm_traffic.md.
 # Keyed by (workload, nz, right-hand sides per launch); a workload without an entry reports null.
-MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9}
+MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9}   # contract_stream_kernel<double, 2>
 
 
 def main():
@@ -176,7 +176,7 @@ def main():
         "true_backward_error": be,
         "converged": conv,
         "green_seconds_per_freq": float(np.mean(green_s)),
-        "roofline": {"kernel": "contract_kernel<double>", "bound": "hbm", "achieved": achieved_gbs,
+        "roofline": {"kernel": "contract_stream_kernel<double, 2>" if m.nz == 32 else "contract_kernel<double>", "bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
                      "avg_launch_ms": contract_avg_ms,
