@@ -88,6 +88,10 @@ def lib():
                                      C.POINTER(Result)]
         L.xo_zrotg.argtypes = [CDouble, CDouble, C.POINTER(C.c_double), C.POINTER(CDouble)]
         L.xo_set_threads.argtypes = [C.c_int]
+        L.xo_mu0.restype = C.c_double
+        L.xo_green_atoo_level.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, C.c_int, f64p, f64p,
+                                          C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.xo_atoo_field.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_int, c128p, c128p]
         L.xo_log10_rho_step.restype = C.c_double
         L.xo_hankel_lambdas.restype = C.c_int
         L.xo_hankel_lambdas.argtypes = [C.c_double, C.c_int, f64p]
@@ -246,6 +250,62 @@ def green_atoa(model, omega, number_of_hankels=10, fft=True):
     if rc:
         raise RuntimeError(f"xo_green_atoa failed: {rc}")
     return {c: v.reshape(L, -1) for c, v in out.items()}
+
+
+AO_COMPONENTS = ("xx", "yy", "zz", "xy", "yx", "xz", "yz", "zx", "zy")
+AO_ELECTRIC = ("xx", "yy", "zz", "xy", "xz", "yz", "zx", "zy")
+AO_MAGNETIC = ("xx", "xy", "yx", "xz", "yz", "zx", "zy")
+
+
+def green_atoo_level(model, omega, level_z=0.0, electric=True, number_of_hankels=10, fft=True):
+    """AtoO level tensor (AtoOCalculator.CalculateGreenTensorAtoOLevel{Electric,Magnetic}):
+    dict comp -> [2Nx*2Ny, Nz] for the components the reference computes."""
+    h0, h1 = hankel_table()
+    k = _Keep()
+    s = make_section(model.section, k)
+    g = make_grid(model, k)
+    L = 4 * model.nx * model.ny
+    comps = AO_ELECTRIC if electric else AO_MAGNETIC
+    out = {c: np.zeros(L * model.nz, dtype=np.complex128) for c in comps}
+    ptrs = (C.c_void_p * 9)(*[out[c].ctypes.data if c in out else None for c in AO_COMPONENTS])
+    rc = lib().xo_green_atoo_level(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1, float(level_z),
+                                   1 if electric else 0, 1 if fft else 0, ptrs)
+    if rc:
+        raise RuntimeError(f"xo_green_atoo_level failed: {rc}")
+    return {c: v.reshape(L, model.nz) for c, v in out.items()}
+
+
+def atoo_field(model, green, electric, jq):
+    """Anomalous E (electric) or H field at the level, [Nx, Ny, 3] (ToOCalculator.cs:21-45)."""
+    keep = {c: np.ascontiguousarray(green[c], dtype=np.complex128).reshape(-1) for c in green}
+    ptrs = (C.c_void_p * 9)(*[keep[c].ctypes.data if c in keep else None for c in AO_COMPONENTS])
+    jq = np.ascontiguousarray(jq, dtype=np.complex128).reshape(-1)
+    out = np.zeros(model.nx * model.ny * 3, dtype=np.complex128)
+    lib().xo_atoo_field(model.nx, model.ny, model.nz, ptrs, 1 if electric else 0, jq, out)
+    return out.reshape(model.nx, model.ny, 3)
+
+
+def impedance(e1, e2, h1, h2):
+    """ResponseFunctionsCalculator.CalculateTensor (ResponseFunctionsCalculator.cs:36-51) per site:
+    Z = [E1 E2] [H1 H2]^-1 from the two polarisations' 3-vectors; arrays [..., 3] -> [..., 2, 2]."""
+    det = h1[..., 0] * h2[..., 1] - h2[..., 0] * h1[..., 1]
+    hi11, hi12, hi21, hi22 = h2[..., 1] / det, -h2[..., 0] / det, -h1[..., 1] / det, h1[..., 0] / det
+    z = np.empty(e1.shape[:-1] + (2, 2), dtype=np.complex128)
+    z[..., 0, 0] = e1[..., 0] * hi11 + e2[..., 0] * hi21
+    z[..., 0, 1] = e1[..., 0] * hi12 + e2[..., 0] * hi22
+    z[..., 1, 0] = e1[..., 1] * hi11 + e2[..., 1] * hi21
+    z[..., 1, 1] = e1[..., 1] * hi12 + e2[..., 1] * hi22
+    return z
+
+
+def rho_phase(z, omega):
+    """PlainTextExporter.CalculateApparentResistivity / ProcessPhase (:360-371, 417-420): rho_a =
+    |Z|^2 / (omega mu0), phase = arg Z in radians."""
+    return np.abs(z) ** 2 / (omega * xo_mu0()), np.angle(z)
+
+
+def xo_mu0():
+    return lib().xo_mu0()
 
 
 def contract(nz, green, src):

@@ -769,3 +769,430 @@ int xo_green_atoa(const xo_section *s, const xo_grid *g, double omega, int numbe
     xo_plan_free(&plan);
     return rc;
 }
+
+/* ====================================================================== */
+/* AtoO: anomaly (volume transmitters) to an observation level (points)   */
+/* ====================================================================== */
+
+/* VolumeToPointCalculatorU (VolumeToPointCalculatorU.cs:34-337): U1..U5 of transmitter layer
+ * [z1, z2] (background layer s) seen at the point zw (background layer r), at lambda index m,
+ * with the P/Q set `sigma` (0: P1/Q1 with A1, 1: PSigma/QSigma with ASigma). */
+typedef struct { xo_c ann, er, es, et, eb, pt, qb, pn, qn; double dt, db1; } pu_ctx;
+
+static xo_c ctop(const xo_c *eta, int nlam, int m, double zw, double z1, double z2, int r, int s)
+{
+    xo_c er = eta[r * nlam + m], es = eta[s * nlam + m];
+    return cexp(-es * z1 + er * zw) - cexp(-es * z2 + er * zw);
+}
+static xo_c cbot(const xo_c *eta, int nlam, int m, double zw, double z1, double z2, int r, int s)
+{
+    xo_c er = eta[r * nlam + m], es = eta[s * nlam + m];
+    return cexp(-er * zw + es * z2) - cexp(-er * zw + es * z1);
+}
+static xo_c dtop(const xo_section *sec, const xo_c *eta, int nlam, int m, double zw, double z1, double z2, int r, int s)
+{
+    int t = r < s ? r : s, b = r > s ? r : s;
+    double dt = layer_depth(sec, t), db1 = layer_depth(sec, b + 1);
+    xo_c er = eta[r * nlam + m], es = eta[s * nlam + m], et = eta[t * nlam + m], eb = eta[b * nlam + m];
+    xo_c base = -2 * (eb * db1 - et * dt);
+    return cexp(base + es * z2 - er * zw) - cexp(base + es * z1 - er * zw);
+}
+static xo_c dbot(const xo_section *sec, const xo_c *eta, int nlam, int m, double zw, double z1, double z2, int r, int s)
+{
+    int t = r < s ? r : s, b = r > s ? r : s;
+    double dt = layer_depth(sec, t), db1 = layer_depth(sec, b + 1);
+    xo_c er = eta[r * nlam + m], es = eta[s * nlam + m], et = eta[t * nlam + m], eb = eta[b * nlam + m];
+    xo_c base = -2 * (eb * db1 - et * dt);
+    return cexp(base - es * z1 + er * zw) - cexp(base - es * z2 + er * zw);
+}
+
+/* which: bit0 U1, bit1 U2, bit2 U3, bit3 U4, bit4 U5 */
+static void point_u(const xo_section *sec, double omega, const item_aux *a, int m, double lambda, int r, int s,
+                    double zw, double z1, double z2, int sigma, int which, xo_c *u)
+{
+    int nlam = a->nlam;
+    const xo_c *P = sigma ? a->ps : a->p1, *Q = sigma ? a->qs : a->q1, *eta = a->eta;
+    int t = r < s ? r : s, b = r > s ? r : s;
+    xo_c ann = ann_at(sec, omega, a, P, Q, r, s, m, sigma);
+    xo_c er = eta[r * nlam + m], es = eta[s * nlam + m], et = eta[t * nlam + m], eb = eta[b * nlam + m];
+    double dt = layer_depth(sec, t), db1 = layer_depth(sec, b + 1);
+    /* CalculateT / CalculateR */
+    xo_c T = cexp(-er * zw - es * z1 + 2 * et * dt) - cexp(-er * zw - es * z2 + 2 * et * dt);
+    xo_c R = cexp(-2 * eb * db1 + es * z2 + er * zw) - cexp(-2 * eb * db1 + es * z1 + er * zw);
+    xo_c pt = P[t * nlam + m], qb = Q[b * nlam + m];
+    int kase = (zw > z1 && zw < z2) ? 0 : (zw >= z2 ? 1 : 2);   /* CalcUGeneral: match, below, above */
+    xo_c psi1, psi3;
+    if (kase == 0) {
+        int n = r;
+        xo_c pq = P[n * nlam + m] * Q[n * nlam + m];
+        psi1 = ctop(eta, nlam, m, zw, zw, z2, n, n) + cbot(eta, nlam, m, zw, z1, zw, n, n) +
+               pq * (dtop(sec, eta, nlam, m, zw, zw, z2, n, n) + dbot(sec, eta, nlam, m, zw, z1, zw, n, n));
+        psi3 = ctop(eta, nlam, m, zw, zw, z2, n, n) - cbot(eta, nlam, m, zw, z1, zw, n, n) +
+               pq * (-dtop(sec, eta, nlam, m, zw, zw, z2, n, n) + dtop(sec, eta, nlam, m, zw, z1, zw, n, n));
+    } else if (kase == 1) {
+        psi1 = cbot(eta, nlam, m, zw, z1, z2, r, s) + pt * qb * dbot(sec, eta, nlam, m, zw, z1, z2, r, s);
+        psi3 = -cbot(eta, nlam, m, zw, z1, z2, r, s) + pt * qb * dbot(sec, eta, nlam, m, zw, z1, z2, r, s);
+    } else {
+        psi1 = ctop(eta, nlam, m, zw, z1, z2, r, s) + pt * qb * dtop(sec, eta, nlam, m, zw, z1, z2, r, s);
+        psi3 = ctop(eta, nlam, m, zw, z1, z2, r, s) - pt * qb * dtop(sec, eta, nlam, m, zw, z1, z2, r, s);
+    }
+    if (which & 1) u[0] = (ann / es) * (pt * T + qb * R + psi1);                 /* CalculateU1And5Special */
+    if (which & 2) u[1] = (ann * er) * (pt * T + qb * R - psi1);                 /* CalculateU2Special */
+    if (which & 4) u[2] = (ann * (er / es)) * (-pt * T + qb * R + psi3);         /* CalculateU3Special */
+    if (which & 8) u[3] = ann * (-pt * T + qb * R - psi3);                       /* CalculateU4Special */
+    if (which & 16) {
+        if (kase == 0) {                                                         /* CalculateU5SpecialForTrAndRcMatch */
+            int n = r;
+            double ln = sec->thickness[n];
+            xo_c kk = I * omega * xo_zeta(sec->sigma[n], omega) * xo_mu0();
+            xo_c en = eta[n * nlam + m], pq = P[n * nlam + m] * Q[n * nlam + m];
+            xo_c psi5 = pq * (cexp(-en * (2 * ln - (z2 - zw))) + cexp(-en * (2 * ln - (zw - z1)))) -
+                        (cexp(-en * (z2 - zw)) + cexp(-en * (zw - z1))) +
+                        2 * kk * ((1 - pq * cexp(-2 * en * ln)) / (lambda * lambda));
+            u[4] = (ann / es) * (pt * T + qb * R + psi5);
+        } else {
+            u[4] = (ann / es) * (pt * T + qb * R + psi1);
+        }
+    }
+}
+
+/* HankelCoefficients.ConvoluteWith (:86-99), the non-fast form used by PlanCalculator:
+ * fr[i] = sum_{j ascending} fk[i + 430 - j] * hank[j] */
+static xo_c hankel_dot_ascending(const double *hank, const xo_c *fk, int i)
+{
+    double re = 0, im = 0;
+    for (int j = 0; j < NTAP; j++) {
+        re += creal(fk[i + NTAP - 1 - j]) * hank[j];
+        im += cimag(fk[i + NTAP - 1 - j]) * hank[j];
+    }
+    return re + I * im;
+}
+
+/* Scalars I1..I5 (electric) or I1..I4 (magnetic) of the nz transmitter layers seen at the level
+ * depth zl, on the merged rho grid: out[c][tr][rho] (PlanCalculator.CalculateForE/H :58-141,
+ * ScalarMathUtils.NormalizeAndPerformHankel :76-92, NormalizeBefore/AfterHankelForE/H). */
+static int atoo_scalars(const xo_section *sec, const xo_grid *g, double omega, const double *hank0,
+                        const double *hank1, const xo_plan *plan, double zl, int electric, xo_c *out)
+{
+    int nz = g->nz, nrho = plan->nrho_merged;
+    int ncomp = electric ? 5 : 4;
+    int r = xo_layer_at_depth(sec, zl);
+    if (r < 0) return -1;
+    double mu0 = xo_mu0();
+    xo_c zrc2 = 2 * xo_zeta(sec->sigma[r], omega);
+    xo_c iwm = I * omega * mu0;
+    item_aux *aux = malloc(sizeof(item_aux) * plan->nitems);
+    for (int it = 0; it < plan->nitems; it++) item_aux_build(sec, omega, plan->lam[it], plan->nlam[it], &aux[it]);
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int tr = 0; tr < nz; tr++) {
+        double z1 = g->depth[tr], z2 = g->depth[tr] + g->thick[tr];
+        int s = xo_layer_at_depth(sec, z1 + (z2 - z1) / 2);   /* transmitter working depth */
+        if (s < 0) { err = 1; continue; }
+        for (int it = 0; it < plan->nitems; it++) {
+            int nlam = plan->nlam[it], nr = plan->nrho[it];
+            const double *lam = plan->lam[it];
+            xo_c *U = malloc(sizeof(xo_c) * (size_t)nlam * ncomp);   /* [comp][lam], comp = I index - 1 */
+            for (int m = 0; m < nlam; m++) {
+                xo_c u1[5], us[5];
+                if (electric) {
+                    point_u(sec, omega, &aux[it], m, lam[m], r, s, zl, z1, z2, 0, 1, u1);
+                    point_u(sec, omega, &aux[it], m, lam[m], r, s, zl, z1, z2, 1, 2 | 4 | 8 | 16, us);
+                    U[0 * nlam + m] = u1[0] * (iwm / 2);                        /* I1 <- U1 (A1 set) */
+                    U[1 * nlam + m] = us[1] / zrc2;                             /* I2 <- U2 */
+                    U[2 * nlam + m] = us[2] * (lam[m] / zrc2);                  /* I3 <- U3 */
+                    U[3 * nlam + m] = us[3] * (-lam[m] / zrc2);                 /* I4 <- U4 */
+                    U[4 * nlam + m] = us[4] * ((lam[m] * lam[m]) / zrc2);       /* I5 <- U5 */
+                } else {
+                    point_u(sec, omega, &aux[it], m, lam[m], r, s, zl, z1, z2, 0, 1 | 4, u1);
+                    point_u(sec, omega, &aux[it], m, lam[m], r, s, zl, z1, z2, 1, 1 | 8, us);
+                    U[0 * nlam + m] = u1[2];                                    /* I1 <- U31 */
+                    U[1 * nlam + m] = us[3];                                    /* I2 <- U4Sigma */
+                    U[2 * nlam + m] = us[0] * lam[m];                           /* I3 <- U1Sigma * lambda */
+                    U[3 * nlam + m] = u1[0] * lam[m];                           /* I4 <- U11 * lambda */
+                }
+            }
+            xo_c *Ir = malloc(sizeof(xo_c) * (size_t)nr * ncomp);
+            for (int i = 0; i < nr; i++) {
+                double rho = plan->rho[it][i];
+                if (electric) {
+                    double d = 2 * XO_PI * rho;
+                    Ir[0 * nr + i] = -hankel_dot_ascending(hank1, U + 0 * nlam, i) / d;
+                    Ir[1 * nr + i] = hankel_dot_ascending(hank1, U + 1 * nlam, i) / d;
+                    Ir[2 * nr + i] = hankel_dot_ascending(hank0, U + 2 * nlam, i) / d;
+                    Ir[3 * nr + i] = hankel_dot_ascending(hank0, U + 3 * nlam, i) / d;
+                    Ir[4 * nr + i] = hankel_dot_ascending(hank1, U + 4 * nlam, i) / d;
+                } else {
+                    double d = 2 * (2 * XO_PI) * rho;
+                    Ir[0 * nr + i] = -hankel_dot_ascending(hank1, U + 0 * nlam, i) / d;
+                    Ir[1 * nr + i] = -hankel_dot_ascending(hank1, U + 1 * nlam, i) / d;
+                    Ir[2 * nr + i] = -hankel_dot_ascending(hank0, U + 2 * nlam, i) / d;
+                    Ir[3 * nr + i] = hankel_dot_ascending(hank0, U + 3 * nlam, i) / d;
+                }
+            }
+            free(U);
+            for (int k = 0; k < nrho; k++)
+                if (plan->merge_item[k] == it)
+                    for (int c = 0; c < ncomp; c++)
+                        out[((int64_t)c * nz + tr) * nrho + k] = Ir[c * nr + plan->merge_index[k]];
+            free(Ir);
+        }
+    }
+    for (int it = 0; it < plan->nitems; it++) item_aux_free(&aux[it]);
+    free(aux);
+    return err ? -1 : 0;
+}
+
+/* component order of the AtoO tensors */
+enum { AO_XX, AO_YY, AO_ZZ, AO_XY, AO_YX, AO_XZ, AO_YZ, AO_ZX, AO_ZY, AO_N };
+
+/* QSigns (QSign.cs:31-38) in AO order; [AlongX, AlongY, AlongXSymm, AlongYSymm] */
+static const int AO_QS_E[4][AO_N] = {
+    {1, -1, -1, -1, 0, 1, 1, 1, 1},
+    {-1, 1, -1, -1, 0, 1, 1, 1, 1},
+    {1, -1, -1, 1, 0, -1, 1, -1, 1},
+    {-1, 1, -1, 1, 0, 1, -1, 1, -1},
+};
+static const int AO_QS_H[4][AO_N] = {
+    {-1, -1, 0, -1, 1, 1, 1, 1, 1},
+    {-1, -1, 0, 1, -1, 1, 1, 1, 1},
+    {1, 1, 0, -1, 1, 1, -1, 1, -1},
+    {1, 1, 0, 1, -1, 1, 1, -1, 1},
+};
+
+typedef struct {
+    int electric, nz, nseg, ncomp;
+    const xo_c *segA, *segB;      /* [seg][comp][tr] */
+    const double *radii;
+    xo_c *q[AO_N];                /* [nz] each; NULL when the component is not computed */
+    xo_c *G[AO_N];                /* [2Nx][2Ny][nz] */
+    int nx, ny;
+} actx;
+
+#define SEGV(arr, seg, comp, tr) (arr)[((int64_t)(seg) * c->ncomp + (comp)) * c->nz + (tr)]
+
+/* AddToQBuffAlong{X,Y}{Electric,Magnetic} (GreenTensorCalculator.cs:505-560) */
+static void ao_add_seg(actx *c, const factors *f, int seg, int alongY)
+{
+    for (int p = 0; p < c->nz; p++) {
+#define ADD(comp, Ic, DA, DB) c->q[comp][p] += f->DA * SEGV(c->segA, seg, (Ic) - 1, p), \
+                              c->q[comp][p] += f->DB * SEGV(c->segB, seg, (Ic) - 1, p)
+        if (c->electric) {
+            if (!alongY) {
+                ADD(AO_XX, 1, d2a, d2b); ADD(AO_YY, 2, d2a, d2b); ADD(AO_ZZ, 5, d2a, d2b);
+                ADD(AO_XY, 2, d3a, d3b); ADD(AO_YZ, 3, d1a, d1b); ADD(AO_ZY, 4, d1a, d1b);
+            } else {
+                ADD(AO_XX, 2, d2a, d2b); ADD(AO_YY, 1, d2a, d2b); ADD(AO_ZZ, 5, d2a, d2b);
+                ADD(AO_XY, 1, d3a, d3b); ADD(AO_XZ, 3, d1a, d1b); ADD(AO_ZX, 4, d1a, d1b);
+            }
+        } else {
+            if (!alongY) {
+                ADD(AO_XX, 2, d3a, d3b); ADD(AO_YX, 1, d2a, d2b); ADD(AO_XY, 2, d2a, d2b);
+                ADD(AO_XZ, 3, d1a, d1b); ADD(AO_ZX, 4, d1a, d1b);
+            } else {
+                ADD(AO_YX, 2, d2a, d2b); ADD(AO_XY, 1, d2a, d2b); ADD(AO_XX, 1, d3a, d3b);
+                ADD(AO_YZ, 3, d1a, d1b); ADD(AO_ZY, 4, d1a, d1b);
+            }
+        }
+#undef ADD
+    }
+}
+
+static void ao_integrate(actx *c, knot left, knot right, double s, double u, double v, int alongY)
+{
+    for (int t = 0; t < AO_N; t++)
+        if (c->q[t]) memset(c->q[t], 0, sizeof(xo_c) * c->nz);
+    int i1 = left.index, i2 = right.index;
+    if (i1 == i2) {
+        factors f = prep_factors(s, u, left.radius, right.radius, v);
+        ao_add_seg(c, &f, i1, alongY);
+    } else {
+        double r1 = left.radius, r2 = c->radii[i1 + 1];
+        double sj = s, uj = sqrt(r2 * r2 - v * v);
+        factors f = prep_factors(sj, uj, r1, r2, v);
+        ao_add_seg(c, &f, i1, alongY);
+        for (int i = i1 + 1; i < i2; i++) {
+            r1 = r2; r2 = c->radii[i + 1];
+            sj = uj; uj = sqrt(r2 * r2 - v * v);
+            f = prep_factors(sj, uj, r1, r2, v);
+            ao_add_seg(c, &f, i, alongY);
+        }
+        sj = uj; uj = u; r1 = r2; r2 = right.radius;
+        f = prep_factors(sj, uj, r1, r2, v);
+        ao_add_seg(c, &f, i2, alongY);
+    }
+}
+
+/* FillGreen with AtoOLevelGreenTensorCalculator.TransformI/J (:106-117): negative offsets wrap */
+static void ao_fill(actx *c, int i, int j, int sign, int qsel)
+{
+    int ii = i < 0 ? 2 * c->nx + i % (2 * c->nx) : i;
+    int jj = j < 0 ? 2 * c->ny + j % (2 * c->ny) : j;
+    int64_t shift = ((int64_t)ii * 2 * c->ny + jj) * c->nz;
+    const int (*qs)[AO_N] = c->electric ? AO_QS_E : AO_QS_H;
+    for (int t = 0; t < AO_N; t++) {
+        if (!c->q[t]) continue;
+        double al = (double)(sign * qs[qsel][t]);
+        for (int p = 0; p < c->nz; p++) c->G[t][shift + p] += al * c->q[t][p];
+    }
+}
+
+static void ao_fill_x(actx *c, int i, int j, int left, int right, int qsel)
+{
+    if (j == left) ao_fill(c, i, left, -1, qsel);
+    else if (j == right) ao_fill(c, i, right - 1, 1, qsel);
+    else { ao_fill(c, i, j - 1, 1, qsel); ao_fill(c, i, j, -1, qsel); }
+}
+
+static void ao_fill_y(actx *c, int i, int j, int left, int right, int qsel)
+{
+    if (i == left) ao_fill(c, left, j, -1, qsel);
+    else if (i == right) ao_fill(c, right - 1, j, 1, qsel);
+    else { ao_fill(c, i - 1, j, 1, qsel); ao_fill(c, i, j, -1, qsel); }
+}
+
+int xo_green_atoo_level(const xo_section *sec, const xo_grid *g, double omega, int number_of_hankels,
+                        const double *hank0, const double *hank1, double zl, int electric, int fft, xo_c **out)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    /* CreateForAnomalyToObservationLevels (ScalarPlansCreater.cs:51-66) for a level with no lateral
+     * shift: rho from min(dx, dy)/2 to the diagonal, the AtoA range (FindRhoMin/Max :180-250) */
+    xo_plan plan;
+    if (xo_plan_create(g, number_of_hankels, &plan)) return -1;
+    int nrho = plan.nrho_merged, nseg = nrho - 1;
+    int ncomp = electric ? 5 : 4;
+    xo_c *Isc = malloc(sizeof(xo_c) * (size_t)ncomp * nz * nrho);
+    if (atoo_scalars(sec, g, omega, hank0, hank1, &plan, zl, electric, Isc)) {
+        free(Isc); xo_plan_free(&plan); return -2;
+    }
+    xo_c *segA = malloc(sizeof(xo_c) * (size_t)nseg * ncomp * nz);
+    xo_c *segB = malloc(sizeof(xo_c) * (size_t)nseg * ncomp * nz);
+    const double *radii = plan.rho_merged;
+    for (int k = 0; k < nseg; k++) {                      /* ScalarSegments.CalculateScalarSegment */
+        double s = radii[k], u = radii[k + 1], div = u - s;
+        for (int cc = 0; cc < ncomp; cc++)
+            for (int p = 0; p < nz; p++) {
+                xo_c i0 = Isc[((int64_t)cc * nz + p) * nrho + k], i1 = Isc[((int64_t)cc * nz + p) * nrho + k + 1];
+                xo_c a = (i1 - i0) / div;
+                segA[((int64_t)k * ncomp + cc) * nz + p] = a;
+                segB[((int64_t)k * ncomp + cc) * nz + p] = i0 - a * s;
+            }
+    }
+    free(Isc);
+    /* which components: electric all but yx; magnetic all but yy, zz (AtoOLevelGreenTensorCalculator.cs:60-75) */
+    int have[AO_N];
+    for (int t = 0; t < AO_N; t++) have[t] = 1;
+    if (electric) have[AO_YX] = 0; else { have[AO_YY] = 0; have[AO_ZZ] = 0; }
+    int64_t csize = (int64_t)4 * nx * ny * nz;
+    for (int t = 0; t < AO_N; t++)
+        if (out[t]) memset(out[t], 0, sizeof(xo_c) * csize);
+    /* knots over offsets [-Nx+1, Nx] x [-Ny+1, Ny] (PrepareKnotsAtoO, TensorPlan serial case:
+     * DistributedUtils.cs:54-61) */
+    int lx = -nx + 1, rx = nx, ly = -ny + 1, ry = ny;
+    int kx = rx - lx + 1, ky = ry - ly + 1;
+    knot *K = malloc(sizeof(knot) * kx * ky), *ZX = malloc(sizeof(knot) * ky), *ZY = malloc(sizeof(knot) * kx);
+    for (int i = lx; i <= rx; i++)
+        for (int j = ly; j <= ry; j++)
+            K[(i - lx) * ky + (j - ly)] = make_knot(radii, nrho, g->dx * (i - 0.5), g->dy * (j - 0.5));
+    for (int j = ly; j <= ry; j++) ZX[j - ly] = make_knot(radii, nrho, 0, g->dy * (j - 0.5));
+    for (int i = lx; i <= rx; i++) ZY[i - lx] = make_knot(radii, nrho, g->dx * (i - 0.5), 0);
+#define KN(i, j) K[((i) - lx) * ky + ((j) - ly)]
+    actx c;
+    memset(&c, 0, sizeof(c));
+    c.electric = electric; c.nz = nz; c.nseg = nseg; c.ncomp = ncomp; c.segA = segA; c.segB = segB;
+    c.radii = radii; c.nx = nx; c.ny = ny;
+    for (int t = 0; t < AO_N; t++) {
+        c.q[t] = have[t] ? calloc(nz, sizeof(xo_c)) : NULL;
+        c.G[t] = have[t] ? out[t] : NULL;
+        if (have[t] && !out[t]) { c.q[t] = NULL; }
+    }
+    double dx = g->dx, dy = g->dy;
+    /* RunAlongX (GreenTensorCalculator.cs:192-230) */
+    for (int j = ly; j <= ry; j++) {
+        double y = dy * (j - 0.5);
+        for (int i = lx; i <= rx - 1; i++) {
+            double x1 = dx * (i - 0.5), x2 = dx * (i + 0.5);
+            if (x1 > 0 && x2 > 0) {
+                ao_integrate(&c, KN(i, j), KN(i + 1, j), x1, x2, y, 0);
+                ao_fill_x(&c, i, j, ly, ry, 0);
+            } else if (x1 < 0 && x2 < 0) {
+                ao_integrate(&c, KN(i + 1, j), KN(i, j), -x2, -x1, y, 0);
+                ao_fill_x(&c, i, j, ly, ry, 2);
+            } else {
+                ao_integrate(&c, ZX[j - ly], KN(i, j), 0, -x1, y, 0);
+                ao_fill_x(&c, i, j, ly, ry, 2);
+                ao_integrate(&c, ZX[j - ly], KN(i + 1, j), 0, x2, y, 0);
+                ao_fill_x(&c, i, j, ly, ry, 0);
+            }
+        }
+    }
+    /* RunAlongY (:246-281) */
+    for (int i = lx; i <= rx; i++) {
+        double x = dx * (i - 0.5);
+        for (int j = ly; j <= ry - 1; j++) {
+            double y1 = dy * (j - 0.5), y2 = dy * (j + 0.5);
+            if (y1 > 0 && y2 > 0) {
+                ao_integrate(&c, KN(i, j), KN(i, j + 1), y1, y2, x, 1);
+                ao_fill_y(&c, i, j, lx, rx, 1);
+            } else if (y1 < 0 && y2 < 0) {
+                ao_integrate(&c, KN(i, j + 1), KN(i, j), -y2, -y1, x, 1);
+                ao_fill_y(&c, i, j, lx, rx, 3);
+            } else {
+                ao_integrate(&c, ZY[i - lx], KN(i, j), 0, -y1, x, 1);
+                ao_fill_y(&c, i, j, lx, rx, 3);
+                ao_integrate(&c, ZY[i - lx], KN(i, j + 1), 0, y2, x, 1);
+                ao_fill_y(&c, i, j, lx, rx, 1);
+            }
+        }
+    }
+#undef KN
+    for (int t = 0; t < AO_N; t++) free(c.q[t]);
+    free(K); free(ZX); free(ZY); free(segA); free(segB);
+    xo_plan_free(&plan);
+    /* AtoOCalculator.PerformGreenTensorFft (:138-147) */
+    if (fft)
+        for (int t = 0; t < AO_N; t++)
+            if (have[t] && out[t]) xo_fft2_batch(out[t], 2 * nx, 2 * ny, nz, -1);
+    return 0;
+}
+
+/* ToOCalculator.CalculateAnomalyField (ToOCalculator.cs:21-45) for one level: FFT of jQ, the
+ * per-wavenumber nz -> 1 contraction (ApplyGreenTensorAlongZForE/H :47-125), inverse FFT,
+ * 1/(4 Nx Ny), crop. G[t]: AO-ordered FFT'd level tensors [2Nx][2Ny][nz] (NULL if absent).
+ * field: [Nx][Ny][3]. */
+void xo_atoo_field(int nx, int ny, int nz, xo_c **G, int electric, const xo_c *jq, xo_c *field)
+{
+    int64_t L = (int64_t)4 * nx * ny;
+    xo_c *b1 = calloc((size_t)L * 3 * nz, sizeof(xo_c));
+    xo_c *b2 = calloc((size_t)L * 3, sizeof(xo_c));
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            memcpy(b1 + ((int64_t)i * 2 * ny + j) * 3 * nz, jq + ((int64_t)i * ny + j) * 3 * nz, sizeof(xo_c) * 3 * nz);
+    xo_fft2_batch(b1, 2 * nx, 2 * ny, 3 * nz, -1);
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < L; l++) {
+        const xo_c *sx = b1 + l * 3 * nz, *sy = sx + nz, *sz = sy + nz;
+        xo_c dxv = 0, dyv = 0, dzv = 0;
+#define G_(t, k) G[t][l * nz + (k)]
+        for (int k = 0; k < nz; k++) {
+            if (electric) {
+                dxv += G_(AO_XX, k) * sx[k] + G_(AO_XY, k) * sy[k] + G_(AO_XZ, k) * sz[k];
+                dyv += G_(AO_XY, k) * sx[k] + G_(AO_YY, k) * sy[k] + G_(AO_YZ, k) * sz[k];
+                dzv += G_(AO_ZX, k) * sx[k] + G_(AO_ZY, k) * sy[k] + G_(AO_ZZ, k) * sz[k];
+            } else {
+                dxv += -(G_(AO_XX, k) * sx[k] + G_(AO_XY, k) * sy[k] + G_(AO_XZ, k) * sz[k]);
+                dyv += G_(AO_YX, k) * sx[k] + G_(AO_XX, k) * sy[k] + G_(AO_YZ, k) * sz[k];
+                dzv += -G_(AO_ZX, k) * sx[k] + G_(AO_ZY, k) * sy[k];
+            }
+        }
+#undef G_
+        b2[l * 3 + 0] = dxv; b2[l * 3 + 1] = dyv; b2[l * 3 + 2] = dzv;
+    }
+    xo_fft2_batch(b2, 2 * nx, 2 * ny, 3, +1);
+    double sc = 1.0 / ((double)nx * (double)ny * 4);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int cc = 0; cc < 3; cc++)
+                field[((int64_t)i * ny + j) * 3 + cc] = b2[((int64_t)i * 2 * ny + j) * 3 + cc] * sc;
+    free(b1); free(b2);
+}

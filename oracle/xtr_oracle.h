@@ -102,6 +102,17 @@ int xo_green_atoa(const xo_section *s, const xo_grid *g, double omega, int numbe
                   const double *hank0, const double *hank1, int fft,
                   xo_c *xx, xo_c *xy, xo_c *xz, xo_c *yy, xo_c *yz, xo_c *zz);
 
+/* AtoO level tensors (P7): anomaly layers to the observation level at depth zl (no lateral shift).
+ * electric = 1: components xx yy zz xy - xz yz zx zy; electric = 0 (magnetic): xx - - xy yx xz yz
+ * zx zy (AtoOLevelGreenTensorCalculator.cs:21-75). out[9] in the order xx yy zz xy yx xz yz zx zy,
+ * each [2Nx][2Ny][nz] (NULL for an absent component); fft = 0 stops before the FFT. */
+int xo_green_atoo_level(const xo_section *s, const xo_grid *g, double omega, int number_of_hankels,
+                        const double *hank0, const double *hank1, double zl, int electric, int fft, xo_c **out);
+
+/* Anomalous field at the level from jQ [Nx][Ny][3][Nz] (ToOCalculator.CalculateAnomalyField,
+ * ToOCalculator.cs:21-45): field [Nx][Ny][3]. */
+void xo_atoo_field(int nx, int ny, int nz, xo_c **G, int electric, const xo_c *jq, xo_c *field);
+
 /* ---- operator (A1-A9) --------------------------------------------------- */
 typedef struct {
     int nx, ny, nz;
