@@ -691,4 +691,403 @@ void build_green_atoa(const GreenInput &in, double2 *G, const double2 *tw_x, con
     launch_fft2_plain(G, 2 * g.nx, 2 * g.ny, g.gblk(), tw_x, tw_y, -1, s);
 }
 
+
+// =============================================================================================
+// AtoO: anomaly layers (volume transmitters) to one observation level (point receivers)
+// =============================================================================================
+namespace {
+
+// slots of the AtoO components (AtoOLevelGreenTensorCalculator.cs:21-75)
+enum { AO_XX, AO_YY, AO_ZZ, AO_XY, AO_YX, AO_XZ, AO_YZ, AO_ZX, AO_ZY, AO_N };
+
+struct PointInfo {            // one transmitter layer seen from the level
+    int r, s, t, b, kase;     // background layers of receiver / transmitter, min / max; 0 match 1 below 2 above
+    double zw, z1, z2, dt, db1;
+};
+
+// CalculateCTop/CBot/DTop/DBot (VolumeToPointCalculatorU.cs:166-208) for layers (r, s)
+__device__ inline double2 ctop_(const double2 *E, int r, int s, double zw, double z1, double z2)
+{
+    return cexp_d(-z1 * E[s] + zw * E[r]) - cexp_d(-z2 * E[s] + zw * E[r]);
+}
+__device__ inline double2 cbot_(const double2 *E, int r, int s, double zw, double z1, double z2)
+{
+    return cexp_d(-zw * E[r] + z2 * E[s]) - cexp_d(-zw * E[r] + z1 * E[s]);
+}
+__device__ inline double2 dtop_(const LayerModel &lm, const double2 *E, int r, int s, double zw, double z1, double z2)
+{
+    const int t = r < s ? r : s, b = r > s ? r : s;
+    const double2 base = -2.0 * (lm.depth[b + 1] * E[b] - lm.depth[t] * E[t]);
+    return cexp_d(base + z2 * E[s] - zw * E[r]) - cexp_d(base + z1 * E[s] - zw * E[r]);
+}
+__device__ inline double2 dbot_(const LayerModel &lm, const double2 *E, int r, int s, double zw, double z1, double z2)
+{
+    const int t = r < s ? r : s, b = r > s ? r : s;
+    const double2 base = -2.0 * (lm.depth[b + 1] * E[b] - lm.depth[t] * E[t]);
+    return cexp_d(base - z1 * E[s] + zw * E[r]) - cexp_d(base - z2 * E[s] + zw * E[r]);
+}
+
+// U[tr][item][comp][lambda], normalised before the Hankel filter (PlanCalculator.CalculateForE/H
+// :58-108, VolumeToPointCalculatorU.cs:34-337, ScalarMathUtils.NormalizeBeforeHankelForE/H).
+// electric: comps I1..I5 <- U1 (A1 set) * i w mu0 / 2, U2 / 2z, U3 l / 2z, U4 (-l) / 2z, U5 l^2 / 2z
+// magnetic: comps I1..I4 <- U31, U4Sigma, U1Sigma * l, U11 * l
+__global__ void point_u_kernel(LayerModel lm, int nz, const PointInfo *pts, int electric, int nitems, int nlam_max,
+                               const int *nlam, const double *lam, const double2 *eta, const double2 *ex,
+                               const double2 *p1, const double2 *q1, const double2 *ps, const double2 *qs,
+                               double2 *U)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int it = blockIdx.y;
+    const int tr = blockIdx.z;
+    if (it >= nitems || m >= nlam[it] || tr >= nz) return;
+    const PointInfo pi = pts[tr];
+    const int nl = lm.nl;
+    const int64_t base = ((int64_t)it * nlam_max + m) * nl;
+    const double2 *E = eta + base, *X = ex + base;
+    const double l = lam[(int64_t)it * nlam_max + m];
+    const int r = pi.r, s = pi.s, t = pi.t, b = pi.b;
+    const double zw = pi.zw, z1 = pi.z1, z2 = pi.z2;
+    const double2 er = E[r], es = E[s], et = E[t], eb = E[b];
+    // CalculateT / CalculateR (:150-165)
+    const double2 T = cexp_d(-zw * er - z1 * es + 2 * pi.dt * et) - cexp_d(-zw * er - z2 * es + 2 * pi.dt * et);
+    const double2 R = cexp_d(-2 * pi.db1 * eb + z2 * es + zw * er) - cexp_d(-2 * pi.db1 * eb + z1 * es + zw * er);
+    const double2 zrc2 = 2.0 * lm.zeta[r];
+    const int ncomp = electric ? 5 : 4;
+    double2 *out = U + (((int64_t)tr * nitems + it) * ncomp) * nlam_max + m;
+    for (int set = 0; set < 2; ++set) {
+        const double2 *P = (set ? ps : p1) + base;
+        const double2 *Q = (set ? qs : q1) + base;
+        const double2 pt = P[t], qb = Q[b];
+        const double2 ann = ann_at(lm, E, X, P, Q, r, s, set);
+        double2 psi1, psi3;
+        if (pi.kase == 0) {          // receiver inside the transmitter layer (:210-240)
+            const int n = r;
+            const double2 pq = P[n] * Q[n];
+            psi1 = ctop_(E, n, n, zw, zw, z2) + cbot_(E, n, n, zw, z1, zw) +
+                   pq * (dtop_(lm, E, n, n, zw, zw, z2) + dbot_(lm, E, n, n, zw, z1, zw));
+            psi3 = ctop_(E, n, n, zw, zw, z2) - cbot_(E, n, n, zw, z1, zw) +
+                   pq * (dtop_(lm, E, n, n, zw, z1, zw) - dtop_(lm, E, n, n, zw, zw, z2));
+        } else if (pi.kase == 1) {   // receiver below (:227-233, 249-256)
+            psi1 = cbot_(E, r, s, zw, z1, z2) + pt * qb * dbot_(lm, E, r, s, zw, z1, z2);
+            psi3 = -1.0 * cbot_(E, r, s, zw, z1, z2) + pt * qb * dbot_(lm, E, r, s, zw, z1, z2);
+        } else {                     // receiver above (:219-226, 241-248)
+            psi1 = ctop_(E, r, s, zw, z1, z2) + pt * qb * dtop_(lm, E, r, s, zw, z1, z2);
+            psi3 = ctop_(E, r, s, zw, z1, z2) - pt * qb * dtop_(lm, E, r, s, zw, z1, z2);
+        }
+        const double2 u1 = (ann / es) * (pt * T + qb * R + psi1);
+        const double2 u3 = (ann * (er / es)) * (-1.0 * (pt * T) + qb * R + psi3);
+        const double2 u4 = ann * (-1.0 * (pt * T) + qb * R - psi3);
+        if (electric) {
+            if (set == 0) {
+                out[0] = u1 * (c_(0, lm.omega * kMu0) * c_(0.5));
+                continue;
+            }
+            const double2 u2 = (ann * er) * (pt * T + qb * R - psi1);
+            double2 u5;
+            if (pi.kase == 0) {      // CalcPsi5RcAndTrMatch (:258-272)
+                const int n = r;
+                const double ln = lm.thick[n];
+                const double2 kk = (c_(0, lm.omega) * lm.zeta[n]) * c_(kMu0);
+                const double2 en = E[n], pq = P[n] * Q[n];
+                const double2 psi5 = pq * (cexp_d(-(2 * ln - (z2 - zw)) * en) + cexp_d(-(2 * ln - (zw - z1)) * en)) -
+                                     (cexp_d(-(z2 - zw) * en) + cexp_d(-(zw - z1) * en)) +
+                                     2.0 * (kk * ((c_(1) - pq * cexp_d(-2 * ln * en)) / c_(l * l)));
+                u5 = (ann / es) * (pt * T + qb * R + psi5);
+            } else {
+                u5 = u1;
+            }
+            out[(int64_t)1 * nlam_max] = u2 / zrc2;
+            out[(int64_t)2 * nlam_max] = u3 * (c_(l) / zrc2);
+            out[(int64_t)3 * nlam_max] = u4 * (c_(-l) / zrc2);
+            out[(int64_t)4 * nlam_max] = u5 * (c_(l * l) / zrc2);
+        } else {
+            if (set == 0) {
+                out[0] = u3;                                 // I1 <- U31
+                out[(int64_t)3 * nlam_max] = u1 * c_(l);     // I4 <- U11 * lambda
+            } else {
+                out[(int64_t)1 * nlam_max] = u4;             // I2 <- U4Sigma
+                out[(int64_t)2 * nlam_max] = u1 * c_(l);     // I3 <- U1Sigma * lambda
+            }
+        }
+    }
+}
+
+// Hankel filter in the non-fast order of HankelCoefficients.ConvoluteWith (:86-99) and
+// NormalizeAfterHankelForE/H (ScalarMathUtils.cs:131-155): comp c uses J1 or J0 and the factor
+// sign / (den rho). electric: J1 -, J1 +, J0 +, J0 +, J1 + with den 2 pi; magnetic: J1 -, J1 -,
+// J0 -, J0 + with den 4 pi.
+__global__ void atoo_hankel_kernel(int nz, int nitems, int ncomp, int nlam_max, const double2 *U, int nmerged,
+                                   const int *m_item, const int *m_idx, const double *radii, const double *hank0,
+                                   const double *hank1, int electric, double2 *I)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tr = blockIdx.y;
+    const int c = blockIdx.z;
+    if (k >= nmerged || tr >= nz || c >= ncomp) return;
+    const int it = m_item[k], idx = m_idx[k];
+    const bool j0 = c == 2 || c == 3;
+    const double *hank = j0 ? hank0 : hank1;
+    const double sgn = electric ? (c == 0 ? -1.0 : 1.0) : (c == 3 ? 1.0 : -1.0);
+    const double den = (electric ? 2 * kPi : 4 * kPi) * radii[k];
+    const double2 *f = U + (((int64_t)tr * nitems + it) * ncomp + c) * nlam_max + idx;
+    double re = 0, im = 0;
+    for (int j = 0; j < kTaps; ++j) {
+        const double h = hank[j];
+        const double2 v = f[kTaps - 1 - j];
+        re += v.x * h;
+        im += v.y * h;
+    }
+    I[((int64_t)c * nz + tr) * nmerged + k] = make_double2(sgn * re / den, sgn * im / den);
+}
+
+struct AoArgs {
+    int nx, ny, nz, nr, electric, ncomp, ntc;
+    double dx, dy;
+    const double2 *segA, *segB;   // [seg][comp][tr]
+    const double *radii;
+    double2 *G;                   // [2Nx][2Ny][ntc][nz]
+    int slot[AO_N];               // compact component index of each slot, -1 if not computed
+};
+
+struct Q9 { double2 q[AO_N]; };
+
+// AddToQBuffAlong{X,Y}{Electric,Magnetic} (GreenTensorCalculator.cs:505-560); comps I1..I5 at 0..4
+__device__ inline void ao_add(const AoArgs &a, int tr, const Factors &f, int seg, bool alongY, Q9 &Q)
+{
+    auto A = [&](int ic) { return a.segA[((int64_t)seg * a.ncomp + (ic - 1)) * a.nz + tr]; };
+    auto B = [&](int ic) { return a.segB[((int64_t)seg * a.ncomp + (ic - 1)) * a.nz + tr]; };
+    auto add = [&](int slot, int ic, double da, double db) {
+        Q.q[slot] = Q.q[slot] + da * A(ic);
+        Q.q[slot] = Q.q[slot] + db * B(ic);
+    };
+    if (a.electric) {
+        if (!alongY) {
+            add(AO_XX, 1, f.d2a, f.d2b); add(AO_YY, 2, f.d2a, f.d2b); add(AO_ZZ, 5, f.d2a, f.d2b);
+            add(AO_XY, 2, f.d3a, f.d3b); add(AO_YZ, 3, f.d1a, f.d1b); add(AO_ZY, 4, f.d1a, f.d1b);
+        } else {
+            add(AO_XX, 2, f.d2a, f.d2b); add(AO_YY, 1, f.d2a, f.d2b); add(AO_ZZ, 5, f.d2a, f.d2b);
+            add(AO_XY, 1, f.d3a, f.d3b); add(AO_XZ, 3, f.d1a, f.d1b); add(AO_ZX, 4, f.d1a, f.d1b);
+        }
+    } else {
+        if (!alongY) {
+            add(AO_XX, 2, f.d3a, f.d3b); add(AO_YX, 1, f.d2a, f.d2b); add(AO_XY, 2, f.d2a, f.d2b);
+            add(AO_XZ, 3, f.d1a, f.d1b); add(AO_ZX, 4, f.d1a, f.d1b);
+        } else {
+            add(AO_YX, 2, f.d2a, f.d2b); add(AO_XY, 1, f.d2a, f.d2b); add(AO_XX, 1, f.d3a, f.d3b);
+            add(AO_YZ, 3, f.d1a, f.d1b); add(AO_ZY, 4, f.d1a, f.d1b);
+        }
+    }
+}
+
+__device__ inline Q9 ao_integrate(const AoArgs &a, int tr, Knot left, Knot right, double s, double u, double v,
+                                  bool alongY)
+{
+    Q9 Q;
+    for (int t = 0; t < AO_N; ++t) Q.q[t] = c_(0);
+    const int i1 = left.index, i2 = right.index;
+    if (i1 == i2) {
+        ao_add(a, tr, factors(s, u, left.radius, right.radius, v), i1, alongY, Q);
+        return Q;
+    }
+    double r1 = left.radius, r2 = a.radii[i1 + 1];
+    double sj = s, uj = sqrt(r2 * r2 - v * v);
+    ao_add(a, tr, factors(sj, uj, r1, r2, v), i1, alongY, Q);
+    for (int i = i1 + 1; i < i2; ++i) {
+        r1 = r2;
+        r2 = a.radii[i + 1];
+        sj = uj;
+        uj = sqrt(r2 * r2 - v * v);
+        ao_add(a, tr, factors(sj, uj, r1, r2, v), i, alongY, Q);
+    }
+    sj = uj; uj = u; r1 = r2; r2 = right.radius;
+    ao_add(a, tr, factors(sj, uj, r1, r2, v), i2, alongY, Q);
+    return Q;
+}
+
+// QSigns (QSign.cs:31-38) by slot; rows AlongX, AlongY, AlongXSymm, AlongYSymm
+__constant__ int kQsE[4][AO_N] = {{1, -1, -1, -1, 0, 1, 1, 1, 1},
+                                  {-1, 1, -1, -1, 0, 1, 1, 1, 1},
+                                  {1, -1, -1, 1, 0, -1, 1, -1, 1},
+                                  {-1, 1, -1, 1, 0, 1, -1, 1, -1}};
+__constant__ int kQsH[4][AO_N] = {{-1, -1, 0, -1, 1, 1, 1, 1, 1},
+                                  {-1, -1, 0, 1, -1, 1, 1, 1, 1},
+                                  {1, 1, 0, -1, 1, 1, -1, 1, -1},
+                                  {1, 1, 0, 1, -1, 1, 1, -1, 1}};
+
+__device__ inline void ao_fill(const AoArgs &a, Q9 &G, const Q9 &Q, int sign, int qsel)
+{
+    for (int t = 0; t < AO_N; ++t) {
+        const int qs = a.electric ? kQsE[qsel][t] : kQsH[qsel][t];
+        G.q[t] = G.q[t] + (double)(sign * qs) * Q.q[t];
+    }
+}
+
+// One thread per (lateral offset (i, j) in (-Nx, Nx) x (-Ny, Ny), transmitter layer): the four edge
+// integrals of RunAlongX / RunAlongY (GreenTensorCalculator.cs:192-281) that FillResultAlongX/Y
+// (:413-437) add into this offset, written at TransformI/J (AtoOLevelGreenTensorCalculator.cs:106-117).
+__global__ __launch_bounds__(64) void atoo_lateral_kernel(AoArgs a)
+{
+    const int ncy = 2 * a.ny - 1;
+    const int cell = blockIdx.x;
+    const int i = cell / ncy - (a.nx - 1), j = cell % ncy - (a.ny - 1);
+    const int tr = blockIdx.y * blockDim.x + threadIdx.x;
+    if (tr >= a.nz) return;
+    const double dx = a.dx, dy = a.dy;
+    Q9 G;
+    for (int t = 0; t < AO_N; ++t) G.q[t] = c_(0);
+    for (int e = 0; e < 2; ++e) {     // edges along x at y = dy (jj - 1/2): jj = j (-1), j + 1 (+1)
+        const int jj = j + e, sign = e == 0 ? -1 : 1;
+        const double y = dy * (jj - 0.5);
+        const double x1 = dx * (i - 0.5), x2 = dx * (i + 0.5);
+        const Knot kl = make_knot(a.radii, a.nr, x1, y), kr = make_knot(a.radii, a.nr, x2, y);
+        if (x1 > 0 && x2 > 0) {
+            ao_fill(a, G, ao_integrate(a, tr, kl, kr, x1, x2, y, false), sign, 0);
+        } else if (x1 < 0 && x2 < 0) {
+            ao_fill(a, G, ao_integrate(a, tr, kr, kl, -x2, -x1, y, false), sign, 2);
+        } else {
+            const Knot kz = make_knot(a.radii, a.nr, 0, y);
+            ao_fill(a, G, ao_integrate(a, tr, kz, kl, 0, -x1, y, false), sign, 2);
+            ao_fill(a, G, ao_integrate(a, tr, kz, kr, 0, x2, y, false), sign, 0);
+        }
+    }
+    for (int e = 0; e < 2; ++e) {     // edges along y at x = dx (ii - 1/2): ii = i (-1), i + 1 (+1)
+        const int ii = i + e, sign = e == 0 ? -1 : 1;
+        const double x = dx * (ii - 0.5);
+        const double y1 = dy * (j - 0.5), y2 = dy * (j + 0.5);
+        const Knot kl = make_knot(a.radii, a.nr, x, y1), kr = make_knot(a.radii, a.nr, x, y2);
+        if (y1 > 0 && y2 > 0) {
+            ao_fill(a, G, ao_integrate(a, tr, kl, kr, y1, y2, x, true), sign, 1);
+        } else if (y1 < 0 && y2 < 0) {
+            ao_fill(a, G, ao_integrate(a, tr, kr, kl, -y2, -y1, x, true), sign, 3);
+        } else {
+            const Knot kz = make_knot(a.radii, a.nr, x, 0);
+            ao_fill(a, G, ao_integrate(a, tr, kz, kl, 0, -y1, x, true), sign, 3);
+            ao_fill(a, G, ao_integrate(a, tr, kz, kr, 0, y2, x, true), sign, 1);
+        }
+    }
+    const int64_t ti = i < 0 ? 2 * a.nx + i : i, tj = j < 0 ? 2 * a.ny + j : j;
+    double2 *dst = a.G + (ti * 2 * a.ny + tj) * (int64_t)(a.ntc * a.nz) + tr;
+    for (int t = 0; t < AO_N; ++t)
+        if (a.slot[t] >= 0) dst[(int64_t)a.slot[t] * a.nz] = G.q[t];
+}
+
+}  // namespace
+
+int atoo_components(int electric, int *slot)
+{
+    // compact order: electric xx yy zz xy xz yz zx zy; magnetic xx xy yx xz yz zx zy
+    static const int e[AO_N] = {0, 1, 2, 3, -1, 4, 5, 6, 7};
+    static const int h[AO_N] = {0, -1, -1, 1, 2, 3, 4, 5, 6};
+    for (int t = 0; t < AO_N; ++t) slot[t] = electric ? e[t] : h[t];
+    return electric ? 8 : 7;
+}
+
+void build_green_atoo_level(const GreenInput &in, double level_z, int electric, double2 *G, const double2 *tw_x,
+                            const double2 *tw_y, hipStream_t s)
+{
+    const Geom &g = in.g;
+    const int nz = g.nz;
+    const int nl = (int)in.section.thickness.size();
+    if (nl > kMaxLayers) throw hip_error("too many background layers for the device recursion");
+    LayerModel lm{};
+    lm.nl = nl;
+    lm.omega = in.omega;
+    const double eps0 = 1.0 / (kMu0 * 299792458.0 * 299792458.0);
+    double d = in.section.zero_level;
+    for (int k = 0; k < nl; ++k) {
+        lm.zeta[k] = make_double2(in.section.sigma[k], -in.omega * eps0);
+        lm.thick[k] = in.section.thickness[k];
+        lm.depth[k] = d;
+        d += in.section.thickness[k];
+    }
+    lm.depth[nl] = d;
+    // a level without lateral shift has the anomaly's rho range (ScalarPlansCreater.cs:51-66, 180-250)
+    const HostPlan hp = make_plan(in);
+    const int nitems = (int)hp.rho.size();
+    int nlam_max = 0;
+    std::vector<int> nlam(nitems);
+    for (int it = 0; it < nitems; ++it) {
+        nlam[it] = (int)hp.lam[it].size();
+        nlam_max = std::max(nlam_max, nlam[it]);
+    }
+    std::vector<double> lam_flat((size_t)nitems * nlam_max, 0.0);
+    for (int it = 0; it < nitems; ++it) std::copy(hp.lam[it].begin(), hp.lam[it].end(), lam_flat.begin() + (size_t)it * nlam_max);
+    const int nmerged = (int)hp.merged.size();
+    Dev<int> d_nlam(nlam);
+    Dev<double> d_lam(lam_flat);
+    Dev<double> d_radii(hp.merged);
+    Dev<int> d_mitem(hp.m_item), d_midx(hp.m_idx);
+    Dev<double> d_h0(in.hank0), d_h1(in.hank1);
+    const size_t auxn = (size_t)nitems * nlam_max * nl;
+    Dev<double2> eta(auxn), ex(auxn), p1(auxn), q1(auxn), ps(auxn), qs(auxn);
+    {
+        dim3 grid((unsigned)ceil_div(nlam_max, 128), nitems);
+        hipLaunchKernelGGL(layer_aux_kernel, grid, dim3(128), 0, s, lm, nitems, nlam_max, d_nlam.p, d_lam.p, eta.p,
+                           ex.p, p1.p, q1.p, ps.p, qs.p);
+        XTR_LAUNCH_CHECK("layer_aux_kernel");
+    }
+    // transmitters: the anomaly layers; receiver: the level point (AuxContainer.CreateContainer)
+    const int r = layer_at_depth(in.section, level_z);
+    std::vector<PointInfo> pts(nz);
+    for (int tr = 0; tr < nz; ++tr) {
+        PointInfo pi{};
+        pi.zw = level_z;
+        pi.z1 = in.depth[tr];
+        pi.z2 = in.depth[tr] + in.thick[tr];
+        pi.r = r;
+        pi.s = layer_at_depth(in.section, pi.z1 + (pi.z2 - pi.z1) / 2);
+        pi.t = std::min(pi.r, pi.s);
+        pi.b = std::max(pi.r, pi.s);
+        pi.dt = lm.depth[pi.t];
+        pi.db1 = lm.depth[pi.b + 1];
+        pi.kase = (pi.zw > pi.z1 && pi.zw < pi.z2) ? 0 : (pi.zw >= pi.z2 ? 1 : 2);   // CalcUGeneral (:54-62)
+        pts[tr] = pi;
+    }
+    const int ncomp = electric ? 5 : 4;
+    Dev<PointInfo> d_pts(pts);
+    Dev<double2> U((size_t)nz * nitems * ncomp * nlam_max);
+    {
+        dim3 grid((unsigned)ceil_div(nlam_max, 64), nitems, nz);
+        hipLaunchKernelGGL(point_u_kernel, grid, dim3(64), 0, s, lm, nz, d_pts.p, electric, nitems, nlam_max, d_nlam.p,
+                           d_lam.p, eta.p, ex.p, p1.p, q1.p, ps.p, qs.p, U.p);
+        XTR_LAUNCH_CHECK("point_u_kernel");
+    }
+    Dev<double2> I((size_t)ncomp * nz * nmerged);
+    {
+        dim3 grid((unsigned)ceil_div(nmerged, 128), nz, ncomp);
+        hipLaunchKernelGGL(atoo_hankel_kernel, grid, dim3(128), 0, s, nz, nitems, ncomp, nlam_max, U.p, nmerged,
+                           d_mitem.p, d_midx.p, d_radii.p, d_h0.p, d_h1.p, electric, I.p);
+        XTR_LAUNCH_CHECK("atoo_hankel_kernel");
+    }
+    const int nseg = nmerged - 1;
+    Dev<double2> segA((size_t)nseg * ncomp * nz), segB((size_t)nseg * ncomp * nz);
+    {
+        dim3 grid((unsigned)ceil_div(nz, 128), ncomp, nseg);
+        hipLaunchKernelGGL(segments_kernel, grid, dim3(128), 0, s, nz, ncomp, nmerged, d_radii.p, I.p, segA.p, segB.p);
+        XTR_LAUNCH_CHECK("segments_kernel");
+    }
+    AoArgs a{};
+    a.nx = g.nx;
+    a.ny = g.ny;
+    a.nz = nz;
+    a.nr = nmerged;
+    a.electric = electric;
+    a.ncomp = ncomp;
+    a.ntc = atoo_components(electric, a.slot);
+    a.dx = in.dx;
+    a.dy = in.dy;
+    a.segA = segA.p;
+    a.segB = segB.p;
+    a.radii = d_radii.p;
+    a.G = G;
+    XTR_CHECK(hipMemsetAsync(G, 0, (size_t)g.L() * a.ntc * nz * sizeof(double2), s));
+    {
+        dim3 grid((unsigned)((2 * g.nx - 1) * (2 * g.ny - 1)), (unsigned)ceil_div(nz, 64));
+        hipLaunchKernelGGL(atoo_lateral_kernel, grid, dim3(64), 0, s, a);
+        XTR_LAUNCH_CHECK("atoo_lateral_kernel");
+    }
+    launch_fft2_plain(G, 2 * g.nx, 2 * g.ny, (int64_t)a.ntc * nz, tw_x, tw_y, -1, s);
+    XTR_CHECK(hipStreamSynchronize(s));   // the temporaries above are freed on return
+}
+
 }  // namespace xtr

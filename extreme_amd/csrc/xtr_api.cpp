@@ -115,6 +115,9 @@ struct xtr_op {
     DevBuf<float2> tw32_x, tw32_y;
     DevBuf<unsigned char> buf;    // max_rhs * M spectrum
     DevBuf<double2> stage_x, stage_y;   // host-pointer apply staging
+    DevBuf<double2> Go[2];        // AtoO level tensors [L][ncomp][nz]: [0] magnetic, [1] electric
+    bool go_ready[2] = {false, false};
+    DevBuf<double2> lvl_spec, lvl_small;   // level-field scratch: L * 3nz and L * 3
 
     // Per-stage timing: one set of six events per apply while profiling is on, summed on demand.
     bool profiling = false;
@@ -703,6 +706,22 @@ void xtr_op_destroy(xtr_op *op)
     }
 }
 
+static xtr::GreenInput green_input(const xtr_op *op)
+{
+    xtr::GreenInput in;
+    in.g = op->g;
+    in.dx = op->dx;
+    in.dy = op->dy;
+    in.depth = op->depth;
+    in.thick = op->thick;
+    in.section = op->section;
+    in.omega = op->omega;
+    in.number_of_hankels = op->number_of_hankels;
+    in.hank0 = op->hank0;
+    in.hank1 = op->hank1;
+    return in;
+}
+
 static void finish_green(xtr_op *op)
 {
     if (op->precision == XTR_FP32 || op->precision == XTR_MIXED) {
@@ -747,18 +766,7 @@ int xtr_op_build_green(xtr_op *op)
         const xtr::Geom &g = op->g;
         op->G64.alloc((size_t)(g.L() * g.gblk()));
         op->G32.release();
-        xtr::GreenInput in;
-        in.g = g;
-        in.dx = op->dx;
-        in.dy = op->dy;
-        in.depth = op->depth;
-        in.thick = op->thick;
-        in.section = op->section;
-        in.omega = op->omega;
-        in.number_of_hankels = op->number_of_hankels;
-        in.hank0 = op->hank0;
-        in.hank1 = op->hank1;
-        xtr::build_green_atoa(in, op->G64.p, op->tw64_x.p, op->tw64_y.p, op->stream);
+        xtr::build_green_atoa(green_input(op), op->G64.p, op->tw64_x.p, op->tw64_y.p, op->stream);
         XTR_CHECK(hipStreamSynchronize(op->stream));
         finish_green(op);
     });
@@ -1052,6 +1060,69 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
         XTR_CHECK(hipStreamSynchronize(op->stream));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (int k = 0; k < nrhs; ++k) info[k].seconds = secs;
+    });
+}
+
+
+int xtr_op_build_green_atoo(xtr_op *op, double level_z, int electric)
+{
+    return guarded([&] {
+        require(op, "null argument");
+        require(op->hank0.size() == 431 && op->hank1.size() == 431, "the 431-tap Hankel filter is required");
+        require(op->number_of_hankels > 0, "NumberOfHankels must be positive");
+        const int e = electric ? 1 : 0;
+        int slot[9];
+        const int ncomp = xtr::atoo_components(e, slot);
+        op->go_ready[e] = false;
+        op->Go[e].alloc((size_t)(op->g.L() * ncomp * op->g.nz));
+        xtr::build_green_atoo_level(green_input(op), level_z, e, op->Go[e].p, op->tw64_x.p, op->tw64_y.p, op->stream);
+        op->go_ready[e] = true;
+    });
+}
+
+int xtr_op_get_green_atoo(xtr_op *op, int electric, xtr_complex *const *comps)
+{
+    return guarded([&] {
+        require(op && comps, "null argument");
+        const int e = electric ? 1 : 0;
+        require(op->go_ready[e], "the level tensor is not built");
+        int slot[9];
+        const int ncomp = xtr::atoo_components(e, slot);
+        const int64_t L = op->g.L(), nz = op->g.nz;
+        std::vector<double2> h((size_t)(L * ncomp * nz));
+        XTR_CHECK(hipMemcpy(h.data(), op->Go[e].p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        for (int t = 0; t < 9; ++t) {
+            if (!comps[t] || slot[t] < 0) continue;
+            auto *dst = reinterpret_cast<double2 *>(comps[t]);
+            for (int64_t l = 0; l < L; ++l)
+                std::memcpy(dst + l * nz, h.data() + (l * ncomp + slot[t]) * nz, nz * sizeof(double2));
+        }
+    });
+}
+
+int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq)
+{
+    return guarded([&] {
+        require(op && chi && js && jq, "null argument");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before computing jQ");
+        xtr::launch_jq((const double2 *)chi, (const double2 *)js, op->R.p, op->f.p, op->g, (double2 *)jq, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    });
+}
+
+int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *field)
+{
+    return guarded([&] {
+        require(op && jq && field, "null argument");
+        const int e = electric ? 1 : 0;
+        require(op->go_ready[e], "build the level tensor first (xtr_op_build_green_atoo)");
+        const xtr::Geom &g = op->g;
+        if (op->lvl_spec.count < (size_t)g.M()) op->lvl_spec.alloc((size_t)g.M());
+        if (op->lvl_small.count < (size_t)(3 * g.L())) op->lvl_small.alloc((size_t)(3 * g.L()));
+        XTR_CHECK(hipDeviceSynchronize());   // jQ may have been written on another stream
+        xtr::launch_level_field(op->Go[e].p, e, (const double2 *)jq, g, op->lvl_spec.p, op->lvl_small.p, op->tw64_x.p,
+                                op->tw64_y.p, (double2 *)field, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
     });
 }
 

@@ -71,6 +71,14 @@ void launch_green_interleave(const double2 *comp[6], double2 *dev, const Geom &g
 void launch_green_deinterleave(const double2 *dev, double2 *comp[6], const Geom &g, hipStream_t s);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
 
+// Observation level (obs_kernels.hip): jQ from chi (operator prepared as A: R and f of that type),
+// and the anomalous field at a level from jQ with the AtoO tensor G ([L][ncomp][nz]); spec holds
+// L * 3nz and small L * 3 complex128 of scratch; field is [Nx][Ny][3].
+void launch_jq(const double2 *chi, const double2 *js, const double2 *R, const double *f, const Geom &g, double2 *jq,
+               hipStream_t s);
+void launch_level_field(const double2 *G, int electric, const double2 *jq, const Geom &g, double2 *spec,
+                        double2 *small, const double2 *tw_x, const double2 *tw_y, double2 *field, hipStream_t s);
+
 // ---- Krylov (krylov_kernels.hip) -----------------------------------------------------------
 constexpr int kRedBlocks = 512;   // partial-sum blocks of the reductions
 // partial[k][blk] of conj(V_k) . w, k < cnt (V columns contiguous, stride n)

@@ -118,6 +118,33 @@ def scattered_field_and_jq(model: cm.CartesianModel, omega: float, chi: np.ndarr
     return es.reshape(-1), jq.reshape(-1)
 
 
+def impedance(e1, e2, h1, h2):
+    """ResponseFunctionsCalculator.CalculateTensor (Magnetotellurics/ResponseFunctionsCalculator.cs:36-51):
+    Z = [E1 E2] [H1 H2]^-1 per site from the two polarisations' field 3-vectors ([..., 3] -> [..., 2, 2])."""
+    det = h1[..., 0] * h2[..., 1] - h2[..., 0] * h1[..., 1]
+    hi11, hi12 = h2[..., 1] / det, -h2[..., 0] / det
+    hi21, hi22 = -h1[..., 1] / det, h1[..., 0] / det
+    z = np.empty(np.shape(e1)[:-1] + (2, 2), dtype=np.complex128)
+    z[..., 0, 0] = e1[..., 0] * hi11 + e2[..., 0] * hi21
+    z[..., 0, 1] = e1[..., 0] * hi12 + e2[..., 0] * hi22
+    z[..., 1, 0] = e1[..., 1] * hi11 + e2[..., 1] * hi21
+    z[..., 1, 1] = e1[..., 1] * hi12 + e2[..., 1] * hi22
+    return z
+
+
+def apparent_resistivity_phase(z, omega):
+    """PlainTextExporter.CalculateApparentResistivity / ProcessPhase (Results/PlainTextExporter.cs:360-420):
+    rho_a = |Z|^2 / (omega mu0) and phase = arg Z in radians (no degree conversion, no sign flip)."""
+    return np.abs(z) ** 2 / (omega * cm.MU0), np.angle(z)
+
+
+def level_normal_fields(section: cm.Section1D, omega: float, level_z: float):
+    """Mt3DForwardSolver.CalculateNormalFieldE/H(level) (Solver/Mt3DForwardSolver.cs:236-271): pol X
+    has E = (E0, 0, 0), H = (0, H0, 0); pol Y has E = (0, E0, 0), H = (-H0, 0, 0)."""
+    e0, h0 = plane_wave_e(section, omega, level_z), plane_wave_h(section, omega, level_z)
+    return (np.array([e0, 0, 0]), np.array([0, h0, 0])), (np.array([0, e0, 0]), np.array([-h0, 0, 0]))
+
+
 class FrequencySolve:
     """One frequency of Mt3DForwardSolver.SolvePrivate on the GPU: Green tensor, chi0 for both
     polarisations, and the side-by-side FGMRES solve of A chi = chi0."""
@@ -148,6 +175,27 @@ class FrequencySolve:
         self.chi.copy_(self.chi0)
         self.solver.history.clear()
         return self.solver.solve_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]])
+
+    def observe(self, level_z: float = 0.0) -> dict:
+        """Fields and responses at the observation level after solve(): level tensors on the GPU,
+        jQ from chi, anomalous E and H by the AtoO convolution (AtoOCalculator.cs:63-137), plus the
+        level's normal field; then Z, rho_a and phase per site (cell centres, no shift)."""
+        torch = self.torch
+        op = self.op
+        op.calculate_green_atoo(level_z, electric=True)
+        op.calculate_green_atoo(level_z, electric=False)
+        normals = level_normal_fields(self.model.section, self.omega, level_z)
+        E, H = [], []
+        for pol in (0, 1):
+            js = torch.from_numpy(self.js[pol]).to("cuda")
+            jq = op.jq(self.chi[pol], js)
+            ea = op.level_field(jq, electric=True).cpu().numpy()
+            ha = op.level_field(jq, electric=False).cpu().numpy()
+            E.append(normals[pol][0][None, None, :] + ea)
+            H.append(normals[pol][1][None, None, :] + ha)
+        z = impedance(E[0], E[1], H[0], H[1])
+        rho, phase = apparent_resistivity_phase(z, self.omega)
+        return {"E": E, "H": H, "Z": z, "rho": rho, "phase": phase}
 
     def solve_mixed(self, inner_floor=1e-8, max_refinements=10):
         """The same two solves in mixed precision (needs precision="mixed"): chi <- chi0, then FP64

@@ -92,6 +92,10 @@ def lib():
         L.xtr_refined_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_RefineParams), C.c_int,
                                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                      C.POINTER(_RefineInfo)]
+        L.xtr_op_build_green_atoo.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.xtr_op_get_green_atoo.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.xtr_op_jq_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xtr_op_level_field_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -257,6 +261,43 @@ class ConvolutionOperator:
             y = np.empty_like(x)
         _check(lib().xtr_op_apply(self._h, _ptr(x), _ptr(y), nrhs))
         return y
+
+    # --- observation level (AtoO) ----------------------------------------------------------
+    AO_COMPONENTS = ("xx", "yy", "zz", "xy", "yx", "xz", "yz", "zx", "zy")
+    AO_ELECTRIC = ("xx", "yy", "zz", "xy", "xz", "yz", "zx", "zy")
+    AO_MAGNETIC = ("xx", "xy", "yx", "xz", "yz", "zx", "zy")
+
+    def calculate_green_atoo(self, level_z: float = 0.0, electric: bool = True):
+        """AtoOCalculator.CalculateGreenTensorAtoOLevelElectric / ...Magnetic (AtoOCalculator.cs:101-137):
+        the level tensor, assembled and FFT'd on the GPU and kept in the handle."""
+        _check(lib().xtr_op_build_green_atoo(self._h, float(level_z), 1 if electric else 0))
+
+    def green_atoo(self, electric: bool = True) -> dict:
+        """The level tensor copied back, component -> [2Nx*2Ny, Nz]."""
+        m = self.model
+        L = 4 * m.nx * m.ny
+        comps = self.AO_ELECTRIC if electric else self.AO_MAGNETIC
+        out = {c: np.zeros((L, m.nz), dtype=np.complex128) for c in comps}
+        ptrs = (C.c_void_p * 9)(*[out[c].ctypes.data if c in out else None for c in self.AO_COMPONENTS])
+        _check(lib().xtr_op_get_green_atoo(self._h, 1 if electric else 0, ptrs))
+        return out
+
+    def jq(self, chi, js, out=None):
+        """ForwardSolver.CalculateJqFrom (ForwardSolver.cs:364-389, 424-436) on device tensors; the
+        operator must be prepared as "A"."""
+        if out is None:
+            out = chi.new_empty(chi.shape)
+        _check(lib().xtr_op_jq_device(self._h, _ptr(chi), _ptr(js), _ptr(out)))
+        return out
+
+    def level_field(self, jq, electric: bool = True, out=None):
+        """AtoOCalculator.CalculateAnomalyFieldE/H at the level (AtoOCalculator.cs:63-99): device
+        tensor [Nx, Ny, 3]."""
+        m = self.model
+        if out is None:
+            out = jq.new_empty((m.nx, m.ny, 3))
+        _check(lib().xtr_op_level_field_device(self._h, 1 if electric else 0, _ptr(jq), _ptr(out)))
+        return out
 
     def set_profiling(self, on: bool):
         """Starts (and resets) or stops per-stage HIP-event timing of every apply."""

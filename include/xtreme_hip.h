@@ -183,6 +183,29 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
 int xtr_op_set_profiling(xtr_op *op, int enable);
 int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
 
+/* ---- observation level (AtoO) ----------------------------------------------------------------- */
+
+/* Replaces: AtoOCalculator.CalculateGreenTensorAtoOLevelElectric / ...Magnetic
+ * (Extreme.Cartesian/Forward/ToObs/AtoOCalculator.cs:101-137): the FP64 level tensor of the
+ * observation level at depth level_z (cell centres, no lateral shift), assembled and FFT'd on the
+ * device and kept in the handle. electric = 1: E (xx yy zz xy xz yz zx zy); 0: H (xx xy yx xz yz zx zy). */
+int xtr_op_build_green_atoo(xtr_op *op, double level_z, int electric);
+
+/* Copies a level tensor back: comps[9] in the order xx yy zz xy yx xz yz zx zy, each [2Nx*2Ny][Nz]
+ * (the reference's AlongVertical layout, AtoOLevelGreenTensorCalculator.cs:52-57); NULL entries
+ * and components the tensor lacks are skipped (test and export hook). */
+int xtr_op_get_green_atoo(xtr_op *op, int electric, xtr_complex *const *comps);
+
+/* Replaces: ForwardSolver.CalculateEScatteredFrom + CalculateJqFrom (ForwardSolver.cs:319-436):
+ * jQ = E_s (zeta - zeta0) + j_s from chi and j_s (device vectors of n values). The operator must be
+ * prepared as XTR_OP_A. */
+int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq);
+
+/* Replaces: AtoOCalculator.CalculateAnomalyFieldE/H(level, jQ, field) (AtoOCalculator.cs:63-99)
+ * through ToOCalculator.CalculateAnomalyField (ToOCalculator.cs:21-45): the anomalous E (electric
+ * = 1) or H at the level, field [Nx][Ny][3] on the device, from the tensor built above. */
+int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *field);
+
 /* Sizes: n = 3 Nx Ny Nz, the Green tensor's device bytes, the spectrum buffer bytes. */
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes);
 

@@ -1,0 +1,119 @@
+"""Observation level (AtoO) on the GPU against the oracle: level tensors, jQ, anomalous fields, and
+the end-to-end impedance, apparent resistivity and phase at C1 (FP64, within 1e-9 relative)."""
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import forward as F
+from extreme_amd import xtreme as X
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_tensor(got, ref, tag):
+    """Each component within 1e-10 of its own norm, or, for components that vanish physically at
+    the level (E_z, H from J_z at the air interface: below 1e-4 of the tensor), within 1e-11 of the
+    largest component's norm."""
+    scale = max(np.linalg.norm(v) for v in ref.values())
+    for c in ref:
+        nrm = np.linalg.norm(ref[c])
+        err = np.linalg.norm(got[c] - ref[c])
+        print(f"{tag} {c}: rel {err / max(nrm, 1e-300):.2e} (|c|/max {nrm / scale:.1e})", flush=True)
+        if nrm > 1e-4 * scale:
+            assert err < 1e-10 * nrm, (tag, c)
+        else:
+            assert err < 1e-11 * scale, (tag, c)
+
+
+def _three_layer(nx, ny, nz):
+    sec = cm.Section1D(0.0, [0.0, 400.0, 900.0, 0.0], [0.0, 1e-2, 0.2, 2e-3])
+    depth = np.linspace(20.0, 380.0, nz + 1)[:-1]
+    thick = np.diff(np.linspace(20.0, 380.0, nz + 1))
+    sigma = np.full((nx, ny, nz), 1e-2)
+    sigma[1:3, 1:3, :] = 0.5
+    m = cm.CartesianModel(nx, ny, 300.0, 250.0, depth, thick, sigma, sec, "three-layer", [0.3])
+    return m
+
+
+@pytest.mark.parametrize("which", ["small", "c1"])
+@pytest.mark.parametrize("electric", [True, False])
+def test_level_tensor_matches_oracle(oracle, torch_cuda, which, electric):
+    m = cm.small_model(6, 4, 3, seed=2) if which == "small" else cm.config_c1()
+    w = cm.omega_of(m.frequencies[0])
+    op = X.ConvolutionOperator(m, w)
+    op.calculate_green_atoo(0.0, electric)
+    _check_tensor(op.green_atoo(electric), oracle.green_atoo_level(m, w, 0.0, electric),
+                  f"{which} {'E' if electric else 'H'}")
+
+
+def test_level_tensor_inside_the_anomaly_matches_oracle(oracle, torch_cuda):
+    """A level inside an anomaly layer takes the receiver-inside-transmitter branches."""
+    m = _three_layer(4, 4, 4)
+    w = cm.omega_of(0.3)
+    zl = m.depth[1] + 0.3 * m.thick[1]
+    op = X.ConvolutionOperator(m, w)
+    for electric in (True, False):
+        op.calculate_green_atoo(zl, electric)
+        _check_tensor(op.green_atoo(electric), oracle.green_atoo_level(m, w, zl, electric), f"inside {electric}")
+
+
+def test_jq_and_level_fields_match_oracle(oracle, torch_cuda):
+    torch = torch_cuda
+    m = cm.small_model(6, 4, 3, seed=5)
+    w = cm.omega_of(m.frequencies[0])
+    op = X.ConvolutionOperator(m, w)
+    op.calculate_green_tensor()
+    op.prepare_operator("A")
+    rng = np.random.default_rng(1)
+    chi = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    e0 = oracle.normal_field(m, w)
+    js = oracle.jscattered(m, 1, z, z0, e0)
+    _, jq_ref = oracle.escattered_jq(m, z, z0, chi, js)
+    jq = op.jq(torch.from_numpy(chi).to("cuda"), torch.from_numpy(js).to("cuda"))
+    assert rel(jq.cpu().numpy(), jq_ref) < 1e-13
+    for electric in (True, False):
+        op.calculate_green_atoo(0.0, electric)
+        field = op.level_field(jq, electric).cpu().numpy()
+        ref = oracle.atoo_field(m, oracle.green_atoo_level(m, w, 0.0, electric), electric, jq_ref)
+        print(f"field {'E' if electric else 'H'}: {rel(field, ref):.2e}", flush=True)
+        assert rel(field, ref) < 1e-10
+
+
+def test_c1_impedance_rho_phase_match_oracle(oracle, torch_cuda):
+    """The whole forward chain at C1: chi by FGMRES, jQ, level fields, normal fields, Z, rho_a, phase."""
+    m = cm.config_c1()
+    p = F.FrequencySolve(m, m.frequencies[0])
+    p.solve()
+    obs = p.observe(0.0)
+    w = p.omega
+    G = oracle.green_atoa(m, w)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    oop = oracle.Operator(m, G)
+    e0 = oracle.normal_field(m, w)
+    Ge = oracle.green_atoo_level(m, w, 0.0, True)
+    Gh = oracle.green_atoo_level(m, w, 0.0, False)
+    eN, hN = oracle.plane_wave_e(m.section, w, 0.0), oracle.plane_wave_h(m.section, w, 0.0)
+    normals = [(np.array([eN, 0, 0]), np.array([0, hN, 0])), (np.array([0, eN, 0]), np.array([-hN, 0, 0]))]
+    E, H = [], []
+    for pol in (0, 1):
+        js = oracle.jscattered(m, pol, z, z0, e0)
+        oop.prepare("Chi0", z, z0)
+        chi0 = oop.apply(js)
+        oop.prepare("A", z, z0)
+        chi, _, _ = oracle.fgmres(oop.raw_apply, chi0, chi0, 30, 10, 100, 1e-8)
+        _, jq = oracle.escattered_jq(m, z, z0, chi, js)
+        E.append(normals[pol][0] + oracle.atoo_field(m, Ge, True, jq))
+        H.append(normals[pol][1] + oracle.atoo_field(m, Gh, False, jq))
+    Z = oracle.impedance(E[0], E[1], H[0], H[1])
+    rho, phase = oracle.rho_phase(Z, w)
+    for pol in (0, 1):
+        assert rel(obs["E"][pol], E[pol]) < 1e-9
+        assert rel(obs["H"][pol], H[pol]) < 1e-9
+    eZ = rel(obs["Z"], Z)
+    erho = float(np.max(np.abs(obs["rho"][..., 0, 1] - rho[..., 0, 1]) / rho[..., 0, 1]))
+    ephi = float(np.max(np.abs(obs["phase"][..., 0, 1] - phase[..., 0, 1])))
+    print(f"C1: Z {eZ:.2e}, rho_xy {erho:.2e}, phase_xy {ephi:.2e} rad; rho_xy range "
+          f"{rho[..., 0, 1].min():.2f}..{rho[..., 0, 1].max():.2f} ohm-m", flush=True)
+    assert eZ < 1e-9 and erho < 1e-9 and ephi < 1e-9
