@@ -2,7 +2,8 @@
 
 The reference ships no test vectors for its solver path, so these fixtures freeze the oracle's
 outputs on small seeded cases: the AtoA Green tensor (after mirroring and FFT), the A and Chi0
-operator applies, an FGMRES solve with its iteration history, and plane-wave fields. The CPU
+operator applies, an FGMRES solve with its iteration history, plane-wave fields, and the
+observation-level (AtoO) tensors, jQ and anomalous fields of that solve. The CPU
 suite checks that the oracle still reproduces them; the GPU suite checks the HIP library
 against them. Regenerate with:  python tests/golden/make_golden.py
 """
@@ -56,7 +57,16 @@ def compute():
     freqs = np.array([1e-3, 0.5, 30.0])
     pw_e = np.array([[O.plane_wave_e(m.section, cm.omega_of(f), d) for d in depths] for f in freqs])
     pw_h = np.array([[O.plane_wave_h(m.section, cm.omega_of(f), d) for d in depths] for f in freqs])
+    # observation level at the surface: level tensors, jQ of the solved chi, anomalous fields
+    Ge = O.green_atoo_level(m, w, 0.0, True)
+    Gh = O.green_atoo_level(m, w, 0.0, False)
+    _, jq = O.escattered_jq(m, z, z0, chi, js)
+    field_e = O.atoo_field(m, Ge, True, jq)
+    field_h = O.atoo_field(m, Gh, False, jq)
     out = {f"green_{c}": G[c] for c in O.COMPONENTS}
+    out.update({f"atoo_e_{c}": v for c, v in Ge.items()})
+    out.update({f"atoo_h_{c}": v for c, v in Gh.items()})
+    out.update(jq=jq, level_field_e=field_e, level_field_h=field_h)
     out.update(x=x, y_a=y_a, y_chi0=y_chi0, js=js, chi0=chi0, chi=chi,
                fgmres_iterations=np.int64(info["iterations"]), fgmres_matvecs=np.int64(info["matvecs"]),
                fgmres_bea=np.array([h[2] for h in hist if h[0] == 1]), pw_depths=depths, pw_freqs=freqs, pw_e=pw_e, pw_h=pw_h)

@@ -64,3 +64,17 @@ def test_fgmres_matches_fixture(case):
     assert beas.shape == g["fgmres_bea"].shape
     assert np.all(np.abs(beas - g["fgmres_bea"]) <= 1e-9 * g["fgmres_bea"] + 1e-13)
     assert rel(x, g["chi"]) < TOL
+
+
+def test_level_tensors_and_fields_match_fixture(case, torch_cuda):
+    torch = torch_cuda
+    g, m, w, s, op = case
+    for electric, tag in ((True, "e"), (False, "h")):
+        op.calculate_green_atoo(0.0, electric)
+        got = op.green_atoo(electric)
+        scale = max(np.linalg.norm(g[f"atoo_{tag}_{c}"]) for c in got)
+        for c, v in got.items():
+            assert np.linalg.norm(v - g[f"atoo_{tag}_{c}"]) < 1e-11 * scale, (tag, c)
+        field = op.level_field(torch.from_numpy(g["jq"]).to("cuda"), electric).cpu().numpy()
+        print(f"level field {tag}: rel {rel(field, g[f'level_field_{tag}']):.2e}", flush=True)
+        assert rel(field, g[f"level_field_{tag}"]) < 1e-10
