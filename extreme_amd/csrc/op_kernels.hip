@@ -128,8 +128,7 @@ struct IoPlain {
 // padded buffer; columns c = i*3Nz + col; points j < Ny are data, j >= Ny are the padding.
 template <typename T>
 struct IoFwdY {
-    RhsPtrs v;
-    const double2 *R;
+    RhsPtrs v;                // x, and R per right-hand side
     const double *f;
     cplx<T> *buf;
     int64_t M;
@@ -141,7 +140,7 @@ struct IoFwdY {
         const int64_t i = c / C, col = c % C;
         const int k = (int)(col % nz);
         const int64_t ij = i * ny + p;
-        double2 rx = cmul<double>(R[ij * nz + k], v.x[rhs][ij * C + col]);
+        double2 rx = cmul<double>(v.R[rhs][ij * nz + k], v.x[rhs][ij * C + col]);
         return from_d<T>(cscale<double>(rx, f[k]));
     }
     __device__ void store(int rhs, int64_t c, int p, cplx<T> val) const
@@ -158,8 +157,7 @@ struct IoFwdY {
 // y = x - b*conv - R*x (A) or y = b*conv + R*x (Chi0) in FP64.
 template <typename T>
 struct IoInvY {
-    RhsPtrs v;
-    const double2 *R;
+    RhsPtrs v;                // x, y, and R per right-hand side
     const double *b;
     cplx<T> *buf;
     int64_t M;
@@ -182,7 +180,7 @@ struct IoInvY {
         if constexpr (sizeof(T) == 8) conv = cscale<double>(val, b[k]);
         else conv = cscale<double>(to_d(val), b[k]);
         const double2 xv = v.x[rhs][t];
-        const double2 rx = cmul<double>(R[ij * nz + k], xv);
+        const double2 rx = cmul<double>(v.R[rhs][ij * nz + k], xv);
         double2 out;
         if (type == 0) out = csub<double>(csub<double>(xv, conv), rx);
         else out = cadd<double>(conv, rx);
@@ -274,10 +272,19 @@ bool fft_length_supported(int p)
 
 // ---- stage launchers -------------------------------------------------------------------------
 
+// right-hand sides without a model R of their own use the handle's R
+static RhsPtrs with_r(const ApplyArgs &a)
+{
+    RhsPtrs v = a.v;
+    for (int i = 0; i < a.nrhs; ++i)
+        if (!v.R[i]) v.R[i] = a.R;
+    return v;
+}
+
 template <typename T>
 static void fwd_y(const ApplyArgs &a, hipStream_t s)
 {
-    IoFwdY<T> io{a.v, a.R, a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
+    IoFwdY<T> io{with_r(a), a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
     launch_pencil<T, -1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
 }
 
@@ -300,7 +307,7 @@ static void inv_x(const ApplyArgs &a, hipStream_t s)
 template <typename T>
 static void inv_y(const ApplyArgs &a, hipStream_t s)
 {
-    IoInvY<T> io{a.v, a.R, a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
+    IoInvY<T> io{with_r(a), a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
     launch_pencil<T, +1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
 }
 
@@ -442,7 +449,7 @@ template <typename T> struct Cfg {
     static constexpr int E = (int)sizeof(cplx<T>);            // bytes per element
     static constexpr int EPL = 16 / E;                        // elements per lane per wave-load
     static constexpr int EPC = 64 * EPL;                      // elements per wave-load
-    static constexpr int RAW = (GB + 2 * C + EPC - 1) / EPC;  // wave-loads for Green + 2 rhs columns
+    static constexpr int RAW = (GB + 4 * C + EPC - 1) / EPC;  // wave-loads for Green + 4 rhs columns
     static constexpr int NCHUNK = (RAW + 3) / 4 * 4;          // padded to the 4 waves
     static constexpr int PER_WAVE = NCHUNK / 4;
     static constexpr int BUF = NCHUNK * EPC;                  // elements per LDS buffer
@@ -453,6 +460,7 @@ static_assert(Cfg<double>::PER_WAVE * (Cfg<double>::NBUF - 1) <= 63, "vmcnt rang
 static_assert(Cfg<float>::PER_WAVE * (Cfg<float>::NBUF - 1) <= 63, "vmcnt range");
 static_assert(Cfg<double>::NBUF * Cfg<double>::BUF * 16 + 4 * 2 * C * 16 <= 160 * 1024, "LDS");
 static_assert(Cfg<float>::NBUF * Cfg<float>::BUF * 8 + 4 * 2 * C * 8 <= 160 * 1024, "LDS");
+constexpr int kMaxStreamRhs = 4;
 }  // namespace stream
 
 template <int N>
@@ -467,10 +475,11 @@ __device__ __forceinline__ void wait_vm()
 
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-// This wave's PER_WAVE wave-loads of wavenumber l into the LDS buffer at byte address dst.
-template <typename T, int NRHS>
+// This wave's PER_WAVE wave-loads of wavenumber l (Green block and nrhs spectrum columns) into the
+// LDS buffer at byte address dst.
+template <typename T>
 __device__ __forceinline__ void stream_issue(const cplx<T> *__restrict__ G, const cplx<T> *buf, int64_t M, int64_t l,
-                                             unsigned dst, int wave, int lane)
+                                             int nrhs, unsigned dst, int wave, int lane)
 {
     using namespace stream;
     using K = Cfg<T>;
@@ -483,7 +492,7 @@ __device__ __forceinline__ void stream_issue(const cplx<T> *__restrict__ G, cons
             src = G + l * GB + e;
         } else {
             int q = e - GB;                                         // spectrum column of this wavenumber
-            if (q > NRHS * C - K::EPL) q = NRHS * C - K::EPL;       // padding lanes re-read valid data
+            if (q > nrhs * C - K::EPL) q = nrhs * C - K::EPL;       // padding lanes re-read valid data
             const int rhs = q / C;
             src = buf + rhs * M + l * C + (q - rhs * C);
         }
@@ -493,12 +502,13 @@ __device__ __forceinline__ void stream_issue(const cplx<T> *__restrict__ G, cons
 }
 #pragma clang diagnostic pop
 
+// Right-hand sides h0 .. h0 + NRHS - 1 of the wavenumber in blk; partial rows into part [4][NRHS][C].
 template <typename T, int NRHS>
 __device__ __forceinline__ void stream_compute(const cplx<T> *__restrict__ blk, cplx<T> *__restrict__ part, int wave,
-                                               int lane)
+                                               int lane, int h0)
 {
     using namespace stream;
-    const cplx<T> *u = blk + GB;                                 // [NRHS][C]
+    const cplx<T> *u = blk + GB + h0 * C;                        // [NRHS][C] of this pass
     const int s = lane >> 5, r = lane & 31, r15 = r & MASK;
     const int cb = wave * 8 + s * 4;
     const int trir = r * (r + 1) / 2;
@@ -573,15 +583,15 @@ __device__ __forceinline__ void wait_groups(int after)
     }
 }
 
-template <typename T, int NRHS>
+template <typename T>
 __global__ __launch_bounds__(256) void contract_stream_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t L,
-                                                              int64_t M)
+                                                              int64_t M, int nrhs)
 {
     using namespace stream;
     using K = Cfg<T>;
     static_assert(K::NBUF <= 4, "wait_groups covers up to three groups in flight after the current one");
     __shared__ cplx<T> ring[K::NBUF * K::BUF];
-    __shared__ cplx<T> part[4 * NRHS * C];
+    __shared__ cplx<T> part[4 * 2 * C];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
     const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
     const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= L
@@ -589,25 +599,32 @@ __global__ __launch_bounds__(256) void contract_stream_kernel(const cplx<T> *__r
     const int64_t cnt = (L - 1 - l0) / g + 1;                    // wavenumbers of this workgroup
 #pragma unroll
     for (int p = 0; p < K::NBUF - 1; ++p)
-        if (p < cnt) stream_issue<T, NRHS>(G, buf, M, l0 + p * g, base + (unsigned)(p * K::BUF * K::E), wave, lane);
+        if (p < cnt)
+            stream_issue<T>(G, buf, M, l0 + p * g, nrhs, base + (unsigned)(p * K::BUF * K::E), wave, lane);
     for (int64_t i = 0; i < cnt; ++i) {
         const int64_t l = l0 + i * g;
         const int64_t ahead = i + K::NBUF - 1;                   // the wavenumber index to prefetch now
         if (ahead < cnt)
-            stream_issue<T, NRHS>(G, buf, M, l0 + ahead * g, base + (unsigned)((ahead % K::NBUF) * K::BUF * K::E),
-                                  wave, lane);
+            stream_issue<T>(G, buf, M, l0 + ahead * g, nrhs, base + (unsigned)((ahead % K::NBUF) * K::BUF * K::E),
+                            wave, lane);
         // groups issued after wavenumber i's: i+1 .. min(cnt-1, i+NBUF-1)
         const int64_t after = (cnt - 1 - i) < (K::NBUF - 1) ? (cnt - 1 - i) : (K::NBUF - 1);
         wait_groups<K::PER_WAVE>((int)after);
-        XTR_BARRIER();                                           // wavenumber i is in its buffer
-        stream_compute<T, NRHS>(ring + (i % K::NBUF) * K::BUF, part, wave, lane);
-        XTR_BARRIER();                                           // partials complete; buffer free
-        if (t < NRHS * C) {
-            cplx<T> acc = part[t];
+        const cplx<T> *blk = ring + (i % K::NBUF) * K::BUF;
+        // right-hand sides in passes of two over the same LDS-resident Green block
+        for (int h0 = 0; h0 < nrhs; h0 += 2) {
+            const int nh = nrhs - h0 >= 2 ? 2 : 1;
+            XTR_BARRIER();                                       // block in LDS / previous pass read
+            if (nh == 2) stream_compute<T, 2>(blk, part, wave, lane, h0);
+            else stream_compute<T, 1>(blk, part, wave, lane, h0);
+            XTR_BARRIER();                                       // partials complete
+            if (t < nh * C) {
+                cplx<T> acc = part[t];
 #pragma unroll
-            for (int w = 1; w < 4; ++w) acc = cadd<T>(acc, part[w * NRHS * C + t]);
-            const int h = t / C;
-            buf[h * M + l * C + (t - h * C)] = acc;
+                for (int w = 1; w < 4; ++w) acc = cadd<T>(acc, part[w * nh * C + t]);
+                const int h = h0 + t / C;
+                buf[h * M + l * C + (t % C)] = acc;
+            }
         }
     }
     wait_vm<0>();                                                // no copy may land after exit
@@ -630,19 +647,15 @@ static int device_cus()
     return cus[dev];
 }
 
-// The streaming kernel covers nz = 32 with one or two right-hand sides; other shapes use contract_kernel.
+// The streaming kernel covers nz = 32 with one to four right-hand sides; other shapes use contract_kernel.
 template <typename T>
 static bool contract_stream(const ApplyArgs &a, hipStream_t s)
 {
-    if (a.g.nz != stream::NZ || (a.nrhs != 1 && a.nrhs != 2) || contract_force_generic()) return false;
+    if (a.g.nz != stream::NZ || a.nrhs < 1 || a.nrhs > stream::kMaxStreamRhs || contract_force_generic()) return false;
     const int64_t L = a.g.L();
     const unsigned grid = (unsigned)std::min<int64_t>(L, device_cus());
-    if (a.nrhs == 2)
-        hipLaunchKernelGGL((contract_stream_kernel<T, 2>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
-                           (cplx<T> *)a.buf, L, a.g.M());
-    else
-        hipLaunchKernelGGL((contract_stream_kernel<T, 1>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
-                           (cplx<T> *)a.buf, L, a.g.M());
+    hipLaunchKernelGGL((contract_stream_kernel<T>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
+                       (cplx<T> *)a.buf, L, a.g.M(), a.nrhs);
     XTR_LAUNCH_CHECK("contract_stream_kernel");
     return true;
 }

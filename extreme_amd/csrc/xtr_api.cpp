@@ -108,7 +108,15 @@ struct xtr_op {
     DevBuf<double2> G64;          // [L][gblk] FP64 Green tensor
     DevBuf<float2> G32;           // [L][gblk] complex64 copy for XTR_FP32 and XTR_MIXED
     bool green_ready = false;
-    DevBuf<double2> R;            // [nx*ny*nz]
+    DevBuf<double2> R;            // [nx*ny*nz] of model 0
+    std::vector<std::unique_ptr<DevBuf<double2>>> Rm;   // R of models 1 .. nmodels-1 (same Green tensor)
+    int nmodels = 1;
+
+    const double2 *model_r(int m) const
+    {
+        require(m >= 0 && m < nmodels, "model index out of range (see xtr_op_prepare_models)");
+        return m == 0 ? R.p : Rm[m - 1]->p;
+    }
     DevBuf<double> f, b;          // [nz]
     DevBuf<uint16_t> packed_rc;   // [S][2]
     DevBuf<double2> tw64_x, tw64_y;
@@ -506,7 +514,7 @@ struct ApplyGate {
     int active;
     int pending = 0;
     uint64_t round = 0;
-    struct Sub { const double2 *x; double2 *y; hipStream_t s; hipEvent_t ready; };
+    struct Sub { const double2 *x; double2 *y; hipStream_t s; hipEvent_t ready; const double2 *R; };
     std::vector<Sub> subs;
     hipEvent_t done = nullptr;
     std::exception_ptr err;
@@ -525,6 +533,7 @@ struct ApplyGate {
                 XTR_CHECK(hipStreamWaitEvent(op->stream, subs[i].ready, 0));
                 v.x[i] = subs[i].x;
                 v.y[i] = subs[i].y;
+                v.R[i] = subs[i].R;
             }
             op->apply_raw(v, (int)subs.size(), op->stream);
             XTR_CHECK(hipEventRecord(done, op->stream));
@@ -538,12 +547,12 @@ struct ApplyGate {
         cv.notify_all();
     }
 
-    void submit(const double2 *x, double2 *y, hipStream_t s, hipEvent_t ready)
+    void submit(const double2 *x, double2 *y, hipStream_t s, hipEvent_t ready, const double2 *R)
     {
         XTR_CHECK(hipEventRecord(ready, s));
         std::unique_lock<std::mutex> lk(mu);
         if (err) std::rethrow_exception(err);
-        subs.push_back({x, y, s, ready});
+        subs.push_back({x, y, s, ready, R});
         ++pending;
         const uint64_t mine = round;
         if (pending == active) fire();
@@ -567,7 +576,7 @@ struct ApplyGate {
 
 // Side-by-side FGMRES solves of nrhs right-hand sides with one ApplyGate (shared applies).
 static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, const void *const *b, void *const *x,
-                        xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
+                        xtr_result_info *info, xtr_iteration_fn on_iteration, void *user, const int *model = nullptr)
 {
     ApplyGate gate(op, nrhs);
     std::vector<std::unique_ptr<SolveCtx>> ctxs;
@@ -596,7 +605,8 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
             hipStream_t s = streams[r];
             hipEvent_t ev = ready[r];
             ApplyGate *gp = &gate;
-            c->apply = [gp, s, ev](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev); };
+            const double2 *Rp = model ? op->model_r(model[r]) : nullptr;   // null: the handle's R
+            c->apply = [gp, s, ev, Rp](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev, Rp); };
             ctxs.push_back(std::move(c));
         }
         // inputs written on the caller's default stream must be visible to the solve streams
@@ -809,12 +819,20 @@ int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *
 
 int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta)
 {
+    const xtr_complex *z[1] = {zeta};
+    return xtr_op_prepare_models(op, type, 1, z);
+}
+
+int xtr_op_prepare_models(xtr_op *op, int type, int nmodels, const xtr_complex *const *zeta)
+{
     return guarded([&] {
         require(op && zeta, "null argument");
         require(type == XTR_OP_A || type == XTR_OP_CHI0, "type must be XTR_OP_A or XTR_OP_CHI0");
+        require(nmodels >= 1 && nmodels <= xtr::kMaxRhs, "nmodels must be 1..8");
+        for (int m = 0; m < nmodels; ++m) require(zeta[m] != nullptr, "null zeta");
         const xtr::Geom &g = op->g;
         const int nz = g.nz;
-        // GetBackgroundZeta (ConvolutionOperator.cs:80-88)
+        // GetBackgroundZeta (ConvolutionOperator.cs:80-88): the background, so shared by the models
         std::vector<cd> z0(nz);
         for (int k = 0; k < nz; ++k) {
             const int li = background_layer(op->section, op->depth[k], op->depth[k] + op->thick[k]);
@@ -827,20 +845,42 @@ int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta)
             f[k] = std::sqrt(z0[k].real());
             b[k] = 2 * std::sqrt(z0[k].real()) / op->thick[k] * nf;
         }
-        // PrepareRFunction (:113-143)
-        std::vector<double2> R((size_t)g.nx * g.ny * nz);
-        for (int64_t idx = 0; idx < (int64_t)R.size(); ++idx) {
-            const int k = (int)(idx % nz);
-            const cd z(zeta[idx].re, zeta[idx].im);
-            const cd cz0 = std::conj(z0[k]);
-            const cd r = type == XTR_OP_CHI0 ? std::sqrt(cd(z0[k].real(), 0)) / (z + cz0) : (z - z0[k]) / (z + cz0);
-            R[idx] = make_double2(r.real(), r.imag());
-        }
         XTR_CHECK(hipStreamSynchronize(op->stream));
-        XTR_CHECK(hipMemcpy(op->R.p, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice));
+        while ((int)op->Rm.size() < nmodels - 1) op->Rm.push_back(std::make_unique<DevBuf<double2>>((size_t)g.nx * g.ny * nz));
+        // PrepareRFunction (:113-143) per model
+        std::vector<double2> R((size_t)g.nx * g.ny * nz);
+        for (int m = 0; m < nmodels; ++m) {
+            for (int64_t idx = 0; idx < (int64_t)R.size(); ++idx) {
+                const int k = (int)(idx % nz);
+                const cd z(zeta[m][idx].re, zeta[m][idx].im);
+                const cd cz0 = std::conj(z0[k]);
+                const cd r = type == XTR_OP_CHI0 ? std::sqrt(cd(z0[k].real(), 0)) / (z + cz0) : (z - z0[k]) / (z + cz0);
+                R[idx] = make_double2(r.real(), r.imag());
+            }
+            double2 *dst = m == 0 ? op->R.p : op->Rm[m - 1]->p;
+            XTR_CHECK(hipMemcpy(dst, R.data(), R.size() * sizeof(double2), hipMemcpyHostToDevice));
+        }
         XTR_CHECK(hipMemcpy(op->f.p, f.data(), nz * sizeof(double), hipMemcpyHostToDevice));
         XTR_CHECK(hipMemcpy(op->b.p, b.data(), nz * sizeof(double), hipMemcpyHostToDevice));
         op->type = type;
+        op->nmodels = nmodels;
+    });
+}
+
+int xtr_op_apply_models_device(xtr_op *op, int nrhs, const int *model, const void *const *x, void *const *y)
+{
+    return guarded([&] {
+        require(op && model && x && y, "null argument");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs out of range");
+        xtr::RhsPtrs v{};
+        for (int r = 0; r < nrhs; ++r) {
+            v.x[r] = (const double2 *)x[r];
+            v.y[r] = (double2 *)y[r];
+            v.R[r] = op->model_r(model[r]);
+        }
+        XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
+        op->apply_raw(v, nrhs, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
     });
 }
 
@@ -970,6 +1010,19 @@ int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, in
     });
 }
 
+
+int xtr_fgmres_solve_models_device(xtr_op *op, const xtr_gmres_params *params, int nrhs, const int *model,
+                                   const void *const *b, void *const *x, xtr_result_info *info,
+                                   xtr_iteration_fn on_iteration, void *user)
+{
+    return guarded([&] {
+        require(op && params && model && b && x && info, "null argument");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs must be within 1..max_rhs");
+        for (int r = 0; r < nrhs; ++r) (void)op->model_r(model[r]);
+        solve_batch(op, params, nrhs, b, x, info, on_iteration, user, model);
+    });
+}
 
 int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const void *const *b,
                                    void *const *x, xtr_refine_info *info)

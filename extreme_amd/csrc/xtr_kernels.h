@@ -29,9 +29,12 @@ __host__ __device__ inline int asym_swizzle_mask(int nz) { return nz % 16 == 0 ?
 __host__ __device__ inline int asym_pos(int rc, int tr, int nz, int mask) { return rc * nz + (tr ^ (rc & mask)); }
 
 // Per-call vector pointers (device, FP64) for up to kMaxRhs right-hand sides.
+// R[i] is the R function of right-hand side i's model ([nx*ny*nz]); models that share the grid,
+// background and frequency share the Green tensor, so one apply can serve several of them.
 struct RhsPtrs {
     const double2 *x[kMaxRhs];
     double2 *y[kMaxRhs];
+    const double2 *R[kMaxRhs];
 };
 
 // ---- operator apply (op_kernels.hip) -------------------------------------------------------
@@ -42,7 +45,7 @@ struct ApplyArgs {
     int precision;
     int nrhs;
     int type;                 // 0 = A, 1 = Chi0
-    const double2 *R;         // [nx*ny*nz] FP64
+    const double2 *R;         // [nx*ny*nz] FP64, used for right-hand sides whose v.R[i] is null
     const double *f;          // [nz] forward factors (real)
     const double *b;          // [nz] backward factors (real)
     const void *G;            // [L][gblk] in apply precision

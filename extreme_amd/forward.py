@@ -203,3 +203,46 @@ class FrequencySolve:
         self.chi.copy_(self.chi0)
         return self.solver.solve_refined_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]],
                                                inner_floor=inner_floor, max_refinements=max_refinements)
+
+
+class ModelGroupSolve:
+    """Several conductivity models on one grid, background and frequency solved together: one Green
+    tensor, chi0 of every (model, polarisation) from one Chi0 apply, and one batched FGMRES whose
+    every apply streams the Green tensor once for all right-hand sides (the reference solves each
+    model and polarisation in turn, Mt3DForwardSolver.cs:78-129, reusing the tensor through
+    TensorCache, Forward/TensorCache.cs:8-35)."""
+
+    def __init__(self, models, frequency: float, settings: X.ForwardSettings | None = None):
+        import torch
+        self.torch = torch
+        base = models[0]
+        for m in models[1:]:
+            same = (m.nx, m.ny, m.nz, m.dx, m.dy) == (base.nx, base.ny, base.nz, base.dx, base.dy) and \
+                np.array_equal(m.depth, base.depth) and np.array_equal(m.thick, base.thick) and \
+                np.array_equal(m.section.sigma, base.section.sigma) and \
+                np.array_equal(m.section.thickness, base.section.thickness)
+            if not same:
+                raise ValueError("models of a group must share the grid and the background")
+        self.models = list(models)
+        self.frequency = frequency
+        self.omega = cm.omega_of(frequency)
+        self.settings = settings or X.ForwardSettings()
+        K = len(models)
+        self.op = X.ConvolutionOperator(base, self.omega, self.settings, max_rhs=2 * K)
+        self.op.calculate_green_tensor()
+        e0 = normal_field(base, self.omega)
+        self.rhs_model = [k for k in range(K) for _ in (0, 1)]
+        self.js = [scattered_current(m, self.omega, pol, e0) for m in models for pol in (0, 1)]
+        zetas = [m.zeta(self.omega) for m in models]
+        self.op.prepare_models("Chi0", zetas)
+        js_dev = [torch.from_numpy(j).to("cuda") for j in self.js]
+        self.chi0 = self.op.apply_models(js_dev, self.rhs_model)
+        self.op.prepare_models("A", zetas)
+        self.solver = X.AnomalyCurrentFgmresSolver(self.op, self.settings)
+        self.chi = [c.clone() for c in self.chi0]
+
+    def solve(self):
+        for c, c0 in zip(self.chi, self.chi0):
+            c.copy_(c0)
+        self.solver.history.clear()
+        return self.solver.solve_models_batch(self.chi0, self.chi, self.rhs_model)

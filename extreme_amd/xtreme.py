@@ -92,6 +92,12 @@ def lib():
         L.xtr_refined_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_RefineParams), C.c_int,
                                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                      C.POINTER(_RefineInfo)]
+        L.xtr_op_prepare_models.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.xtr_op_apply_models_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                                 C.POINTER(C.c_void_p)]
+        L.xtr_fgmres_solve_models_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_int, C.POINTER(C.c_int),
+                                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                                     C.POINTER(_Info), ITER_FN, C.c_void_p]
         L.xtr_op_build_green_atoo.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.xtr_op_get_green_atoo.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.xtr_op_jq_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -248,6 +254,29 @@ class ConvolutionOperator:
         _check(lib().xtr_op_prepare(self._h, t, C.c_void_p(z.ctypes.data)))
         self.operator_type = operator_type
 
+    def prepare_models(self, operator_type: str, zetas):
+        """PrepareOperator for several conductivity models that share the grid, background and
+        frequency (one Green tensor): zetas[m] is model m's [Nx][Ny][Nz] complex conductivity."""
+        arrs = [np.ascontiguousarray(np.asarray(z, dtype=np.complex128).reshape(-1)) for z in zetas]
+        n = self.model.nx * self.model.ny * self.model.nz
+        if any(a.size != n for a in arrs):
+            raise ValueError("each zeta must hold Nx*Ny*Nz values")
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        t = {"A": XTR_OP_A, "Chi0": XTR_OP_CHI0}[operator_type]
+        _check(lib().xtr_op_prepare_models(self._h, t, len(arrs), ptrs))
+        self.operator_type = operator_type
+
+    def apply_models(self, xs, models, ys=None):
+        """One apply of device vectors xs[r] on models[r]: the Green tensor is streamed once."""
+        n = len(xs)
+        if ys is None:
+            ys = [x.new_empty(x.shape) for x in xs]
+        mi = (C.c_int * n)(*[int(m) for m in models])
+        xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        yp = (C.c_void_p * n)(*[y.data_ptr() for y in ys])
+        _check(lib().xtr_op_apply_models_device(self._h, n, mi, xp, yp))
+        return ys
+
     def apply(self, x, y=None):
         """Apply(input, output): numpy arrays (host) or torch GPU tensors (device), n or [nrhs, n]."""
         nrhs = 1 if x.ndim == 1 else x.shape[0]
@@ -360,6 +389,19 @@ class AnomalyCurrentFgmresSolver:
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
         cb = self._cb()
         _check(lib().xtr_fgmres_solve_batch_device(self.op._h, C.byref(self.params), n, bp, xp, infos, cb, None))
+        return [ResultInfo._from(infos[i]) for i in range(n)]
+
+    def solve_models_batch(self, bs, xs, models) -> list:
+        """Side-by-side solves of right-hand sides on different models that share the Green tensor
+        (operator prepared with prepare_models("A", ...)); device tensors."""
+        n = len(bs)
+        infos = (_Info * n)()
+        mi = (C.c_int * n)(*[int(m) for m in models])
+        bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
+        xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        cb = self._cb()
+        _check(lib().xtr_fgmres_solve_models_device(self.op._h, C.byref(self.params), n, mi, bp, xp, infos, cb,
+                                                    None))
         return [ResultInfo._from(infos[i]) for i in range(n)]
 
     def solve_refined_batch(self, bs, xs, tolerance=None, inner_floor=1e-8, max_refinements=10) -> list:

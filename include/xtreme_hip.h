@@ -114,6 +114,17 @@ int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *
  * zeta: host [Nx][Ny][Nz] complex conductivity of the anomaly (OmegaAnomaly.Zeta). */
 int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta);
 
+/* Replaces: ConvolutionOperator.PrepareOperator (ConvolutionOperator.cs:64-143) for nmodels
+ * conductivity models that share the grid, the background and the frequency, hence the Green
+ * tensor and the forward/backward factors: zeta[m] is model m's [Nx][Ny][Nz] (the reference
+ * prepares one model per operator; Mt3DForwardSolver.cs:78-112 reuses its Green tensor through
+ * TensorCache). xtr_op_prepare is the nmodels = 1 case. */
+int xtr_op_prepare_models(xtr_op *op, int type, int nmodels, const xtr_complex *const *zeta);
+
+/* One apply of nrhs device vectors x[r] -> y[r], right-hand side r on model model[r]: the Green
+ * tensor is streamed once for all of them. */
+int xtr_op_apply_models_device(xtr_op *op, int nrhs, const int *model, const void *const *x, void *const *y);
+
 /* Replaces: ConvolutionOperator.Apply (ConvolutionOperator.cs:45-62,162-327).
  * x, y: host arrays of nrhs consecutive vectors of n = 3*Nx*Ny*Nz complex values. */
 int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs);
@@ -144,6 +155,14 @@ int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const vo
 int xtr_fgmres_solve_batch_device(xtr_op *op, const xtr_gmres_params *params, int nrhs,
                                   const void *const *b, void *const *x, xtr_result_info *info,
                                   xtr_iteration_fn on_iteration, void *user);
+
+/* xtr_fgmres_solve_batch_device for right-hand sides of different models sharing the Green
+ * tensor (prepared by xtr_op_prepare_models as XTR_OP_A): right-hand side r solves model model[r]'s
+ * system; every apply serves all running right-hand sides (the batch scheduler's grouping of
+ * (model, polarisation) pairs, Mt3DForwardSolver.cs:78-129 per model). */
+int xtr_fgmres_solve_models_device(xtr_op *op, const xtr_gmres_params *params, int nrhs, const int *model,
+                                   const void *const *b, void *const *x, xtr_result_info *info,
+                                   xtr_iteration_fn on_iteration, void *user);
 
 /* Mixed-precision solve: FP64 iterative refinement around complex64 FGMRES correction solves.
  * No counterpart in the reference (its arithmetic is complex128 throughout); it solves the same
