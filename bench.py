@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision pass (complex64 operator inside FP64 refinement)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-models", action="store_true",
+                    help="skip the two-model batching measurement (two seeded models on one Green tensor)")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the known-size kernel that "
@@ -144,6 +146,11 @@ def main():
     if not args.no_mixed:
         mixed = mixed_pass(problems, args.steps, elapsed / args.steps)
 
+    # ---- batching two models that share the Green tensor (4 right-hand sides per apply) --------
+    batching = None
+    if not args.no_models and args.config == "c2":
+        batching = model_batching(problems[min(4, nsolve - 1)], settings)
+
     # ---- the host-pointer boundary: one apply of both polarisations with PCIe copies ---------
     pcie = host_boundary_apply(problems[0])
 
@@ -189,6 +196,7 @@ def main():
         "cpu_baseline": cpu,
         "host_boundary": pcie,
         "mixed_precision": mixed,
+        "model_batching": batching,
     }
     print(json.dumps(line), flush=True)
 
@@ -223,6 +231,36 @@ def mixed_pass(problems, steps, fp64_s_per_freq):
             "fp32_applies": [[i.matvecs32 for i in inf] for _, inf in results],
             "fp64_applies": [[i.matvecs64 for i in inf] for _, inf in results],
             "solution_rel_diff_vs_fp64_max": max(diffs)}
+
+
+def model_batching(problem0, settings):
+    """Two seeded C2 models (same grid and background, different boxes) at one frequency: solved
+    one after the other (2 right-hand sides per apply each) and as one group (4 per apply)."""
+    import torch
+    from extreme_amd import cartesian_model as cm
+    from extreme_amd import forward as F
+    f = problem0.frequency
+    m1 = cm.config_c2(model_index=1)
+    p1 = F.FrequencySolve(m1, f, settings)
+    grp = F.ModelGroupSolve([problem0.model, m1], f, settings)
+    p1.solve()
+    grp.solve()                                  # warm both paths
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    problem0.solve()
+    p1.solve()
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    infos = grp.solve()
+    torch.cuda.synchronize()
+    grouped = time.perf_counter() - t0
+    out = {"frequency_hz": f, "models": 2, "right_hand_sides": 4, "sequential_s": seq, "grouped_s": grouped,
+           "speedup": seq / grouped, "iterations": [i.number_of_iterations for i in infos],
+           "converged": all(i.convergence_achieved for i in infos)}
+    del grp, p1
+    torch.cuda.empty_cache()
+    return out
 
 
 def host_boundary_apply(problem, reps=3):
