@@ -351,7 +351,8 @@ def large_config(model, args):
         "config": {"workload": f"{m.name}: {m.nx}x{m.ny}x{m.nz} cells, 3-layer background + slab, 60 boxes, "
                                f"{args.steps} of {len(m.frequencies)} frequencies one at a time, both polarisations, "
                                "FGMRES(30)/GMRES(10), tol 1e-8", "nx": m.nx, "ny": m.ny, "nz": m.nz},
-        "roofline": {"kernel": "contract_kernel<double>", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "contract_stream64_kernel<2>" if nz == 64 else "contract_kernel<double>",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": contract_ms},
         "cpu_baseline": None,

@@ -630,6 +630,220 @@ __global__ __launch_bounds__(256) void contract_stream_kernel(const cplx<T> *__r
     wait_vm<0>();                                                // no copy may land after exit
 }
 
+// ---- streaming contraction, nz = 64 --------------------------------------------------------------------
+// A wavenumber's Green block (264 KB in FP64, 132 KB in FP32) is streamed as four units through a
+// ring of 72-wave-load slots: U0 = xx|xy (+ the right-hand sides' spectrum columns), U1 = yy|zz,
+// U2 = xz, U3 = yz. The units of all the workgroup's wavenumbers form one stream: step s contracts
+// unit s while units s+1 .. s+NSLOT-1 are in flight (FP64: 2 slots of 72 KB; FP32: 4 slots of 36 KB).
+// Wave w owns columns [16w, 16w+16); lane (hc, r) owns rows r and r + 32 and columns 16w + 8hc + j
+// (j < 8); accumulators persist across a wavenumber's four units. The spectrum columns are copied
+// from U0's slot into a small LDS area that outlives the slot; partial rows are reduced in U3's slot.
+namespace stream64 {
+constexpr int NZ = 64, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
+constexpr int MASK = 15;                  // asym_swizzle_mask(64)
+constexpr int SLOT = 72 * 64;             // elements per slot: 72 wave-loads of 1 KB (FP64), 36 (FP32)
+constexpr int MAXRHS = 2;
+__host__ __device__ constexpr int ubegin(int u) { return u == 0 ? 0 : u == 1 ? 2 * S : u == 2 ? 4 * S : 4 * S + A; }
+__host__ __device__ constexpr int ulen(int u) { return u < 2 ? 2 * S : A; }
+__host__ __device__ constexpr int uimage(int u) { return u == 0 ? 2 * S + MAXRHS * C : ulen(u); }
+template <typename T>
+struct Cfg {
+    static constexpr int E = (int)sizeof(cplx<T>);   // bytes per element
+    static constexpr int EPL = 16 / E;               // elements per lane and wave-load
+    static constexpr int EPC = 64 * EPL;             // elements per wave-load
+    static constexpr int NSLOT = E == 16 ? 2 : 4;
+    __host__ __device__ static constexpr int upw(int u) { return ((uimage(u) + EPC - 1) / EPC + 3) / 4; }
+    // wave-loads a wave has issued after unit u's when step u waits (`last`: no next wavenumber)
+    __host__ __device__ static constexpr int ahead(int u, bool last)
+    {
+        int n = 0;
+        for (int k = 1; k < NSLOT; ++k) {
+            if (last && u + k > 3) break;
+            n += upw((u + k) % 4);
+        }
+        return n;
+    }
+};
+static_assert(4 * Cfg<double>::upw(0) * Cfg<double>::EPC <= SLOT && 4 * Cfg<float>::upw(0) * Cfg<float>::EPC <= SLOT,
+              "slot size");
+static_assert(2 * SLOT * 16 + MAXRHS * C * 16 <= 160 * 1024 && 4 * SLOT * 8 + MAXRHS * C * 8 <= 160 * 1024, "LDS");
+static_assert(Cfg<double>::ahead(0, false) < 64 && Cfg<float>::ahead(0, false) < 64, "vmcnt range");
+}  // namespace stream64
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+template <typename T, int U>
+__device__ __forceinline__ void s64_issue(const cplx<T> *__restrict__ G, const cplx<T> *buf, int64_t M, int64_t l,
+                                          int nrhs, unsigned dst, int wave, int lane)
+{
+    using namespace stream64;
+    using K = Cfg<T>;
+#pragma unroll
+    for (int t = 0; t < K::upw(U); ++t) {
+        const int k = wave + 4 * t;
+        const int e = k * K::EPC + lane * K::EPL;        // first element of this unit's LDS image this lane copies
+        const cplx<T> *src;
+        if (e < ulen(U)) {
+            src = G + l * GB + ubegin(U) + e;
+        } else if (U == 0) {
+            int q = e - ulen(0);                         // spectrum column q of this wavenumber
+            if (q > nrhs * C - K::EPL) q = nrhs * C - K::EPL;   // padding lanes re-read valid data
+            const int rhs = q / C;
+            src = buf + rhs * M + l * C + (q - rhs * C);
+        } else {
+            src = G + l * GB + ubegin(U) + ulen(U) - K::EPL;     // padding lanes re-read valid data
+        }
+        const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+    }
+}
+#pragma clang diagnostic pop
+
+template <typename T, int U, int NRHS>
+__device__ __forceinline__ void s64_compute(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua, int wave,
+                                            int lane, cplx<T> (&yx)[2][NRHS], cplx<T> (&yy)[2][NRHS],
+                                            cplx<T> (&yz)[2][NRHS])
+{
+    using namespace stream64;
+    const int hc = lane >> 5, r0 = lane & 31;
+    const int cb = wave * 16 + hc * 8;
+#pragma unroll 2
+    for (int j = 0; j < 8; ++j) {
+        const int c = cb + j;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = r0 + 32 * q;
+            cplx<T> g0, g1;
+            if (U < 2) {
+                const int p = c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2;   // packed (min, max)
+                g0 = blk[p];
+                g1 = blk[S + p];
+            } else {
+                g0 = blk[r * NZ + (c ^ (r & MASK))];       // B(r, c)
+                g1 = blk[c * NZ + (r ^ (c & MASK))];       // B(c, r)
+            }
+#pragma unroll
+            for (int h = 0; h < NRHS; ++h) {
+                const cplx<T> *u = ua + h * C;
+                if (U == 0) {          // xx, xy
+                    const cplx<T> ux = u[c], uy = u[NZ + c];
+                    yx[q][h] = cfma<T>(cfma<T>(yx[q][h], g0, ux), g1, uy);
+                    yy[q][h] = cfma<T>(yy[q][h], g1, ux);
+                } else if (U == 1) {   // yy, zz
+                    yy[q][h] = cfma<T>(yy[q][h], g0, u[NZ + c]);
+                    yz[q][h] = cfma<T>(yz[q][h], g1, u[2 * NZ + c]);
+                } else if (U == 2) {   // xz: row product, then -xz^T
+                    yx[q][h] = cfma<T>(yx[q][h], g0, u[2 * NZ + c]);
+                    yz[q][h] = cfms<T>(yz[q][h], g1, u[c]);
+                } else {               // yz: row product, then -yz^T
+                    yy[q][h] = cfma<T>(yy[q][h], g0, u[2 * NZ + c]);
+                    yz[q][h] = cfms<T>(yz[q][h], g1, u[NZ + c]);
+                }
+            }
+        }
+    }
+}
+
+// One step of the unit stream: contract unit U of wavenumber l (slot U % NSLOT) after issuing unit
+// U + NSLOT - 1 into the slot the previous step freed.
+template <typename T, int NRHS, int U>
+__device__ __forceinline__ void s64_step(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t M, int64_t l,
+                                         int64_t lnext, bool has_next, cplx<T> *ring, cplx<T> *ua, unsigned base,
+                                         int wave, int lane, int t, cplx<T> (&yx)[2][NRHS], cplx<T> (&yy)[2][NRHS],
+                                         cplx<T> (&yz)[2][NRHS])
+{
+    using namespace stream64;
+    using K = Cfg<T>;
+    constexpr int V = U + K::NSLOT - 1;                 // unit issued now
+    constexpr unsigned vdst = (unsigned)((V % K::NSLOT) * SLOT * K::E);
+    if (V <= 3) s64_issue<T, V % 4>(G, buf, M, l, NRHS, base + vdst, wave, lane);
+    else if (has_next) s64_issue<T, V % 4>(G, buf, M, lnext, NRHS, base + vdst, wave, lane);
+    if (has_next) wait_vm<K::ahead(U, false)>();
+    else wait_vm<K::ahead(U, true)>();
+    XTR_BARRIER();                                      // unit U in LDS for every wave
+    cplx<T> *blk = ring + (U % K::NSLOT) * SLOT;
+    if (U == 0) {
+        for (int o = t; o < NRHS * C; o += 256) ua[o] = blk[ulen(0) + o];   // spectrum columns -> u area
+        XTR_BARRIER();
+    }
+    s64_compute<T, U, NRHS>(blk, ua, wave, lane, yx, yy, yz);
+    if (U == 3) {
+        // rows r0 and r0 + 32 summed over the lane halves, then over the waves in this slot
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < NRHS; ++h) {
+                yx[q][h].x += __shfl_xor(yx[q][h].x, 32, 64);
+                yx[q][h].y += __shfl_xor(yx[q][h].y, 32, 64);
+                yy[q][h].x += __shfl_xor(yy[q][h].x, 32, 64);
+                yy[q][h].y += __shfl_xor(yy[q][h].y, 32, 64);
+                yz[q][h].x += __shfl_xor(yz[q][h].x, 32, 64);
+                yz[q][h].y += __shfl_xor(yz[q][h].y, 32, 64);
+            }
+        XTR_BARRIER();                                  // slot read by every wave
+        cplx<T> *part = blk;                            // [4][NRHS][C]
+        const int hc = lane >> 5, r = (lane & 31) + 32 * hc;   // half hc stores row r0 + 32 hc
+#pragma unroll
+        for (int h = 0; h < NRHS; ++h) {
+            cplx<T> *pw = part + (wave * NRHS + h) * C;
+            // component-wise selects (a select of whole vector values would go through scratch)
+            pw[r] = mk<T>(hc ? yx[1][h].x : yx[0][h].x, hc ? yx[1][h].y : yx[0][h].y);
+            pw[NZ + r] = mk<T>(hc ? yy[1][h].x : yy[0][h].x, hc ? yy[1][h].y : yy[0][h].y);
+            pw[2 * NZ + r] = mk<T>(hc ? yz[1][h].x : yz[0][h].x, hc ? yz[1][h].y : yz[0][h].y);
+        }
+        XTR_BARRIER();
+        for (int o = t; o < NRHS * C; o += 256) {
+            const int h = o / C, qq = o % C;
+            cplx<T> acc = part[h * C + qq];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) acc = cadd<T>(acc, part[(w * NRHS + h) * C + qq]);
+            buf[h * M + l * C + qq] = acc;
+        }
+    }
+    XTR_BARRIER();                                      // slot free for the next step's copy
+}
+
+template <typename T, int NRHS>
+__global__ __launch_bounds__(256) void contract_stream64_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
+                                                                int64_t L, int64_t M)
+{
+    using namespace stream64;
+    using K = Cfg<T>;
+    __shared__ cplx<T> ring[K::NSLOT * SLOT];
+    __shared__ cplx<T> ua[MAXRHS * C];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
+    const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= L
+    const int64_t l0 = blockIdx.x;
+    const int64_t cnt = (L - 1 - l0) / g + 1;
+    // units 0 .. NSLOT-2 of the first wavenumber (NSLOT - 1 <= 3)
+    s64_issue<T, 0>(G, buf, M, l0, NRHS, base, wave, lane);
+    if (K::NSLOT > 2) {
+        s64_issue<T, 1>(G, buf, M, l0, NRHS, base + (unsigned)(SLOT * K::E), wave, lane);
+        s64_issue<T, 2>(G, buf, M, l0, NRHS, base + (unsigned)(2 * SLOT * K::E), wave, lane);
+    }
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int64_t l = l0 + i * g, ln = l + g;
+        const bool nx = i + 1 < cnt;
+        cplx<T> yx[2][NRHS], yy[2][NRHS], yz[2][NRHS];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < NRHS; ++h) {
+                yx[q][h] = mk<T>(0, 0);
+                yy[q][h] = yx[q][h];
+                yz[q][h] = yx[q][h];
+            }
+        s64_step<T, NRHS, 0>(G, buf, M, l, ln, nx, ring, ua, base, wave, lane, t, yx, yy, yz);
+        s64_step<T, NRHS, 1>(G, buf, M, l, ln, nx, ring, ua, base, wave, lane, t, yx, yy, yz);
+        s64_step<T, NRHS, 2>(G, buf, M, l, ln, nx, ring, ua, base, wave, lane, t, yx, yy, yz);
+        s64_step<T, NRHS, 3>(G, buf, M, l, ln, nx, ring, ua, base, wave, lane, t, yx, yy, yz);
+    }
+    wait_vm<0>();                                                // no copy may land after exit
+}
+
+static bool contract_stream64(const ApplyArgs &a, hipStream_t s);
+
 // XTR_CONTRACT_KERNEL=generic selects contract_kernel for every shape (to compare the two kernels).
 static bool contract_force_generic()
 {
@@ -645,6 +859,29 @@ static int device_cus()
     if (dev < 0 || dev >= 64) throw hip_error("device index out of range");
     if (!cus[dev]) XTR_CHECK(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
     return cus[dev];
+}
+
+// nz = 64, one or two right-hand sides, FP64 or FP32
+template <typename T>
+static void launch_stream64(const ApplyArgs &a, unsigned grid, hipStream_t s)
+{
+    const int64_t L = a.g.L();
+    if (a.nrhs == 2)
+        hipLaunchKernelGGL((contract_stream64_kernel<T, 2>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
+                           (cplx<T> *)a.buf, L, a.g.M());
+    else
+        hipLaunchKernelGGL((contract_stream64_kernel<T, 1>), dim3(grid), dim3(256), 0, s, (const cplx<T> *)a.G,
+                           (cplx<T> *)a.buf, L, a.g.M());
+}
+
+static bool contract_stream64(const ApplyArgs &a, hipStream_t s)
+{
+    if (a.g.nz != stream64::NZ || (a.nrhs != 1 && a.nrhs != 2) || contract_force_generic()) return false;
+    const unsigned grid = (unsigned)std::min<int64_t>(a.g.L(), device_cus());
+    if (a.precision == 0) launch_stream64<double>(a, grid, s);
+    else launch_stream64<float>(a, grid, s);
+    XTR_LAUNCH_CHECK("contract_stream64_kernel");
+    return true;
 }
 
 // The streaming kernel covers nz = 32 with one to four right-hand sides; other shapes use contract_kernel.
@@ -664,6 +901,7 @@ template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
     if (contract_stream<T>(a, s)) return;
+    if (contract_stream64(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;

@@ -133,7 +133,37 @@ def test_streaming_contraction_matches_generic(oracle, torch_cuda, monkeypatch):
     assert rel(y_stream, y_generic) < 1e-14
 
 
-@pytest.mark.parametrize("nz", [5, 8, 24, 32])
+@pytest.mark.parametrize("shape", [(4, 4, 64), (12, 12, 64)])
+def test_apply_nz64_streaming(oracle, torch_cuda, monkeypatch, shape):
+    """nz = 64 runs the four-unit streaming contraction (the Green block does not fit LDS): fewer
+    wavenumbers than CUs and an uneven split, one and two right-hand sides, against the oracle and
+    against the generic kernel."""
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nx * 3 + ny)
+    w = cm.omega_of(m.frequencies[0])
+    G = _random_green(oracle, m, seed=nx * ny + 64)
+    op = X.ConvolutionOperator(m, w, max_rhs=2)
+    op.set_green_tensor(G)
+    oop = oracle.Operator(m, G)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator("A", z)
+    oop.prepare("A", z, z0)
+    rng = np.random.default_rng(17)
+    xs = rng.standard_normal((2, m.n)) + 1j * rng.standard_normal((2, m.n))
+    ref = np.stack([oop.apply(xs[0]), oop.apply(xs[1])])
+    y2 = op.apply(xs)
+    y1 = op.apply(xs[1])
+    monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
+    yg = op.apply(xs)
+    monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+    print(f"nz64 {shape}: nrhs2 {rel(y2, ref):.3e} nrhs1 {rel(y1, ref[1]):.3e} "
+          f"vs generic {rel(y2, yg):.3e}", flush=True)
+    assert rel(y2, ref) < TOL_FP64
+    assert rel(y1, ref[1]) < TOL_FP64
+    assert rel(y2, yg) < 1e-14
+
+
+@pytest.mark.parametrize("nz", [5, 8, 24, 32, 64])
 def test_green_set_get_roundtrip(oracle, torch_cuda, nz):
     """Reference layout -> device layout (swizzled asymmetric blocks) -> reference layout, bit-exact."""
     m = cm.small_model(3, 2, nz, seed=nz)
@@ -146,10 +176,11 @@ def test_green_set_get_roundtrip(oracle, torch_cuda, nz):
         assert np.array_equal(back[c].reshape(G[c].shape), G[c]), c
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 32), (24, 12, 32)])
-def test_apply_nz32_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
-    """complex64 streaming contraction (4-buffer ring): within the FP32 bound of the FP64 oracle and
-    equal to the complex64 generic kernel up to summation order."""
+@pytest.mark.parametrize("shape", [(4, 4, 32), (24, 12, 32), (4, 4, 64), (12, 12, 64)])
+def test_apply_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
+    """complex64 streaming contractions (nz = 32: whole blocks through a 4-buffer ring; nz = 64: four
+    units per block through a 4-slot ring): within the FP32 bound of the FP64 oracle and equal to the
+    complex64 generic kernel up to summation order."""
     nx, ny, nz = shape
     m = cm.small_model(nx, ny, nz, seed=nx * ny)
     w = cm.omega_of(m.frequencies[0])
@@ -167,7 +198,7 @@ def test_apply_nz32_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
     monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
     yg = op.apply(xs)
     monkeypatch.delenv("XTR_CONTRACT_KERNEL")
-    print(f"fp32 nz32 {shape}: vs oracle {rel(y2, ref):.2e} nrhs1 {rel(y1, ref[1]):.2e} "
+    print(f"fp32 {shape}: vs oracle {rel(y2, ref):.2e} nrhs1 {rel(y1, ref[1]):.2e} "
           f"vs generic {rel(y2, yg):.2e}", flush=True)
     assert rel(y2, ref) < TOL_FP32
     assert rel(y1, ref[1]) < TOL_FP32
