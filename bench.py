@@ -35,7 +35,14 @@ METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s
 # WRITE_SIZE (one counter per run) and scaled by the same counters on a 1 GiB device copy
 # (FETCH x 2.0000, WRITE x 1.0000); summary in profiles/contract_stream_kernel_c2_hbm_traffic.md.
 # Keyed by (workload, nz, right-hand sides per launch); a workload without an entry reports null.
-MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9}   # contract_stream_kernel<double, 2>
+MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9,    # contract_stream_kernel<double, 2>
+                    ("C5", 64, 2): 7.2476e10}   # contract_stream64_kernel<double, 2>
+
+
+def contract_kernel_name(nz):
+    """The FP64 contraction kernel a two-right-hand-side apply launches at this nz."""
+    return {32: "contract_stream_kernel<double, 2>", 64: "contract_stream64_kernel<double, 2>"}.get(
+        nz, "contract_kernel<double>")
 
 
 def main():
@@ -188,7 +195,7 @@ def main():
         "true_backward_error": be,
         "converged": conv,
         "green_seconds_per_freq": float(np.mean(green_s)),
-        "roofline": {"kernel": "contract_stream_kernel<double, 2>" if m.nz == 32 else "contract_kernel<double>", "bound": "hbm", "achieved": achieved_gbs,
+        "roofline": {"kernel": contract_kernel_name(m.nz), "bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
                      "avg_launch_ms": contract_avg_ms,
@@ -351,9 +358,9 @@ def large_config(model, args):
         "config": {"workload": f"{m.name}: {m.nx}x{m.ny}x{m.nz} cells, 3-layer background + slab, 60 boxes, "
                                f"{args.steps} of {len(m.frequencies)} frequencies one at a time, both polarisations, "
                                "FGMRES(30)/GMRES(10), tol 1e-8", "nx": m.nx, "ny": m.ny, "nz": m.nz},
-        "roofline": {"kernel": "contract_stream64_kernel<2>" if nz == 64 else "contract_kernel<double>",
-                     "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"kernel": contract_kernel_name(nz), "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
                      "avg_launch_ms": contract_ms},
         "cpu_baseline": None,
         "mixed_s_per_freq": (sum(r["mixed_s"] for r in rows) / len(rows)) if not args.no_mixed else None,

@@ -345,8 +345,10 @@ void launch_fft2_plain(double2 *data, int n1, int n2, int64_t ncols, const doubl
 template <typename T>
 __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t L,
                                                          int nz, int nrhs, int W, int64_t M,
-                                                         const uint16_t *__restrict__ packed_rc)
+                                                         const uint16_t *__restrict__ packed_rc, int fold_nx,
+                                                         int fold_ny)
 {
+    // fold_nx > 0: G is folded (block b(l), mirrored components negated on u and y)
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *blk = reinterpret_cast<cplx<T> *>(smem_raw);   // [W][nz][nz+1]
     const int ld = nz + 1;
@@ -363,7 +365,12 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
         const int w = e / per_w, rest = e % per_w;
         const int rhs = rest / C, q = rest % C;
         const int64_t l = l0 + w;
-        u[e] = (l < L) ? buf[rhs * M + l * C + q] : mk<T>(0, 0);
+        cplx<T> v = (l < L) ? buf[rhs * M + l * C + q] : mk<T>(0, 0);
+        if (fold_nx > 0 && l < L && q < 2 * nz) {
+            const FoldPos p = fold_pos(l, fold_nx, fold_ny);
+            if (q < nz ? p.mx : p.my) v = mk<T>(-v.x, -v.y);
+        }
+        u[e] = v;
     }
 
     const bool active = tid < W * per_w;
@@ -384,7 +391,8 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
         for (int e = tid; e < W * sz; e += nth) {
             const int ww = e / sz, o = e % sz;
             const int64_t l = l0 + ww;
-            const cplx<T> g = (l < L) ? G[l * gblk + off + o] : mk<T>(0, 0);
+            const int64_t gb = fold_nx > 0 ? fold_pos(l, fold_nx, fold_ny).b : l;
+            const cplx<T> g = (l < L) ? G[gb * gblk + off + o] : mk<T>(0, 0);
             cplx<T> *bb = blk + (int64_t)ww * nz * ld;
             if (sym) {
                 const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
@@ -424,7 +432,13 @@ __global__ __launch_bounds__(1024) void contract_kernel(const cplx<T> *__restric
     }
     if (active) {
         const int64_t l = l0 + w;
-        if (l < L) buf[rhs * M + l * C + q] = acc;
+        if (l < L) {
+            if (fold_nx > 0 && q < 2 * nz) {
+                const FoldPos p = fold_pos(l, fold_nx, fold_ny);
+                if (q < nz ? p.mx : p.my) acc = mk<T>(-acc.x, -acc.y);
+            }
+            buf[rhs * M + l * C + q] = acc;
+        }
     }
 }
 
@@ -900,8 +914,8 @@ static bool contract_stream(const ApplyArgs &a, hipStream_t s)
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
-    if (contract_stream<T>(a, s)) return;
-    if (contract_stream64(a, s)) return;
+    if (!a.folded && contract_stream<T>(a, s)) return;
+    if (!a.folded && contract_stream64(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;
@@ -912,7 +926,7 @@ static void contract(const ApplyArgs &a, hipStream_t s)
     const int64_t L = a.g.L();
     dim3 grid((unsigned)ceil_div(L, W));
     hipLaunchKernelGGL(contract_kernel<T>, grid, dim3(threads), lds, s, (const cplx<T> *)a.G, (cplx<T> *)a.buf, L,
-                       nz, a.nrhs, W, a.g.M(), a.packed_rc);
+                       nz, a.nrhs, W, a.g.M(), a.packed_rc, a.folded ? a.g.nx : 0, a.g.ny);
     XTR_LAUNCH_CHECK("contract_kernel");
 }
 
@@ -960,6 +974,102 @@ void launch_green_deinterleave(const double2 *dev, double2 *comp[6], const Geom 
     hipLaunchKernelGGL(green_interleave_kernel, dim3(2048), dim3(256), 0, s, comp[0], comp[1], comp[2], comp[3],
                        comp[4], comp[5], const_cast<double2 *>(dev), g.L(), g.nz, 1);
     XTR_LAUNCH_CHECK("green_deinterleave_kernel");
+}
+
+// ---- folded Green tensor -------------------------------------------------------------------------
+// Sign of element o of a device block (xx, xy, yy, zz, xz, yz) in the image (mx, my) of its block.
+__device__ __forceinline__ double fold_sign(int64_t o, int64_t S, int64_t A, int mx, int my)
+{
+    if (o < S) return 1.0;
+    if (o < 2 * S) return (mx ^ my) ? -1.0 : 1.0;
+    if (o < 4 * S) return 1.0;
+    if (o < 4 * S + A) return mx ? -1.0 : 1.0;
+    return my ? -1.0 : 1.0;
+}
+
+// dir 0: folded[b] = full[source wavenumber of b]; dir 1: full[l] = sign * folded[b(l)]
+__global__ void green_fold_kernel(double2 *full, double2 *folded, int nx, int ny, int nz, int dir)
+{
+    const int64_t S = (int64_t)nz * (nz + 1) / 2, A = (int64_t)nz * nz, gblk = 4 * S + 2 * A;
+    const int64_t total = (dir == 0 ? (int64_t)(nx + 1) * (ny + 1) : 4LL * nx * ny) * gblk;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e / gblk, o = e % gblk;
+        if (dir == 0) {
+            const int64_t qx = k / (ny + 1), qy = k % (ny + 1);
+            folded[e] = full[(qx * 2 * ny + qy) * gblk + o];
+        } else {
+            const FoldPos p = fold_pos(k, nx, ny);
+            const double sg = fold_sign(o, S, A, p.mx, p.my);
+            const double2 v = folded[p.b * gblk + o];
+            full[e] = make_double2(sg * v.x, sg * v.y);
+        }
+    }
+}
+
+void launch_green_fold(const double2 *full, double2 *folded, const Geom &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(green_fold_kernel, dim3(2048), dim3(256), 0, s, const_cast<double2 *>(full), folded, g.nx, g.ny,
+                       g.nz, 0);
+    XTR_LAUNCH_CHECK("green_fold_kernel");
+}
+
+void launch_green_unfold(const double2 *folded, double2 *full, const Geom &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(green_fold_kernel, dim3(2048), dim3(256), 0, s, full, const_cast<double2 *>(folded), g.nx, g.ny,
+                       g.nz, 1);
+    XTR_LAUNCH_CHECK("green_unfold_kernel");
+}
+
+// Non-negative doubles order like their bit patterns, so an unsigned 64-bit atomic max is a max.
+__global__ __launch_bounds__(256) void green_mirror_check_kernel(const double2 *full, int nx, int ny, int nz,
+                                                                 unsigned long long *stat)
+{
+    const int64_t S = (int64_t)nz * (nz + 1) / 2, A = (int64_t)nz * nz, gblk = 4 * S + 2 * A;
+    const int64_t total = 4LL * nx * ny * gblk;
+    double m[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) m[c] = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = e / gblk, o = e % gblk;
+        const int c = o < 4 * S ? (int)(o / S) : 4 + (int)((o - 4 * S) / A);
+        const double2 v = full[e];
+        const double mag = fmax(fabs(v.x), fabs(v.y));
+        const FoldPos p = fold_pos(l, nx, ny);
+        double d = 0.0;
+        if (p.mx || p.my) {
+            const int64_t qx = p.b / (ny + 1), qy = p.b % (ny + 1);
+            const double sg = fold_sign(o, S, A, p.mx, p.my);
+            const double2 w = full[(qx * 2 * ny + qy) * gblk + o];
+            d = fmax(fabs(v.x - sg * w.x), fabs(v.y - sg * w.y));
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k == c) {
+                m[2 * k] = fmax(m[2 * k], d);
+                m[2 * k + 1] = fmax(m[2 * k + 1], mag);
+            }
+    }
+    __shared__ double red[12][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        double v = m[c];
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+        if (lane == 0) red[c][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const double v = fmax(fmax(red[threadIdx.x][0], red[threadIdx.x][1]), fmax(red[threadIdx.x][2], red[threadIdx.x][3]));
+        atomicMax(stat + threadIdx.x, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+void launch_green_mirror_check(const double2 *full, const Geom &g, double *stat, hipStream_t s)
+{
+    XTR_CHECK(hipMemsetAsync(stat, 0, 12 * sizeof(double), s));
+    hipLaunchKernelGGL(green_mirror_check_kernel, dim3(1024), dim3(256), 0, s, full, g.nx, g.ny, g.nz,
+                       reinterpret_cast<unsigned long long *>(stat));
+    XTR_LAUNCH_CHECK("green_mirror_check_kernel");
 }
 
 __global__ void to_f32_kernel(const double2 *src, float2 *dst, int64_t n)

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -73,6 +74,11 @@ struct DevBuf {
         count = 0;
     }
     ~DevBuf() { release(); }
+    void swap(DevBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(count, o.count);
+    }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
 };
@@ -105,9 +111,11 @@ struct xtr_op {
     std::vector<double> hank0, hank1;
 
     hipStream_t stream = nullptr;
-    DevBuf<double2> G64;          // [L][gblk] FP64 Green tensor
-    DevBuf<float2> G32;           // [L][gblk] complex64 copy for XTR_FP32 and XTR_MIXED
+    DevBuf<double2> G64;          // [nblk][gblk] FP64 Green tensor
+    DevBuf<float2> G32;           // [nblk][gblk] complex64 copy for XTR_FP32 and XTR_MIXED
     bool green_ready = false;
+    bool folded = false;          // G holds blocks kx <= Nx, ky <= Ny only (nblk = Lq, else L)
+    int64_t nblk() const { return folded ? g.Lq() : g.L(); }
     DevBuf<double2> R;            // [nx*ny*nz] of model 0
     std::vector<std::unique_ptr<DevBuf<double2>>> Rm;   // R of models 1 .. nmodels-1 (same Green tensor)
     int nmodels = 1;
@@ -171,6 +179,7 @@ struct xtr_op {
         a.f = f.p;
         a.b = b.p;
         a.G = p == XTR_FP64 ? (const void *)G64.p : (const void *)G32.p;
+        a.folded = folded ? 1 : 0;
         a.packed_rc = packed_rc.p;
         a.buf = buf.p;
         a.tw_x = p == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
@@ -743,6 +752,31 @@ static void finish_green(xtr_op *op)
     op->green_ready = true;
 }
 
+// XTR_GREEN_LAYOUT=full keeps every wavenumber's block (to compare the two layouts).
+static bool green_fold_allowed()
+{
+    const char *e = std::getenv("XTR_GREEN_LAYOUT");
+    return !(e && std::strcmp(e, "full") == 0);
+}
+
+// Takes a full FP64 tensor [L][gblk] in device layout and stores it folded (fold) or as it is.
+static void store_green(xtr_op *op, DevBuf<double2> &full, bool fold)
+{
+    const xtr::Geom &g = op->g;
+    op->G32.release();
+    op->folded = fold;
+    if (fold) {
+        op->G64.alloc((size_t)(g.Lq() * g.gblk()));
+        xtr::launch_green_fold(full.p, op->G64.p, g, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        full.release();
+    } else {
+        op->G64.swap(full);
+        full.release();
+    }
+    finish_green(op);
+}
+
 int xtr_op_set_green(xtr_op *op, const xtr_complex *xx, const xtr_complex *xy, const xtr_complex *xz,
                      const xtr_complex *yy, const xtr_complex *yz, const xtr_complex *zz)
 {
@@ -759,11 +793,26 @@ int xtr_op_set_green(xtr_op *op, const xtr_complex *xx, const xtr_complex *xy, c
             XTR_CHECK(hipMemcpy(tmp[c].p, host[c], L * sizes[c] * sizeof(double2), hipMemcpyHostToDevice));
             dev[c] = tmp[c].p;
         }
-        op->G64.alloc((size_t)(L * g.gblk()));
+        op->green_ready = false;
+        op->G64.release();
         op->G32.release();
-        xtr::launch_green_interleave(dev, op->G64.p, g, op->stream);
-        XTR_CHECK(hipStreamSynchronize(op->stream));
-        finish_green(op);
+        DevBuf<double2> full((size_t)(L * g.gblk()));
+        xtr::launch_green_interleave(dev, full.p, g, op->stream);
+        for (auto &t : tmp) t.release();
+        // fold only a tensor with the mirror symmetry: every mirrored block within 1e-12 of its
+        // source, relative to the component's largest element
+        bool fold = green_fold_allowed();
+        if (fold) {
+            DevBuf<double> stat(12);
+            xtr::launch_green_mirror_check(full.p, g, stat.p, op->stream);
+            double h[12];
+            XTR_CHECK(hipMemcpyAsync(h, stat.p, sizeof(h), hipMemcpyDeviceToHost, op->stream));
+            XTR_CHECK(hipStreamSynchronize(op->stream));
+            for (int c = 0; c < 6; ++c) fold = fold && h[2 * c] <= 1e-12 * h[2 * c + 1];
+        } else {
+            XTR_CHECK(hipStreamSynchronize(op->stream));
+        }
+        store_green(op, full, fold);
     });
 }
 
@@ -774,11 +823,14 @@ int xtr_op_build_green(xtr_op *op)
         require(op->hank0.size() == 431 && op->hank1.size() == 431, "the 431-tap Hankel filter is required");
         require(op->number_of_hankels > 0, "NumberOfHankels must be positive");
         const xtr::Geom &g = op->g;
-        op->G64.alloc((size_t)(g.L() * g.gblk()));
+        op->green_ready = false;
+        op->G64.release();
         op->G32.release();
-        xtr::build_green_atoa(green_input(op), op->G64.p, op->tw64_x.p, op->tw64_y.p, op->stream);
+        DevBuf<double2> full((size_t)(g.L() * g.gblk()));
+        xtr::build_green_atoa(green_input(op), full.p, op->tw64_x.p, op->tw64_y.p, op->stream);
         XTR_CHECK(hipStreamSynchronize(op->stream));
-        finish_green(op);
+        // the assembled tensor has the mirror symmetry by construction (signed mirroring)
+        store_green(op, full, green_fold_allowed());
     });
 }
 
@@ -798,7 +850,7 @@ int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *
             tmp[c].alloc((size_t)(L * sizes[c]));
             dev[c] = tmp[c].p;
         }
-        DevBuf<double2> full;
+        DevBuf<double2> wide, full;
         const double2 *src = op->G64.p;
         if (op->precision == XTR_FP32) {
             // widen the complex64 copy for export
@@ -806,8 +858,13 @@ int xtr_op_get_green(xtr_op *op, xtr_complex *xx, xtr_complex *xy, xtr_complex *
             XTR_CHECK(hipMemcpy(h32.data(), op->G32.p, h32.size() * sizeof(float2), hipMemcpyDeviceToHost));
             std::vector<double2> h64(h32.size());
             for (size_t i = 0; i < h32.size(); ++i) h64[i] = xtr::to_d(h32[i]);
-            full.alloc(h64.size());
-            XTR_CHECK(hipMemcpy(full.p, h64.data(), h64.size() * sizeof(double2), hipMemcpyHostToDevice));
+            wide.alloc(h64.size());
+            XTR_CHECK(hipMemcpy(wide.p, h64.data(), h64.size() * sizeof(double2), hipMemcpyHostToDevice));
+            src = wide.p;
+        }
+        if (op->folded) {
+            full.alloc((size_t)(L * g.gblk()));
+            xtr::launch_green_unfold(src, full.p, g, op->stream);
             src = full.p;
         }
         xtr::launch_green_deinterleave(src, dev, g, op->stream);
@@ -951,13 +1008,23 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies)
     });
 }
 
+int xtr_op_green_layout(xtr_op *op, int *folded, int64_t *blocks)
+{
+    return guarded([&] {
+        require(op, "null argument");
+        require(op->green_ready, "the Green tensor is not set");
+        if (folded) *folded = op->folded ? 1 : 0;
+        if (blocks) *blocks = op->nblk();
+    });
+}
+
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes)
 {
     return guarded([&] {
         require(op, "null argument");
         if (n) *n = op->g.n();
         const int64_t per = op->precision == XTR_FP64 ? 16 : op->precision == XTR_FP32 ? 8 : 24;
-        if (green_bytes) *green_bytes = (int64_t)(op->g.L() * op->g.gblk() * per);
+        if (green_bytes) *green_bytes = (int64_t)(op->nblk() * op->g.gblk() * per);
         if (buffer_bytes) *buffer_bytes = (int64_t)(op->g.M() * op->elem());
     });
 }

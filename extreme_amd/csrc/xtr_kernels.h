@@ -19,7 +19,57 @@ struct Geom {
     int64_t M() const { return L() * C(); }                   // spectrum elements
     int64_t S() const { return (int64_t)nz * (nz + 1) / 2; }  // packed symmetric block
     int64_t gblk() const { return 4 * S() + 2LL * nz * nz; }  // Green elements per wavenumber
+    int64_t Lq() const { return (int64_t)(nx + 1) * (ny + 1); } // blocks of a folded Green tensor
 };
+
+// Folded Green tensor. The mirrored lateral tensor is even or odd in x and in y per component
+// (xx, yy, zz even/even; xy odd/odd; xz odd/even; yz even/odd), so its 2-D spectrum satisfies
+// G(2Nx-kx, ky) = D_x G(kx, ky) D_x and G(kx, 2Ny-ky) = D_y G(kx, ky) D_y with D_x = diag(-I, I, I),
+// D_y = diag(I, -I, I) on the (x, y, z) components. A folded tensor stores only kx <= Nx, ky <= Ny,
+// block b = qx (Ny+1) + qy; wavenumber l = kx 2Ny + ky uses block b with u_x, y_x negated when
+// kx > Nx (mx) and u_y, y_y negated when ky > Ny (my).
+struct FoldPos {
+    int64_t b;
+    int mx, my;
+};
+__host__ __device__ inline FoldPos fold_pos(int64_t l, int nx, int ny)
+{
+    const int kx = (int)(l / (2 * ny)), ky = (int)(l % (2 * ny));
+    FoldPos p;
+    p.mx = kx > nx;
+    p.my = ky > ny;
+    const int qx = p.mx ? 2 * nx - kx : kx, qy = p.my ? 2 * ny - ky : ky;
+    p.b = (int64_t)qx * (ny + 1) + qy;
+    return p;
+}
+// Wavenumbers that use folded block b (its images): 1, 2 or 4; image i has kx mirrored when
+// (i & 1) and ky mirrored when (i & 2) among the images that exist.
+struct FoldImages {
+    int n;
+    int64_t l[4];
+    int mx[4], my[4];
+};
+__host__ __device__ inline FoldImages fold_images(int64_t b, int nx, int ny)
+{
+    const int qx = (int)(b / (ny + 1)), qy = (int)(b % (ny + 1));
+    const int nmx = (qx > 0 && qx < nx) ? 2 : 1, nmy = (qy > 0 && qy < ny) ? 2 : 1;
+    FoldImages im;
+    im.n = 0;
+    for (int j = 0; j < nmy; ++j)
+        for (int i = 0; i < nmx; ++i) {
+            const int kx = i ? 2 * nx - qx : qx, ky = j ? 2 * ny - qy : qy;
+            im.l[im.n] = (int64_t)kx * 2 * ny + ky;
+            im.mx[im.n] = i;
+            im.my[im.n] = j;
+            ++im.n;
+        }
+    for (int k = im.n; k < 4; ++k) {
+        im.l[k] = im.l[0];
+        im.mx[k] = im.mx[0];
+        im.my[k] = im.my[0];
+    }
+    return im;
+}
 
 // Device position of the asymmetric-block element (rc, tr) (xz and yz, nz*nz each): rows are
 // XOR-swizzled, tr ^ (rc & mask), so that a block copied linearly into LDS can be read along rows
@@ -48,7 +98,8 @@ struct ApplyArgs {
     const double2 *R;         // [nx*ny*nz] FP64, used for right-hand sides whose v.R[i] is null
     const double *f;          // [nz] forward factors (real)
     const double *b;          // [nz] backward factors (real)
-    const void *G;            // [L][gblk] in apply precision
+    const void *G;            // [nblk][gblk] in apply precision
+    int folded;               // G holds the folded tensor (nblk = Lq) rather than all L wavenumbers
     const uint16_t *packed_rc;// [S][2] (rc, tr) of each packed element
     void *buf;                // nrhs * M spectrum in apply precision
     const void *tw_x;         // twiddles for P = 2 nx
@@ -72,6 +123,12 @@ bool fft_length_supported(int p);
 // device layout [L][xx S | xy S | yy S | zz S | xz nz^2 | yz nz^2].
 void launch_green_interleave(const double2 *comp[6], double2 *dev, const Geom &g, hipStream_t s);
 void launch_green_deinterleave(const double2 *dev, double2 *comp[6], const Geom &g, hipStream_t s);
+// full [L][gblk] -> folded [Lq][gblk] (blocks kx <= Nx, ky <= Ny), and back with the mirror signs
+void launch_green_fold(const double2 *full, double2 *folded, const Geom &g, hipStream_t s);
+void launch_green_unfold(const double2 *folded, double2 *full, const Geom &g, hipStream_t s);
+// stat[2c] = max |G_c(l) - s_c G_c(mirror source of l)|, stat[2c+1] = max |G_c| over a full
+// tensor, c in device block order (xx, xy, yy, zz, xz, yz); stat (12 doubles) cleared here
+void launch_green_mirror_check(const double2 *full, const Geom &g, double *stat, hipStream_t s);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
 
 // Observation level (obs_kernels.hip): jQ from chi (operator prepared as A: R and f of that type),

@@ -105,6 +105,7 @@ def lib():
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
         _lib = L
     return _lib
@@ -345,6 +346,13 @@ class ConvolutionOperator:
         n, gb, bb = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().xtr_op_sizes(self._h, C.byref(n), C.byref(gb), C.byref(bb)))
         return n.value, gb.value, bb.value
+
+    def green_layout(self):
+        """("folded", (Nx+1)(Ny+1)) when the device keeps only the blocks kx <= Nx, ky <= Ny (the
+        mirror images follow by sign), else ("full", 4 Nx Ny)."""
+        f, nb = C.c_int(), C.c_int64()
+        _check(lib().xtr_op_green_layout(self._h, C.byref(f), C.byref(nb)))
+        return ("folded" if f.value else "full"), nb.value
 
 
 class AnomalyCurrentFgmresSolver:

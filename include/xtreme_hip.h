@@ -228,6 +228,17 @@ int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *fi
 /* Sizes: n = 3 Nx Ny Nz, the Green tensor's device bytes, the spectrum buffer bytes. */
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes);
 
+/* Device storage of the AtoA Green tensor. No counterpart in the reference, which keeps all
+ * 4 Nx Ny wavenumber blocks (GreenTensor.cs:77-99). The mirrored lateral tensor is even or odd in
+ * x and y per component, so block (2Nx-kx, ky) equals block (kx, ky) with xy and xz negated and
+ * block (kx, 2Ny-ky) equals it with xy and yz negated. A folded tensor keeps only the
+ * (Nx+1)(Ny+1) blocks kx <= Nx, ky <= Ny, and the contraction applies the signs. xtr_op_build_green
+ * always folds; xtr_op_set_green folds when every mirrored block is within 1e-12 (relative to the
+ * component's largest element) of its source block, and otherwise keeps all blocks. The
+ * environment variable XTR_GREEN_LAYOUT=full disables folding. folded = 1 when folded;
+ * blocks = the number of blocks stored. */
+int xtr_op_green_layout(xtr_op *op, int *folded, int64_t *blocks);
+
 /* Replaces: ConvolutionOperator.Dispose / GreenTensor.Dispose (ConvolutionOperator.cs:338-344). */
 void xtr_op_destroy(xtr_op *op);
 

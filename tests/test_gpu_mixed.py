@@ -50,6 +50,9 @@ def test_mixed_operator_fp64_applies_unchanged(oracle, torch_cuda):
         ops[prec].prepare_operator("A")
     x = np.random.default_rng(4).standard_normal(m.n) + 0.25j
     assert np.array_equal(ops["fp64"].apply(x), ops["mixed"].apply(x))
+    # the tensor is stored folded; the export rebuilds the mirrored blocks from their sources, which
+    # the oracle's FFT (length 12 here) produces equal up to rounding
+    assert ops["mixed"].green_layout()[0] == "folded"
     back = ops["mixed"].green_tensor()
     for c in X.ConvolutionOperator.COMPONENTS:
-        assert np.array_equal(back[c], G[c])
+        assert np.abs(back[c] - G[c]).max() <= 1e-14 * np.abs(G[c]).max()
