@@ -14,6 +14,7 @@
 // the tile back. The zero padding of the forward passes and the cropping of the inverse
 // passes are applied at load/store time, so padded rows are never read or written twice.
 #include <algorithm>
+#include <utility>
 #include <cstdlib>
 #include <cstring>
 
@@ -911,11 +912,320 @@ static bool contract_stream(const ApplyArgs &a, hipStream_t s)
     return true;
 }
 
+// ---- folded contraction on MFMA, nz = 32 -------------------------------------------------------------
+// Folded block b serves up to 4 wavenumbers (its mirror images), so one LDS-resident block meets up to
+// 4 images x 2 right-hand sides = 8 complex columns U (96 x 8), and Y = G U is a small GEMM for the
+// 16x16x4 MFMA: B = [Re U | Im U] (96 x 16 real), A = Re G and Im G (96 x 96 each), D1 = Re G . B,
+// D2 = Im G . B, then Re Y = D1[:, :8] - D2[:, 8:] and Im Y = D1[:, 8:] + D2[:, :8] (a lane and its
+// xor-8 partner). Mirror signs: u_x (u_y) of an image with kx (ky) mirrored is negated on the way
+// into B, and y_x (y_y) on the way out. Persistent: one 256-thread workgroup per CU walks blocks; each
+// block and its spectrum columns are copied HBM -> LDS by global_load_lds_dwordx4 into a ring of NBUF
+// slots (FP64 2 x 77 KB, FP32 4 x 39 KB) exactly as in the streaming kernel. The six 16-row tiles
+// times (Re, Im) make 12 accumulators, three per wave; tiles 1 and 4 have their two parts on two
+// waves and meet through the block's slot once the MFMAs have read it. Two workgroups share a CU,
+// so one workgroup's copy runs under the other's MFMAs (FP64: one slot per workgroup; FP32: two,
+// the next block copied during the current one's MFMA steps).
+namespace fold32 {
+constexpr int NZ = 32, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
+constexpr int MASK = 15;                  // asym_swizzle_mask(32)
+constexpr int NCOL = 8;                   // complex columns per block (images x right-hand sides)
+constexpr int KSTEPS = C / 4;             // 24
+template <typename T>
+struct Cfg {
+    static constexpr int E = (int)sizeof(cplx<T>);
+    static constexpr int EPL = 16 / E;
+    static constexpr int EPC = 64 * EPL;
+    static constexpr int IMG = GB + NCOL * C;
+    static constexpr int NLOAD = (IMG + EPC - 1) / EPC;     // wave-loads per block
+    static constexpr int SLOT = NLOAD * EPC;
+    // two workgroups per CU: FP64 one slot each (one computes while the other copies), FP32 two
+    static constexpr int NBUF = E == 16 ? 1 : 2;
+    static constexpr int PW_HI = (NLOAD + 3) / 4;           // wave-loads of the waves with the most
+};
+static_assert(2 * 1 * Cfg<double>::SLOT * 16 <= 160 * 1024, "LDS (FP64, two workgroups per CU)");
+static_assert(2 * 2 * Cfg<float>::SLOT * 8 <= 160 * 1024, "LDS (FP32, two workgroups per CU)");
+static_assert(2 * 64 * 4 * 8 <= Cfg<double>::SLOT * 16 && 2 * 64 * 4 * 4 <= Cfg<float>::SLOT * 8, "exchange fits a slot");
+static_assert((Cfg<double>::SLOT * 16) % 32 == 0 && (Cfg<float>::SLOT * 8) % 32 == 0, "slots stay 32-byte aligned");
+static_assert(Cfg<double>::PW_HI < 64 && Cfg<float>::PW_HI < 64, "vmcnt range");
+static_assert(GB % 2 == 0 && C % 2 == 0, "two-element lanes stay inside one column");
+static_assert(2 * KSTEPS >= Cfg<double>::PW_HI && 2 * KSTEPS >= Cfg<float>::PW_HI, "copy spread over the steps");
+}  // namespace fold32
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<double> { typedef mfma_d4 type; };
+template <> struct MfmaAcc<float> { typedef mfma_f4 type; };
+__device__ __forceinline__ mfma_d4 mfma16(double a, double b, mfma_d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mfma_f4 mfma16(float a, float b, mfma_f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Row of D held in register r of lane l (16x16x4 results; column = l % 16): the FP64 instruction
+// spreads consecutive rows over the lane quarters, the FP32 one over the registers.
+template <typename T>
+__device__ __forceinline__ int mfma_row(int lane, int r)
+{
+    return sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+}
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+// Copy of the next block, issued one wave-load at a time between the MFMA steps of the current one
+// (all at once, a wave stalls on the full memory queues before it can start its MFMAs).
+template <typename T>
+struct Fold32Copy {
+    const cplx<T> *G;
+    const cplx<T> *buf;
+    int64_t M, b;
+    FoldImages im;
+    int nrhs;
+    unsigned dst;
+    bool active;
+};
+
+template <typename T>
+__device__ __forceinline__ void fold32_issue_one(const Fold32Copy<T> &cp, int t, int wave, int lane)
+{
+    using namespace fold32;
+    using K = Cfg<T>;
+    const int k = __builtin_amdgcn_readfirstlane(wave + 4 * t);   // uniform: the branches below are scalar
+    if (!cp.active || k >= K::NLOAD) return;
+    const int e = k * K::EPC + lane * K::EPL;
+    const cplx<T> *src;
+    if ((k + 1) * K::EPC <= GB || (k * K::EPC < GB && e < GB)) {
+        // Green data (the second test: the one FP32 wave-load that straddles GB, its lower lanes)
+        src = cp.G + cp.b * GB + e;
+    } else {
+        // spectrum column q at LDS position GB + q C, element kk at (kk + 2q) mod C (the rotation
+        // spreads the columns over the LDS banks for the B reads)
+        const int u = e - GB;
+        int q = u / C, j = u - q * C;
+        const int ncol = cp.im.n * cp.nrhs;
+        if (q >= ncol) q = j = 0;                            // unused column / padding: valid data
+        int kk = j - 2 * q;
+        if (kk < 0) kk += C;
+        const int img = q / cp.nrhs, h = q - img * cp.nrhs;
+        src = cp.buf + h * cp.M + sel4(cp.im.l, img) * C + kk;
+    }
+    const unsigned m0 = __builtin_amdgcn_readfirstlane(cp.dst + (unsigned)k * 1024u);
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+}
+
+template <typename T>
+__device__ __forceinline__ void fold32_issue(const Fold32Copy<T> &cp, int wave, int lane)
+{
+#pragma unroll
+    for (int t = 0; t < fold32::Cfg<T>::PW_HI; ++t) fold32_issue_one<T>(cp, t, wave, lane);
+}
+#pragma clang diagnostic pop
+
+// A element (row 16 TILE + i, column 4 STEP + kl) of the 96 x 96 block matrix
+// [xx xy xz; xy yy yz; -xz^T -yz^T zz].
+template <typename T, int TILE, int STEP>
+__device__ __forceinline__ cplx<T> fold32_a(const cplx<T> *__restrict__ blk, int i, int kl)
+{
+    using namespace fold32;
+    constexpr int o = TILE / 2, ci = STEP / 8;
+    const int r = 16 * (TILE % 2) + i, c = 4 * (STEP % 8) + kl;
+    if constexpr (o == ci || (o < 2 && ci < 2)) {
+        constexpr int base = o != ci ? S : o == 0 ? 0 : o == 1 ? 2 * S : 3 * S;   // xy | xx, yy, zz
+        const int p = c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2;
+        return blk[base + p];
+    } else if constexpr (ci == 2) {
+        constexpr int base = o == 0 ? 4 * S : 4 * S + A;                       // xz, yz (r, c)
+        return blk[base + r * NZ + (c ^ (r & MASK))];
+    } else {
+        constexpr int base = ci == 0 ? 4 * S : 4 * S + A;                      // -xz^T, -yz^T
+        const cplx<T> v = blk[base + c * NZ + (r ^ (c & MASK))];
+        return mk<T>(-v.x, -v.y);
+    }
+}
+
+// Part PART of A element (row 16 TILE + i, column 4 STEP + kl).
+template <typename T, int TILE, int STEP, int PART>
+__device__ __forceinline__ T fold32_a_part(const cplx<T> *__restrict__ blk, int i, int kl)
+{
+    const cplx<T> v = fold32_a<T, TILE, STEP>(blk, i, kl);
+    return PART ? v.y : v.x;
+}
+
+// One k-step: the A values of the next step are read before this step's MFMAs, so the LDS latency
+// is covered by three MFMAs; two wave-loads of the next block's copy go out over the first steps.
+template <typename T, int TF, int TH, int PH, int STEP>
+__device__ __forceinline__ void fold32_step(const cplx<T> *__restrict__ blk, const T (&bf)[fold32::KSTEPS], int i,
+                                            int kl, typename MfmaAcc<T>::type &f1, typename MfmaAcc<T>::type &f2,
+                                            typename MfmaAcc<T>::type &hh, cplx<T> &gf, T &gh,
+                                            const Fold32Copy<T> &cp, int wave, int lane)
+{
+    cplx<T> nf = gf;
+    T nh = gh;
+    if constexpr (STEP + 1 < fold32::KSTEPS) {
+        nf = fold32_a<T, TF, STEP + 1>(blk, i, kl);
+        nh = fold32_a_part<T, TH, STEP + 1, PH>(blk, i, kl);
+    }
+    if constexpr (2 * STEP < fold32::Cfg<T>::PW_HI) fold32_issue_one<T>(cp, 2 * STEP, wave, lane);
+    if constexpr (2 * STEP + 1 < fold32::Cfg<T>::PW_HI) fold32_issue_one<T>(cp, 2 * STEP + 1, wave, lane);
+    f1 = mfma16(gf.x, bf[STEP], f1);
+    f2 = mfma16(gf.y, bf[STEP], f2);
+    hh = mfma16(gh, bf[STEP], hh);
+    gf = nf;
+    gh = nh;
+}
+
+// Wave's three accumulators: tile TF (Re G and Im G parts) and part PH of tile TH.
+template <typename T, int TF, int TH, int PH, int... Ss>
+__device__ __forceinline__ void fold32_wave(std::integer_sequence<int, Ss...>, const cplx<T> *__restrict__ blk,
+                                            const T (&bf)[fold32::KSTEPS], int lane,
+                                            typename MfmaAcc<T>::type &f1, typename MfmaAcc<T>::type &f2,
+                                            typename MfmaAcc<T>::type &hh, const Fold32Copy<T> &cp, int wave)
+{
+    const int i = lane & 15, kl = lane >> 4;
+    cplx<T> gf = fold32_a<T, TF, 0>(blk, i, kl);
+    T gh = fold32_a_part<T, TH, 0, PH>(blk, i, kl);
+    (fold32_step<T, TF, TH, PH, Ss>(blk, bf, i, kl, f1, f2, hh, gf, gh, cp, wave, lane), ...);
+}
+
+// Y of tile TILE from D1 (Re G . B) and D2 (Im G . B), stored with the image signs.
+template <typename T, int TILE>
+__device__ __forceinline__ void fold32_store(typename MfmaAcc<T>::type d1, typename MfmaAcc<T>::type d2,
+                                             cplx<T> *buf, int64_t M, const FoldImages &im, int nrhs, int lane)
+{
+    using namespace fold32;
+    constexpr int o = TILE / 2;
+    const int n = lane & 15, q = n & 7, part = n >> 3;
+    const int img = (q / nrhs) & 3, h = q % nrhs;
+    const bool live = q < im.n * nrhs;
+    const bool flip = (o == 0 && sel4(im.mx, img)) || (o == 1 && sel4(im.my, img));
+    T *dst = reinterpret_cast<T *>(buf + h * M + sel4(im.l, img) * C + o * NZ) + part;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const T other = __shfl_xor(d2[r], 8, 64);
+        T v = part ? d1[r] + other : d1[r] - other;
+        if (flip) v = -v;
+        const int rr = 16 * (TILE % 2) + mfma_row<T>(lane, r);
+        if (live) dst[2 * rr] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
+                                                                 int64_t Lq, int64_t M, int nrhs, int nx, int ny)
+{
+    using namespace fold32;
+    using K = Cfg<T>;
+    using acc_t = typename MfmaAcc<T>::type;
+    __shared__ __attribute__((aligned(32))) cplx<T> ring[K::NBUF * K::SLOT];   // 32: exchange vectors
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
+    const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= Lq
+    const int64_t b0 = blockIdx.x;
+    const int64_t cnt = (Lq - 1 - b0) / g + 1;
+    Fold32Copy<T> cp{G, buf, M, 0, FoldImages{}, nrhs, 0u, false};
+    if (K::NBUF == 2) {                                          // block 0 now, block it + 1 during block it
+        cp.b = b0;
+        cp.im = fold_images(b0, nx, ny);
+        cp.dst = base;
+        cp.active = true;
+        fold32_issue<T>(cp, wave, lane);
+    }
+    for (int64_t it = 0; it < cnt; ++it) {
+        const int64_t b = b0 + it * g;
+        if (K::NBUF == 1) {                                      // the whole block now
+            cp.b = b;
+            cp.im = fold_images(b, nx, ny);
+            cp.dst = base;
+            cp.active = true;
+            fold32_issue<T>(cp, wave, lane);
+            cp.active = false;
+        } else {
+            cp.active = it + 1 < cnt;
+            if (cp.active) {
+                cp.b = b + g;
+                cp.im = fold_images(cp.b, nx, ny);
+                cp.dst = base + (unsigned)(((it + 1) % K::NBUF) * K::SLOT * K::E);
+            }
+        }
+        wait_vm<0>();                                            // block it (the only copy in flight)
+        XTR_BARRIER();                                           // block it in LDS for every wave
+        cplx<T> *blk = ring + (it % K::NBUF) * K::SLOT;
+        const FoldImages im = fold_images(b, nx, ny);
+        // B fragments: lane (kl = lane / 16, n = lane % 16) holds B[4s + kl][n] for each k-step s
+        T bf[KSTEPS];
+        {
+            const int n = lane & 15, q = n & 7, part = n >> 3, kl = lane >> 4;
+            const int img = (q / nrhs) & 3;
+            const T *ucol = reinterpret_cast<const T *>(blk + GB + q * C) + part;
+            const bool fx = sel4(im.mx, img), fy = sel4(im.my, img);
+#pragma unroll
+            for (int st = 0; st < KSTEPS; ++st) {
+                int j = 4 * st + kl + 2 * q;                     // rotated position of element 4 st + kl
+                if (j >= C) j -= C;
+                const T v = ucol[2 * j];
+                const bool flip = (st < 8 && fx) || (st >= 8 && st < 16 && fy);
+                bf[st] = flip ? -v : v;
+            }
+        }
+        acc_t f1 = {0, 0, 0, 0}, f2 = f1, hh = f1;
+        using Seq = std::make_integer_sequence<int, KSTEPS>;
+        switch (wave) {
+        case 0: fold32_wave<T, 0, 1, 0>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
+        case 1: fold32_wave<T, 2, 1, 1>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
+        case 2: fold32_wave<T, 3, 4, 0>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
+        default: fold32_wave<T, 5, 4, 1>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
+        }
+        // tiles 1 and 4: the Im G part (waves 1, 3) joins the Re G part (waves 0, 2), through the slot
+        acc_t (*exch)[64] = reinterpret_cast<acc_t (*)[64]>(blk);
+        XTR_BARRIER();                                           // every wave has read the slot
+        if (wave & 1) exch[wave >> 1][lane] = hh;
+        XTR_BARRIER();
+        acc_t h2 = f1;
+        if (!(wave & 1)) h2 = exch[wave >> 1][lane];
+        switch (wave) {
+        case 0:
+            fold32_store<T, 0>(f1, f2, buf, M, im, nrhs, lane);
+            fold32_store<T, 1>(hh, h2, buf, M, im, nrhs, lane);
+            break;
+        case 1: fold32_store<T, 2>(f1, f2, buf, M, im, nrhs, lane); break;
+        case 2:
+            fold32_store<T, 3>(f1, f2, buf, M, im, nrhs, lane);
+            fold32_store<T, 4>(hh, h2, buf, M, im, nrhs, lane);
+            break;
+        default: fold32_store<T, 5>(f1, f2, buf, M, im, nrhs, lane); break;
+        }
+        XTR_BARRIER();                                           // slot free (exchange read)
+    }
+    wait_vm<0>();                                                // no copy may land after exit
+}
+
+// Folded tensor, nz = 32: right-hand sides in launches of up to two.
+static bool contract_fold32(const ApplyArgs &a, hipStream_t s)
+{
+    if (!a.folded || a.g.nz != fold32::NZ || contract_force_generic()) return false;
+    const int64_t Lq = a.g.Lq();
+    const unsigned grid = (unsigned)std::min<int64_t>(Lq, 2 * (int64_t)device_cus());   // two per CU
+    for (int h0 = 0; h0 < a.nrhs; h0 += 2) {
+        const int nh = std::min(2, a.nrhs - h0);
+        if (a.precision == 0)
+            hipLaunchKernelGGL((contract_fold32_kernel<double>), dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
+                               (double2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
+        else
+            hipLaunchKernelGGL((contract_fold32_kernel<float>), dim3(grid), dim3(256), 0, s, (const float2 *)a.G,
+                               (float2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
+        XTR_LAUNCH_CHECK("contract_fold32_kernel");
+    }
+    return true;
+}
+
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
     if (!a.folded && contract_stream<T>(a, s)) return;
     if (!a.folded && contract_stream64(a, s)) return;
+    if (contract_fold32(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;
