@@ -34,13 +34,16 @@ METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s
 # HBM bytes per launch of the contraction kernel, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (one counter per run) and scaled by the same counters on a 1 GiB device copy
 # (FETCH x 2.0000, WRITE x 1.0000); summary in profiles/contract_stream_kernel_c2_hbm_traffic.md.
-# Keyed by (workload, nz, right-hand sides per launch); a workload without an entry reports null.
-MEASURED_TRAFFIC = {("C2", 32, 2): 4.7647e9,    # contract_stream_kernel<double, 2>
-                    ("C5", 64, 2): 7.2476e10}   # contract_stream64_kernel<double, 2>
+# Keyed by (workload, nz, right-hand sides per launch, Green layout); no entry reports null.
+MEASURED_TRAFFIC = {("C2", 32, 2, "full"): 4.7647e9,    # contract_stream_kernel<double, 2>
+                    ("C5", 64, 2, "full"): 7.2476e10}   # contract_stream64_kernel<double, 2>
 
 
-def contract_kernel_name(nz):
-    """The FP64 contraction kernel a two-right-hand-side apply launches at this nz."""
+def contract_kernel_name(nz, layout):
+    """The FP64 contraction kernel a two-right-hand-side apply launches at this nz and Green layout."""
+    if layout == "folded":
+        return {32: "contract_fold32_kernel<double>", 64: "contract_fold64_kernel<double>"}.get(
+            nz, "contract_kernel<double>")
     return {32: "contract_stream_kernel<double, 2>", 64: "contract_stream64_kernel<double, 2>"}.get(
         nz, "contract_kernel<double>")
 
@@ -71,7 +74,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(frequencies shard across GPUs with no exchange, see DESIGN.md)"}))
+                                   "(the frequencies of a sweep shard across GPUs with no exchange)"}))
         sys.exit(2)
 
     import torch
@@ -115,7 +118,9 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
 
-    # ---- per-stage device time and algorithmic bytes (SURVEY 8(d) formula) -----------------
+    # ---- per-stage device time and algorithmic bytes ------------------------------------------
+    # Green tensor read once per apply (the blocks the device stores: (Nx+1)(Ny+1) when folded, else
+    # 4 Nx Ny), spectrum columns read and written per right-hand side, x, y and R per apply.
     m = model
     nz, L = m.nz, 4 * m.nx * m.ny
     gblk = 4 * (nz * (nz + 1) // 2) + 2 * nz * nz
@@ -134,7 +139,8 @@ def main():
         p.op.set_profiling(False)
     for p, infos in results:
         rhs_applies += sum(i.matvecs for i in infos)
-    green_bytes = SG * L * gblk
+    layout, nblk = problems[0].op.green_layout()
+    green_bytes = SG * nblk * gblk
     contract_bytes = rounds * green_bytes + rhs_applies * 2 * M * SV
     apply_alg_bytes = rounds * green_bytes + rhs_applies * 2 * n * SV + rounds * N * SR
     apply_ms = sum(stage_ms.values())
@@ -195,10 +201,10 @@ def main():
         "true_backward_error": be,
         "converged": conv,
         "green_seconds_per_freq": float(np.mean(green_s)),
-        "roofline": {"kernel": contract_kernel_name(m.nz), "bound": "hbm", "achieved": achieved_gbs,
+        "roofline": {"kernel": contract_kernel_name(m.nz, layout), "bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
-                     "avg_launch_ms": contract_avg_ms,
+                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2, layout)),
+                     "avg_launch_ms": contract_avg_ms, "green_layout": layout, "green_blocks": nblk,
                      "algorithmic_bytes_per_launch": contract_bytes / max(rounds, 1)},
         "cpu_baseline": cpu,
         "host_boundary": pcie,
@@ -323,8 +329,10 @@ def large_config(model, args):
         t64 = time.perf_counter() - t0
         st, rounds = p.op.stage_times()
         p.op.set_profiling(False)
+        layout, nblk = p.op.green_layout()
         ref = p.chi.clone()
-        row = {"frequency_hz": f, "green_build_s": build, "fp64_s": t64,
+        row = {"frequency_hz": f, "green_build_s": build, "fp64_s": t64, "green_layout": layout, "green_blocks": nblk,
+               "contract_ms_total": st["contract"],
                "fp64_iterations": [i.number_of_iterations for i in infos],
                "fp64_matvecs": [i.matvecs for i in infos],
                "fp64_true_residual": [i.backward_error for i in infos],
@@ -349,8 +357,14 @@ def large_config(model, args):
     gblk = 4 * (nz * (nz + 1) // 2) + 2 * nz * nz
     M = L * 3 * nz
     fp64 = sum(r["fp64_s"] for r in rows) / len(rows)
-    contract_ms = sum(r["stage_ms_per_apply"]["contract"] for r in rows) / len(rows)
-    achieved = (16 * L * gblk + 2 * 2 * M * 16) / (contract_ms * 1e-3) / 1e9
+    # Green blocks once per apply, spectrum columns in and out per right-hand side, over all applies
+    total_rounds = sum(r["apply_rounds"] for r in rows)
+    contract_bytes = sum(r["apply_rounds"] * 16 * r["green_blocks"] * gblk + sum(r["fp64_matvecs"]) * 2 * M * 16
+                         for r in rows)
+    contract_total_ms = sum(r["contract_ms_total"] for r in rows)
+    contract_ms = contract_total_ms / max(total_rounds, 1)
+    achieved = contract_bytes / (contract_total_ms * 1e-3) / 1e9
+    layout = rows[0]["green_layout"]
     line = {
         "metric": METRIC, "value": fp64, "unit": "s/freq", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000 * fp64, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -358,10 +372,11 @@ def large_config(model, args):
         "config": {"workload": f"{m.name}: {m.nx}x{m.ny}x{m.nz} cells, 3-layer background + slab, 60 boxes, "
                                f"{args.steps} of {len(m.frequencies)} frequencies one at a time, both polarisations, "
                                "FGMRES(30)/GMRES(10), tol 1e-8", "nx": m.nx, "ny": m.ny, "nz": m.nz},
-        "roofline": {"kernel": contract_kernel_name(nz), "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": contract_kernel_name(nz, layout), "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2)),
-                     "avg_launch_ms": contract_ms},
+                     "traffic": MEASURED_TRAFFIC.get((m.name, nz, 2, layout)),
+                     "avg_launch_ms": contract_ms, "green_layout": layout,
+                     "algorithmic_bytes_per_launch": contract_bytes / max(total_rounds, 1)},
         "cpu_baseline": None,
         "mixed_s_per_freq": (sum(r["mixed_s"] for r in rows) / len(rows)) if not args.no_mixed else None,
         "frequencies": rows,

@@ -1009,7 +1009,7 @@ __device__ __forceinline__ void fold32_issue_one(const Fold32Copy<T> &cp, int t,
         int kk = j - 2 * q;
         if (kk < 0) kk += C;
         const int img = q / cp.nrhs, h = q - img * cp.nrhs;
-        src = cp.buf + h * cp.M + sel4(cp.im.l, img) * C + kk;
+        src = cp.buf + h * cp.M + cp.im.l(img) * C + kk;
     }
     const unsigned m0 = __builtin_amdgcn_readfirstlane(cp.dst + (unsigned)k * 1024u);
     asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
@@ -1099,8 +1099,8 @@ __device__ __forceinline__ void fold32_store(typename MfmaAcc<T>::type d1, typen
     const int n = lane & 15, q = n & 7, part = n >> 3;
     const int img = (q / nrhs) & 3, h = q % nrhs;
     const bool live = q < im.n * nrhs;
-    const bool flip = (o == 0 && sel4(im.mx, img)) || (o == 1 && sel4(im.my, img));
-    T *dst = reinterpret_cast<T *>(buf + h * M + sel4(im.l, img) * C + o * NZ) + part;
+    const bool flip = (o == 0 && im.mx(img)) || (o == 1 && im.my(img));
+    T *dst = reinterpret_cast<T *>(buf + h * M + im.l(img) * C + o * NZ) + part;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const T other = __shfl_xor(d2[r], 8, 64);
@@ -1159,7 +1159,7 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
             const int n = lane & 15, q = n & 7, part = n >> 3, kl = lane >> 4;
             const int img = (q / nrhs) & 3;
             const T *ucol = reinterpret_cast<const T *>(blk + GB + q * C) + part;
-            const bool fx = sel4(im.mx, img), fy = sel4(im.my, img);
+            const bool fx = im.mx(img), fy = im.my(img);
 #pragma unroll
             for (int st = 0; st < KSTEPS; ++st) {
                 int j = 4 * st + kl + 2 * q;                     // rotated position of element 4 st + kl
@@ -1201,6 +1201,286 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
     wait_vm<0>();                                                // no copy may land after exit
 }
 
+// ---- folded contraction on MFMA, nz = 64 -------------------------------------------------------------
+// A folded block (264 KB in FP64) is streamed through LDS in nine units: the spectrum columns of its
+// images, xx, xy, yy, zz, and the xz and yz blocks each as a diagonal and an off-diagonal pair of
+// 32 x 32 quadrants. Each wave owns one 16-row tile of each component (rows 16w..16w+15 of x, y and
+// z), and every unit then gives the four waves the same number of MFMAs. One accumulator per tile:
+// with B1 = [Re U | Im U] and B2 = [-Im U | Re U], Re G . B1 + Im G . B2 = [Re Y | Im Y]. The
+// spectrum unit is copied (mirror signs applied, columns rotated over the LDS banks) into an area that
+// lives for the whole block; the others are contracted from their ring slot while the next NSLOT-1
+// units are in flight (FP64: 4 slots of 33 wave-loads, FP32: 8 of 17).
+namespace fold64 {
+constexpr int NZ = 64, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
+constexpr int MASK = 15;                  // asym_swizzle_mask(64)
+constexpr int NCOL = 8, UCOLS = NCOL * C;
+constexpr int NUNIT = 9;                  // spectrum, xx, xy, yy, zz, xz diag, xz off, yz diag, yz off
+__host__ __device__ constexpr int uelems(int u) { return u == 0 ? UCOLS : u <= 4 ? S : A / 2; }
+template <typename T>
+struct Cfg {
+    static constexpr int E = (int)sizeof(cplx<T>);
+    static constexpr int EPL = 16 / E;
+    static constexpr int EPC = 64 * EPL;
+    static constexpr int NSLOT = E == 16 ? 4 : 8;
+    __host__ __device__ static constexpr int nw(int u) { return (uelems(u) + EPC - 1) / EPC; }
+    static constexpr int SLOTW = (S + EPC - 1) / EPC;       // the largest unit, in wave-loads
+    static constexpr int SLOT = SLOTW * EPC;
+    // wave-loads of unit u issued by wave w (0 or any other: only wave 0 can have one more)
+    __host__ __device__ static constexpr int cnt(int u, int w) { return (nw(u) - w + 3) / 4; }
+    // wave-loads a wave has issued after unit u's when step u waits
+    __host__ __device__ static constexpr int ahead(int u, bool has_next, int w)
+    {
+        int n = 0;
+        for (int k = 1; k < NSLOT; ++k)
+            if (u + k < NUNIT || has_next) n += cnt((u + k) % NUNIT, w);
+        return n;
+    }
+};
+static_assert(Cfg<double>::nw(0) <= Cfg<double>::SLOTW && Cfg<double>::nw(5) <= Cfg<double>::SLOTW, "slot (FP64)");
+static_assert(Cfg<float>::nw(0) <= Cfg<float>::SLOTW && Cfg<float>::nw(5) <= Cfg<float>::SLOTW, "slot (FP32)");
+static_assert(4 * Cfg<double>::SLOT * 16 + UCOLS * 16 <= 160 * 1024, "LDS (FP64)");
+static_assert(8 * Cfg<float>::SLOT * 8 + UCOLS * 8 <= 160 * 1024, "LDS (FP32)");
+static_assert(Cfg<double>::ahead(0, true, 0) < 64 && Cfg<float>::ahead(0, true, 0) < 64, "vmcnt range");
+static_assert(Cfg<double>::NSLOT - 1 < NUNIT && Cfg<float>::NSLOT - 1 < NUNIT, "look-ahead within one block");
+static_assert(S % 2 == 0 && C % 2 == 0, "two-element lanes stay inside one run");
+}  // namespace fold64
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+// Copy of unit UT of block b into the slot at LDS address dst.
+template <typename T, int UT>
+__device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, const cplx<T> *buf, int64_t M, int64_t b,
+                                             const FoldImages &im, int nrhs, unsigned dst, int wave, int lane)
+{
+    using namespace fold64;
+    using K = Cfg<T>;
+    constexpr int NWU = K::nw(UT);
+#pragma unroll
+    for (int t = 0; t < (NWU + 3) / 4; ++t) {
+        const int k = __builtin_amdgcn_readfirstlane(wave + 4 * t);
+        if (k >= NWU) break;
+        const int e = k * K::EPC + lane * K::EPL;
+        const cplx<T> *src;
+        if constexpr (UT == 0) {
+            int q = e / C, j = e - q * C;
+            if (q >= im.n * nrhs) q = j = 0;                 // unused column / padding: valid data
+            const int img = q / nrhs, h = q - img * nrhs;
+            src = buf + h * M + im.l(img) * C + j;
+        } else if constexpr (UT <= 4) {
+            src = G + b * GB + (UT - 1) * S + (e < S ? e : S - K::EPL);   // padding lanes re-read
+        } else {
+            // quadrant pair: rows 0-31 with column half H0, rows 32-63 with half 1 - H0 (diag: H0 = 0,
+            // rows 32-63 with half 1; off: the other halves); 32 elements per quadrant row
+            constexpr int base = UT <= 6 ? 4 * S : 4 * S + A;
+            constexpr bool diag = UT == 5 || UT == 7;
+            const int qd = e >> 10, w = e & 1023;
+            const int rc = qd * 32 + (w >> 5), half = diag ? qd : 1 - qd;
+            src = G + b * GB + base + rc * NZ + 32 * half + (w & 31);
+        }
+        const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+    }
+}
+#pragma clang diagnostic pop
+
+// Wave w's MFMAs on one unit for its tile of components O (rows 16w..) over k-steps s0 .. s0+NS-1 of
+// component CI. KIND 0: packed symmetric unit (offset 0); 1: quadrant pair, B(r, c); 2: quadrant
+// pair, -B(c, r).
+template <typename T, int O, int CI, int KIND, int NS>
+__device__ __forceinline__ void fold64_job(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua, int s0,
+                                           int r, int kl, int q, int part, typename MfmaAcc<T>::type &acc)
+{
+    using namespace fold64;
+    auto a_at = [&](int s) -> cplx<T> {
+        const int c = 4 * s + kl;
+        if constexpr (KIND == 0) {
+            return blk[c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2];
+        } else if constexpr (KIND == 1) {
+            return blk[(r >> 5) * 1024 + (r & 31) * 32 + ((c ^ (r & MASK)) & 31)];
+        } else {
+            const cplx<T> v = blk[(c >> 5) * 1024 + (c & 31) * 32 + ((r ^ (c & MASK)) & 31)];
+            return mk<T>(-v.x, -v.y);
+        }
+    };
+    auto b_at = [&](int s) -> cplx<T> {
+        int j = CI * NZ + 4 * s + kl + 2 * q;             // rotated position of k = CI NZ + 4 s + kl
+        if (j >= C) j -= C;
+        return ua[q * C + j];
+    };
+    cplx<T> ga = a_at(s0), gb = b_at(s0);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        cplx<T> na = ga, nb = gb;
+        if (j + 1 < NS) {
+            na = a_at(s0 + j + 1);
+            nb = b_at(s0 + j + 1);
+        }
+        const T b1 = part ? gb.y : gb.x, b2 = part ? gb.x : -gb.y;
+        acc = mfma16(ga.x, b1, acc);
+        acc = mfma16(ga.y, b2, acc);
+        ga = na;
+        gb = nb;
+    }
+}
+
+template <typename T, int UT>
+__device__ __forceinline__ void fold64_compute(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua,
+                                               int wave, int lane, typename MfmaAcc<T>::type (&acc)[3])
+{
+    const int r = 16 * wave + (lane & 15), kl = lane >> 4, q = lane & 7, part = (lane >> 3) & 1;
+    const int h = wave >> 1;                               // row half of the wave's tiles
+    if constexpr (UT == 1) {
+        fold64_job<T, 0, 0, 0, 16>(blk, ua, 0, r, kl, q, part, acc[0]);
+    } else if constexpr (UT == 2) {
+        fold64_job<T, 0, 1, 0, 16>(blk, ua, 0, r, kl, q, part, acc[0]);
+        fold64_job<T, 1, 0, 0, 16>(blk, ua, 0, r, kl, q, part, acc[1]);
+    } else if constexpr (UT == 3) {
+        fold64_job<T, 1, 1, 0, 16>(blk, ua, 0, r, kl, q, part, acc[1]);
+    } else if constexpr (UT == 4) {
+        fold64_job<T, 2, 2, 0, 16>(blk, ua, 0, r, kl, q, part, acc[2]);
+    } else {
+        constexpr int o = UT <= 6 ? 0 : 1;                   // x (xz) or y (yz) rows and k of the transpose
+        const int hs = (UT == 5 || UT == 7) ? h : 1 - h;      // column half: diag same as rows, off other
+        fold64_job<T, o, 2, 1, 8>(blk, ua, 8 * hs, r, kl, q, part, acc[o]);
+        fold64_job<T, 2, o, 2, 8>(blk, ua, 8 * hs, r, kl, q, part, acc[2]);
+    }
+}
+
+template <typename T, int U>
+__device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t M, int64_t b,
+                                            int64_t bn, bool has_next, const FoldImages &im, const FoldImages &imn,
+                                            int nrhs, int64_t s, cplx<T> *ring, cplx<T> *ua, unsigned base, int wave,
+                                            int lane, int t, typename MfmaAcc<T>::type (&acc)[3])
+{
+    using namespace fold64;
+    using K = Cfg<T>;
+    constexpr int V = U + K::NSLOT - 1;                    // unit copied now (into the slot freed last step)
+    const unsigned vdst = base + (unsigned)(((s + K::NSLOT - 1) % K::NSLOT) * K::SLOT * K::E);
+    if constexpr (V < NUNIT) fold64_issue<T, V>(G, buf, M, b, im, nrhs, vdst, wave, lane);
+    else if (has_next) fold64_issue<T, V - NUNIT>(G, buf, M, bn, imn, nrhs, vdst, wave, lane);
+    if (has_next) {
+        if (wave == 0) wait_vm<K::ahead(U, true, 0)>();
+        else wait_vm<K::ahead(U, true, 1)>();
+    } else {
+        if (wave == 0) wait_vm<K::ahead(U, false, 0)>();
+        else wait_vm<K::ahead(U, false, 1)>();
+    }
+    XTR_BARRIER();                                         // unit U in LDS for every wave
+    const cplx<T> *blk = ring + (s % K::NSLOT) * K::SLOT;
+    if constexpr (U == 0) {
+        // spectrum columns -> block-long area, mirror signs applied, column q rotated by 2q: thread t
+        // copies elements k = t % 32 + 32 m (m < 6) of column q = t / 32, so each element's component
+        // (m / 2) is known at compile time
+        static_assert(UCOLS == 256 * 6 && C == 6 * 32, "copy mapping");
+        const int q = t >> 5, k0 = t & 31;
+        const int img = (q / nrhs) & 3;
+        const bool fx = im.mx(img) != 0, fy = im.my(img) != 0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int k = k0 + 32 * m;
+            cplx<T> v = blk[q * C + k];
+            const bool flip = m < 2 ? fx : m < 4 ? fy : false;
+            if (flip) v = mk<T>(-v.x, -v.y);
+            int j = k + 2 * q;
+            if (j >= C) j -= C;
+            ua[q * C + j] = v;
+        }
+    } else {
+        fold64_compute<T, U>(blk, ua, wave, lane, acc);
+    }
+    XTR_BARRIER();                                         // slot (and at U = 0 the area) ready
+}
+
+template <typename T, int O>
+__device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type d, cplx<T> *buf, int64_t M,
+                                             const FoldImages &im, int nrhs, int wave, int lane)
+{
+    using namespace fold64;
+    const int n = lane & 15, q = n & 7, part = n >> 3;
+    const int img = (q / nrhs) & 3, h = q % nrhs;
+    const bool live = q < im.n * nrhs;
+    const bool flip = (O == 0 && im.mx(img)) || (O == 1 && im.my(img));
+    T *dst = reinterpret_cast<T *>(buf + h * M + im.l(img) * C + O * NZ) + part;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int rr = 16 * wave + mfma_row<T>(lane, r);
+        const T v = flip ? -d[r] : d[r];
+        if (live) dst[2 * rr] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void contract_fold64_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t Lq,
+                                                              int64_t M, int nrhs, int nx, int ny)
+{
+    using namespace fold64;
+    using K = Cfg<T>;
+    using acc_t = typename MfmaAcc<T>::type;
+    __shared__ cplx<T> ring[K::NSLOT * K::SLOT];
+    __shared__ cplx<T> ua[UCOLS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
+    const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= Lq
+    const int64_t b0 = blockIdx.x;
+    const int64_t cnt = (Lq - 1 - b0) / g + 1;
+    {
+        // units 0 .. NSLOT-2 of the first block into slots 0 .. NSLOT-2
+        const FoldImages im0 = fold_images(b0, nx, ny);
+        fold64_issue<T, 0>(G, buf, M, b0, im0, nrhs, base, wave, lane);
+        fold64_issue<T, 1>(G, buf, M, b0, im0, nrhs, base + 1u * K::SLOT * K::E, wave, lane);
+        fold64_issue<T, 2>(G, buf, M, b0, im0, nrhs, base + 2u * K::SLOT * K::E, wave, lane);
+        if constexpr (K::NSLOT > 4) {
+            fold64_issue<T, 3>(G, buf, M, b0, im0, nrhs, base + 3u * K::SLOT * K::E, wave, lane);
+            fold64_issue<T, 4>(G, buf, M, b0, im0, nrhs, base + 4u * K::SLOT * K::E, wave, lane);
+            fold64_issue<T, 5>(G, buf, M, b0, im0, nrhs, base + 5u * K::SLOT * K::E, wave, lane);
+            fold64_issue<T, 6>(G, buf, M, b0, im0, nrhs, base + 6u * K::SLOT * K::E, wave, lane);
+        }
+    }
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int64_t b = b0 + i * g, bn = b + g;
+        const bool hn = i + 1 < cnt;
+        const FoldImages im = fold_images(b, nx, ny);
+        const FoldImages imn = fold_images(hn ? bn : b, nx, ny);
+        const int64_t s = i * NUNIT;
+        acc_t acc[3];
+        acc[0] = acc_t{0, 0, 0, 0};
+        acc[1] = acc[0];
+        acc[2] = acc[0];
+        fold64_step<T, 0>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 0, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 1>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 1, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 2>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 2, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 3>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 3, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 4>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 4, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 5>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 5, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 6>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 6, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 7>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 7, ring, ua, base, wave, lane, t, acc);
+        fold64_step<T, 8>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 8, ring, ua, base, wave, lane, t, acc);
+        fold64_store<T, 0>(acc[0], buf, M, im, nrhs, wave, lane);
+        fold64_store<T, 1>(acc[1], buf, M, im, nrhs, wave, lane);
+        fold64_store<T, 2>(acc[2], buf, M, im, nrhs, wave, lane);
+    }
+    wait_vm<0>();                                                // no copy may land after exit
+}
+
+// Folded tensor, nz = 64: right-hand sides in launches of up to two.
+static bool contract_fold64(const ApplyArgs &a, hipStream_t s)
+{
+    if (!a.folded || a.g.nz != fold64::NZ || contract_force_generic()) return false;
+    const int64_t Lq = a.g.Lq();
+    const unsigned grid = (unsigned)std::min<int64_t>(Lq, device_cus());
+    for (int h0 = 0; h0 < a.nrhs; h0 += 2) {
+        const int nh = std::min(2, a.nrhs - h0);
+        if (a.precision == 0)
+            hipLaunchKernelGGL((contract_fold64_kernel<double>), dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
+                               (double2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
+        else
+            hipLaunchKernelGGL((contract_fold64_kernel<float>), dim3(grid), dim3(256), 0, s, (const float2 *)a.G,
+                               (float2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
+        XTR_LAUNCH_CHECK("contract_fold64_kernel");
+    }
+    return true;
+}
+
 // Folded tensor, nz = 32: right-hand sides in launches of up to two.
 static bool contract_fold32(const ApplyArgs &a, hipStream_t s)
 {
@@ -1226,6 +1506,7 @@ static void contract(const ApplyArgs &a, hipStream_t s)
     if (!a.folded && contract_stream<T>(a, s)) return;
     if (!a.folded && contract_stream64(a, s)) return;
     if (contract_fold32(a, s)) return;
+    if (contract_fold64(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;

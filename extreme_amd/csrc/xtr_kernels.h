@@ -42,40 +42,32 @@ __host__ __device__ inline FoldPos fold_pos(int64_t l, int nx, int ny)
     p.b = (int64_t)qx * (ny + 1) + qy;
     return p;
 }
-// Wavenumbers that use folded block b (its images): n = 1, 2 or 4; l[i] with mirror flags mx[i],
-// my[i] for i < n (the unmirrored wavenumber first).
+// Wavenumbers that use folded block b (its images): n = 1, 2 or 4. Image i < n has kx mirrored
+// (mx) and/or ky mirrored (my); image 0 is the unmirrored wavenumber. Computed from i rather than
+// stored in arrays: a lane-dependent index into a private array would put it in scratch.
 struct FoldImages {
     int n;
-    int64_t l[4];
-    int mx[4], my[4];
+    int qx, qy, nx, ny;
+    bool two_x, two_y;
+    __host__ __device__ int mx(int i) const { return two_x ? (i & 1) : 0; }
+    __host__ __device__ int my(int i) const { return two_y ? (two_x ? (i >> 1) & 1 : i & 1) : 0; }
+    __host__ __device__ int64_t l(int i) const
+    {
+        const int kx = mx(i) ? 2 * nx - qx : qx, ky = my(i) ? 2 * ny - qy : qy;
+        return (int64_t)kx * 2 * ny + ky;
+    }
 };
 __host__ __device__ inline FoldImages fold_images(int64_t b, int nx, int ny)
 {
-    const int qx = (int)(b / (ny + 1)), qy = (int)(b % (ny + 1));
-    const bool two_x = qx > 0 && qx < nx, two_y = qy > 0 && qy < ny;
-    const int64_t kx0 = qx, kx1 = 2 * nx - qx, ky0 = qy, ky1 = 2 * ny - qy, w = 2 * ny;
     FoldImages im;
-    im.n = (two_x ? 2 : 1) * (two_y ? 2 : 1);
-    // fixed slots (no runtime index, which would put the struct in scratch); missing images repeat 0
-    im.l[0] = kx0 * w + ky0;
-    im.mx[0] = 0;
-    im.my[0] = 0;
-    im.l[1] = two_x ? kx1 * w + ky0 : two_y ? kx0 * w + ky1 : im.l[0];
-    im.mx[1] = two_x ? 1 : 0;
-    im.my[1] = (!two_x && two_y) ? 1 : 0;
-    im.l[2] = (two_x && two_y) ? kx0 * w + ky1 : im.l[0];
-    im.mx[2] = 0;
-    im.my[2] = (two_x && two_y) ? 1 : 0;
-    im.l[3] = (two_x && two_y) ? kx1 * w + ky1 : im.l[0];
-    im.mx[3] = (two_x && two_y) ? 1 : 0;
-    im.my[3] = (two_x && two_y) ? 1 : 0;
+    im.qx = (int)(b / (ny + 1));
+    im.qy = (int)(b % (ny + 1));
+    im.nx = nx;
+    im.ny = ny;
+    im.two_x = im.qx > 0 && im.qx < nx;
+    im.two_y = im.qy > 0 && im.qy < ny;
+    im.n = (im.two_x ? 2 : 1) * (im.two_y ? 2 : 1);
     return im;
-}
-// Element i < 4 by selects (a runtime index into a private array would go through scratch).
-template <typename V>
-__host__ __device__ inline V sel4(const V (&a)[4], int i)
-{
-    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
 }
 
 // Device position of the asymmetric-block element (rc, tr) (xz and yz, nz*nz each): rows are

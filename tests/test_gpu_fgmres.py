@@ -8,7 +8,7 @@ from tests.conftest import rel
 
 pytestmark = pytest.mark.gpu
 
-TOL_SOLUTION = 1e-9   # north_star: FP64 GPU path within 1e-9 relative of the FP64 oracle
+TOL_SOLUTION = 1e-9   # FP64 GPU path within 1e-9 relative of the FP64 oracle
 
 
 def _rhs(oracle, m, w, G, pol):
