@@ -115,3 +115,46 @@ def test_folded_fp32(oracle, torch_cuda):
     e = rel(op.apply(x), ref)
     print(f"folded fp32: {e:.2e}", flush=True)
     assert e < 2e-6
+
+
+def _mirror_symmetric_green(oracle, model, seed):
+    """A random Green tensor with the mirror symmetry of the lateral tensor: blocks kx <= Nx,
+    ky <= Ny drawn at random, the others their mirror images (xy negated under either mirror alone,
+    xz under kx, yz under ky), in the reference layout."""
+    nx, ny, nz = model.nx, model.ny, model.nz
+    rng = np.random.default_rng(seed)
+    S, A = nz * (nz + 1) // 2, nz * nz
+    sizes = {"xx": S, "xy": S, "yy": S, "zz": S, "xz": A, "yz": A}
+    src = {c: rng.standard_normal(((nx + 1) * (ny + 1), p)) + 1j * rng.standard_normal(((nx + 1) * (ny + 1), p))
+           for c, p in sizes.items()}
+    kx, ky = np.divmod(np.arange(4 * nx * ny), 2 * ny)
+    mx, my = kx > nx, ky > ny
+    b = np.where(mx, 2 * nx - kx, kx) * (ny + 1) + np.where(my, 2 * ny - ky, ky)
+    sign = {"xx": 1, "yy": 1, "zz": 1, "xy": np.where(mx ^ my, -1, 1), "xz": np.where(mx, -1, 1),
+            "yz": np.where(my, -1, 1)}
+    return {c: src[c][b] * np.reshape(sign[c], (-1, 1)) for c in oracle.COMPONENTS}
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 32), (16, 16, 64)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_folded_several_blocks_per_workgroup(oracle, torch_cuda, shape, precision):
+    """More folded blocks than workgroups (the copy of a workgroup's next block overlaps the current
+    one), one, two and three right-hand sides (three: a second launch with one)."""
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nz + 1)
+    w = cm.omega_of(m.frequencies[0])
+    G = _mirror_symmetric_green(oracle, m, seed=nx + nz)
+    op = X.ConvolutionOperator(m, w, precision=precision, max_rhs=3)
+    op.set_green_tensor(G)
+    assert op.green_layout() == ("folded", (nx + 1) * (ny + 1))
+    oop = oracle.Operator(m, G)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    op.prepare_operator("A", z)
+    oop.prepare("A", z, z0)
+    rng = np.random.default_rng(29)
+    xs = rng.standard_normal((3, m.n)) + 1j * rng.standard_normal((3, m.n))
+    ref = np.stack([oop.apply(x) for x in xs])
+    tol = TOL_FP64 if precision == "fp64" else 2e-6
+    errs = [rel(op.apply(xs[:k]), ref[:k]) for k in (1, 2, 3)]
+    print(f"folded {shape} {precision}: nrhs 1/2/3 {errs[0]:.2e} {errs[1]:.2e} {errs[2]:.2e}", flush=True)
+    assert max(errs) < tol
