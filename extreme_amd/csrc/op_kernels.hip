@@ -1205,7 +1205,10 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
 // A folded block (264 KB in FP64) is streamed through LDS in nine units: the spectrum columns of its
 // images, xx, xy, yy, zz, and the xz and yz blocks each as a diagonal and an off-diagonal pair of
 // 32 x 32 quadrants. Each wave owns one 16-row tile of each component (rows 16w..16w+15 of x, y and
-// z), and every unit then gives the four waves the same number of MFMAs. One accumulator per tile:
+// z), and every unit then gives the four tile owners the same number of MFMAs; eight waves (two per
+// SIMD, so one computes while the other waits) pair up: waves w and w+4 own the same tiles and take
+// half of each unit's k-steps each, their partial sums meet through LDS at the end of the block. One
+// accumulator per tile:
 // with B1 = [Re U | Im U] and B2 = [-Im U | Re U], Re G . B1 + Im G . B2 = [Re Y | Im Y]. The
 // spectrum unit is copied (mirror signs applied, columns rotated over the LDS banks) into an area that
 // lives for the whole block; the others are contracted from their ring slot while the next NSLOT-1
@@ -1215,6 +1218,7 @@ constexpr int NZ = 64, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * 
 constexpr int MASK = 15;                  // asym_swizzle_mask(64)
 constexpr int NCOL = 8, UCOLS = NCOL * C;
 constexpr int NUNIT = 9;                  // spectrum, xx, xy, yy, zz, xz diag, xz off, yz diag, yz off
+constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
 __host__ __device__ constexpr int uelems(int u) { return u == 0 ? UCOLS : u <= 4 ? S : A / 2; }
 template <typename T>
 struct Cfg {
@@ -1225,13 +1229,14 @@ struct Cfg {
     __host__ __device__ static constexpr int nw(int u) { return (uelems(u) + EPC - 1) / EPC; }
     static constexpr int SLOTW = (S + EPC - 1) / EPC;       // the largest unit, in wave-loads
     static constexpr int SLOT = SLOTW * EPC;
-    // wave-loads of unit u issued by wave w (0 or any other: only wave 0 can have one more)
-    __host__ __device__ static constexpr int cnt(int u, int w) { return (nw(u) - w + 3) / 4; }
-    // wave-loads a wave has issued after unit u's when step u waits
+    // wave-loads of unit u issued by wave w
+    __host__ __device__ static constexpr int cnt(int u, int w) { return (nw(u) - w + NWAVE - 1) / NWAVE; }
+    // wave-loads a wave has issued after unit u's when step u waits (units u+1 .. u+NSLOT-2: unit
+    // u+NSLOT-1 goes out after that step's barrier)
     __host__ __device__ static constexpr int ahead(int u, bool has_next, int w)
     {
         int n = 0;
-        for (int k = 1; k < NSLOT; ++k)
+        for (int k = 1; k < NSLOT - 1; ++k)
             if (u + k < NUNIT || has_next) n += cnt((u + k) % NUNIT, w);
         return n;
     }
@@ -1245,6 +1250,16 @@ static_assert(Cfg<double>::NSLOT - 1 < NUNIT && Cfg<float>::NSLOT - 1 < NUNIT, "
 static_assert(S % 2 == 0 && C % 2 == 0, "two-element lanes stay inside one run");
 }  // namespace fold64
 
+// Step U's wait: wave w lets through the wave-loads it issued after unit U's (counts differ by wave).
+template <typename T, int U, bool HN, int W = 0>
+__device__ __forceinline__ void fold64_wait_wave(int wave)
+{
+    if constexpr (W < fold64::NWAVE) {
+        if (wave == W) wait_vm<fold64::Cfg<T>::ahead(U, HN, W)>();
+        else fold64_wait_wave<T, U, HN, W + 1>(wave);
+    }
+}
+
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 // Copy of unit UT of block b into the slot at LDS address dst.
@@ -1256,8 +1271,8 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
     using K = Cfg<T>;
     constexpr int NWU = K::nw(UT);
 #pragma unroll
-    for (int t = 0; t < (NWU + 3) / 4; ++t) {
-        const int k = __builtin_amdgcn_readfirstlane(wave + 4 * t);
+    for (int t = 0; t < (NWU + NWAVE - 1) / NWAVE; ++t) {
+        const int k = __builtin_amdgcn_readfirstlane(wave + NWAVE * t);
         if (k >= NWU) break;
         const int e = k * K::EPC + lane * K::EPL;
         const cplx<T> *src;
@@ -1323,29 +1338,34 @@ __device__ __forceinline__ void fold64_job(const cplx<T> *__restrict__ blk, cons
     }
 }
 
+// Wave w: tiles of rows 16 (w % 4) .. of each component; waves w and w + 4 split each job's k-steps
+// (first and second half).
 template <typename T, int UT>
 __device__ __forceinline__ void fold64_compute(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua,
                                                int wave, int lane, typename MfmaAcc<T>::type (&acc)[3])
 {
-    const int r = 16 * wave + (lane & 15), kl = lane >> 4, q = lane & 7, part = (lane >> 3) & 1;
-    const int h = wave >> 1;                               // row half of the wave's tiles
+    const int tw = wave & 3, g = wave >> 2;
+    const int r = 16 * tw + (lane & 15), kl = lane >> 4, q = lane & 7, part = (lane >> 3) & 1;
+    const int h = tw >> 1;                                 // row half of the wave's tiles
     if constexpr (UT == 1) {
-        fold64_job<T, 0, 0, 0, 16>(blk, ua, 0, r, kl, q, part, acc[0]);
+        fold64_job<T, 0, 0, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[0]);
     } else if constexpr (UT == 2) {
-        fold64_job<T, 0, 1, 0, 16>(blk, ua, 0, r, kl, q, part, acc[0]);
-        fold64_job<T, 1, 0, 0, 16>(blk, ua, 0, r, kl, q, part, acc[1]);
+        fold64_job<T, 0, 1, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[0]);
+        fold64_job<T, 1, 0, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[1]);
     } else if constexpr (UT == 3) {
-        fold64_job<T, 1, 1, 0, 16>(blk, ua, 0, r, kl, q, part, acc[1]);
+        fold64_job<T, 1, 1, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[1]);
     } else if constexpr (UT == 4) {
-        fold64_job<T, 2, 2, 0, 16>(blk, ua, 0, r, kl, q, part, acc[2]);
+        fold64_job<T, 2, 2, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[2]);
     } else {
         constexpr int o = UT <= 6 ? 0 : 1;                   // x (xz) or y (yz) rows and k of the transpose
         const int hs = (UT == 5 || UT == 7) ? h : 1 - h;      // column half: diag same as rows, off other
-        fold64_job<T, o, 2, 1, 8>(blk, ua, 8 * hs, r, kl, q, part, acc[o]);
-        fold64_job<T, 2, o, 2, 8>(blk, ua, 8 * hs, r, kl, q, part, acc[2]);
+        fold64_job<T, o, 2, 1, 4>(blk, ua, 8 * hs + 4 * g, r, kl, q, part, acc[o]);
+        fold64_job<T, 2, o, 2, 4>(blk, ua, 8 * hs + 4 * g, r, kl, q, part, acc[2]);
     }
 }
 
+// One step of the unit stream: wait for unit U of block b (slot s % NSLOT); after the barrier every
+// wave is done with the previous step's slot, so unit U + NSLOT - 1 is copied into it; then contract.
 template <typename T, int U>
 __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t M, int64_t b,
                                             int64_t bn, bool has_next, const FoldImages &im, const FoldImages &imn,
@@ -1354,32 +1374,26 @@ __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<
 {
     using namespace fold64;
     using K = Cfg<T>;
-    constexpr int V = U + K::NSLOT - 1;                    // unit copied now (into the slot freed last step)
+    if (has_next) fold64_wait_wave<T, U, true>(wave);
+    else fold64_wait_wave<T, U, false>(wave);
+    XTR_BARRIER();                                         // unit U in LDS; previous slot and ua reads done
+    constexpr int V = U + K::NSLOT - 1;                    // unit copied now, into the previous step's slot
     const unsigned vdst = base + (unsigned)(((s + K::NSLOT - 1) % K::NSLOT) * K::SLOT * K::E);
     if constexpr (V < NUNIT) fold64_issue<T, V>(G, buf, M, b, im, nrhs, vdst, wave, lane);
     else if (has_next) fold64_issue<T, V - NUNIT>(G, buf, M, bn, imn, nrhs, vdst, wave, lane);
-    if (has_next) {
-        if (wave == 0) wait_vm<K::ahead(U, true, 0)>();
-        else wait_vm<K::ahead(U, true, 1)>();
-    } else {
-        if (wave == 0) wait_vm<K::ahead(U, false, 0)>();
-        else wait_vm<K::ahead(U, false, 1)>();
-    }
-    XTR_BARRIER();                                         // unit U in LDS for every wave
     const cplx<T> *blk = ring + (s % K::NSLOT) * K::SLOT;
     if constexpr (U == 0) {
         // spectrum columns -> block-long area, mirror signs applied, column q rotated by 2q: thread t
-        // copies elements k = t % 32 + 32 m (m < 6) of column q = t / 32, so each element's component
-        // (m / 2) is known at compile time
-        static_assert(UCOLS == 256 * 6 && C == 6 * 32, "copy mapping");
-        const int q = t >> 5, k0 = t & 31;
+        // copies elements k = t % 64 + 64 m (m < 3, the component) of column q = t / 64
+        static_assert(UCOLS == NTHR * 3 && C == 3 * 64, "copy mapping");
+        const int q = t >> 6, k0 = t & 63;
         const int img = (q / nrhs) & 3;
         const bool fx = im.mx(img) != 0, fy = im.my(img) != 0;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            const int k = k0 + 32 * m;
+        for (int m = 0; m < 3; ++m) {
+            const int k = k0 + 64 * m;
             cplx<T> v = blk[q * C + k];
-            const bool flip = m < 2 ? fx : m < 4 ? fy : false;
+            const bool flip = m == 0 ? fx : m == 1 ? fy : false;
             if (flip) v = mk<T>(-v.x, -v.y);
             int j = k + 2 * q;
             if (j >= C) j -= C;
@@ -1388,7 +1402,6 @@ __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<
     } else {
         fold64_compute<T, U>(blk, ua, wave, lane, acc);
     }
-    XTR_BARRIER();                                         // slot (and at U = 0 the area) ready
 }
 
 template <typename T, int O>
@@ -1410,14 +1423,14 @@ __device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type d, cplx<T
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void contract_fold64_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t Lq,
+__global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t Lq,
                                                               int64_t M, int nrhs, int nx, int ny)
 {
     using namespace fold64;
     using K = Cfg<T>;
     using acc_t = typename MfmaAcc<T>::type;
     __shared__ cplx<T> ring[K::NSLOT * K::SLOT];
-    __shared__ cplx<T> ua[UCOLS];
+    __shared__ __attribute__((aligned(32))) cplx<T> ua[UCOLS];   // 32: the end-of-block exchange
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
     const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
     const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= Lq
@@ -1455,9 +1468,21 @@ __global__ __launch_bounds__(256) void contract_fold64_kernel(const cplx<T> *__r
         fold64_step<T, 6>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 6, ring, ua, base, wave, lane, t, acc);
         fold64_step<T, 7>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 7, ring, ua, base, wave, lane, t, acc);
         fold64_step<T, 8>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 8, ring, ua, base, wave, lane, t, acc);
-        fold64_store<T, 0>(acc[0], buf, M, im, nrhs, wave, lane);
-        fold64_store<T, 1>(acc[1], buf, M, im, nrhs, wave, lane);
-        fold64_store<T, 2>(acc[2], buf, M, im, nrhs, wave, lane);
+        // waves 4..7 hand their partial sums to waves 0..3 through the spectrum area, once every wave
+        // is done reading it; the next block's spectrum copy comes after its first step's barrier
+        acc_t *ex = reinterpret_cast<acc_t *>(ua);                 // [4][3][64]
+        XTR_BARRIER();
+        if (wave >= 4)
+#pragma unroll
+            for (int o = 0; o < 3; ++o) ex[((wave - 4) * 3 + o) * 64 + lane] = acc[o];
+        XTR_BARRIER();
+        if (wave < 4) {
+#pragma unroll
+            for (int o = 0; o < 3; ++o) acc[o] += ex[(wave * 3 + o) * 64 + lane];
+            fold64_store<T, 0>(acc[0], buf, M, im, nrhs, wave, lane);
+            fold64_store<T, 1>(acc[1], buf, M, im, nrhs, wave, lane);
+            fold64_store<T, 2>(acc[2], buf, M, im, nrhs, wave, lane);
+        }
     }
     wait_vm<0>();                                                // no copy may land after exit
 }
@@ -1471,10 +1496,10 @@ static bool contract_fold64(const ApplyArgs &a, hipStream_t s)
     for (int h0 = 0; h0 < a.nrhs; h0 += 2) {
         const int nh = std::min(2, a.nrhs - h0);
         if (a.precision == 0)
-            hipLaunchKernelGGL((contract_fold64_kernel<double>), dim3(grid), dim3(256), 0, s, (const double2 *)a.G,
-                               (double2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
+            hipLaunchKernelGGL((contract_fold64_kernel<double>), dim3(grid), dim3(fold64::NTHR), 0, s,
+                               (const double2 *)a.G, (double2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
         else
-            hipLaunchKernelGGL((contract_fold64_kernel<float>), dim3(grid), dim3(256), 0, s, (const float2 *)a.G,
+            hipLaunchKernelGGL((contract_fold64_kernel<float>), dim3(grid), dim3(fold64::NTHR), 0, s, (const float2 *)a.G,
                                (float2 *)a.buf + h0 * a.g.M(), Lq, a.g.M(), nh, a.g.nx, a.g.ny);
         XTR_LAUNCH_CHECK("contract_fold64_kernel");
     }
