@@ -9,7 +9,7 @@
 //
 // A pencil kernel owns TC adjacent columns and all P points of each: it stages the
 // TC x P tile in LDS (column index fastest, so global loads of a row of TC columns are
-// contiguous), runs a Stockham radix-4/2/3 transform with every stage reading its
+// contiguous), runs a Stockham radix-4/2/3/5 transform with every stage reading its
 // butterfly inputs into registers before a barrier and writing after it, and writes
 // the tile back. The zero padding of the forward passes and the cropping of the inverse
 // passes are applied at load/store time, so padded rows are never read or written twice.
@@ -25,10 +25,10 @@ namespace xtr {
 
 constexpr int kFftThreads = 256;
 
-// ---- compile-time radix plan: 4s first, then 2, then 3 -----------------------------------
+// ---- compile-time radix plan: 4s first, then 2, then 3, then 5 ---------------------------------
 constexpr int radix_of(int len)
 {
-    return (len % 4 == 0) ? 4 : (len % 2 == 0) ? 2 : 3;
+    return (len % 4 == 0) ? 4 : (len % 2 == 0) ? 2 : (len % 3 == 0) ? 3 : 5;
 }
 
 template <typename T, int R, int DIR>
@@ -47,6 +47,26 @@ __device__ inline void dft_small(cplx<T> *a)
         a[1] = cadd<T>(t1, t3);
         a[2] = csub<T>(t0, t2);
         a[3] = csub<T>(t1, t3);
+    } else if constexpr (R == 5) {
+        // w = exp(DIR 2 pi i / 5): with s1 = a1 + a4, d1 = a1 - a4, s2 = a2 + a3, d2 = a2 - a3,
+        // y1,4 = a0 + c1 s1 + c2 s2 +- DIR i (n1 d1 + n2 d2), y2,3 = a0 + c2 s1 + c1 s2 +- DIR i (n2 d1 - n1 d2),
+        // c1 = cos(2 pi/5) = (sqrt5 - 1)/4, c2 = cos(4 pi/5) = -(sqrt5 + 1)/4, n1 = sin(2 pi/5), n2 = sin(4 pi/5)
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T n1 = (T)0.95105651629515357212, n2 = (T)0.58778525229247312917;
+        const cplx<T> s1 = cadd<T>(a[1], a[4]), d1 = csub<T>(a[1], a[4]);
+        const cplx<T> s2 = cadd<T>(a[2], a[3]), d2 = csub<T>(a[2], a[3]);
+        const cplx<T> m1 = mk<T>(a[0].x + c1 * s1.x + c2 * s2.x, a[0].y + c1 * s1.y + c2 * s2.y);
+        const cplx<T> m2 = mk<T>(a[0].x + c2 * s1.x + c1 * s2.x, a[0].y + c2 * s1.y + c1 * s2.y);
+        const cplx<T> e1 = mk<T>(n1 * d1.x + n2 * d2.x, n1 * d1.y + n2 * d2.y);
+        const cplx<T> e2 = mk<T>(n2 * d1.x - n1 * d2.x, n2 * d1.y - n1 * d2.y);
+        // DIR i e
+        const cplx<T> j1 = DIR < 0 ? mk<T>(e1.y, -e1.x) : mk<T>(-e1.y, e1.x);
+        const cplx<T> j2 = DIR < 0 ? mk<T>(e2.y, -e2.x) : mk<T>(-e2.y, e2.x);
+        a[0] = mk<T>(a[0].x + s1.x + s2.x, a[0].y + s1.y + s2.y);
+        a[1] = cadd<T>(m1, j1);
+        a[4] = csub<T>(m1, j1);
+        a[2] = cadd<T>(m2, j2);
+        a[3] = csub<T>(m2, j2);
     } else {  // R == 3, w = exp(DIR 2 pi i / 3)
         const T h = (T)0.86602540378443864676;  // sqrt(3)/2
         cplx<T> s = cadd<T>(a[1], a[2]), d = csub<T>(a[1], a[2]);
@@ -243,28 +263,76 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
     case 4: go(std::integral_constant<int, 4>{}); break;
     case 6: go(std::integral_constant<int, 6>{}); break;
     case 8: go(std::integral_constant<int, 8>{}); break;
+    case 10: go(std::integral_constant<int, 10>{}); break;
     case 12: go(std::integral_constant<int, 12>{}); break;
     case 16: go(std::integral_constant<int, 16>{}); break;
+    case 20: go(std::integral_constant<int, 20>{}); break;
     case 24: go(std::integral_constant<int, 24>{}); break;
+    case 30: go(std::integral_constant<int, 30>{}); break;
     case 32: go(std::integral_constant<int, 32>{}); break;
+    case 40: go(std::integral_constant<int, 40>{}); break;
     case 48: go(std::integral_constant<int, 48>{}); break;
+    case 50: go(std::integral_constant<int, 50>{}); break;
+    case 60: go(std::integral_constant<int, 60>{}); break;
     case 64: go(std::integral_constant<int, 64>{}); break;
+    case 80: go(std::integral_constant<int, 80>{}); break;
     case 96: go(std::integral_constant<int, 96>{}); break;
+    case 100: go(std::integral_constant<int, 100>{}); break;
+    case 120: go(std::integral_constant<int, 120>{}); break;
     case 128: go(std::integral_constant<int, 128>{}); break;
+    case 150: go(std::integral_constant<int, 150>{}); break;
+    case 160: go(std::integral_constant<int, 160>{}); break;
     case 192: go(std::integral_constant<int, 192>{}); break;
+    case 200: go(std::integral_constant<int, 200>{}); break;
+    case 240: go(std::integral_constant<int, 240>{}); break;
     case 256: go(std::integral_constant<int, 256>{}); break;
+    case 300: go(std::integral_constant<int, 300>{}); break;
+    case 320: go(std::integral_constant<int, 320>{}); break;
     case 384: go(std::integral_constant<int, 384>{}); break;
+    case 400: go(std::integral_constant<int, 400>{}); break;
+    case 480: go(std::integral_constant<int, 480>{}); break;
     case 512: go(std::integral_constant<int, 512>{}); break;
     default: throw hip_error("unsupported FFT length " + std::to_string(P));
     }
     XTR_LAUNCH_CHECK("pencil_fft_kernel");
 }
 
+// even lengths 2^a m, m in {1, 3, 5, 15, 25, 75}, up to 512 (Nx, Ny <= 256)
 bool fft_length_supported(int p)
 {
     switch (p) {
-    case 4: case 6: case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
-    case 256: case 384: case 512:
+    case 4:
+    case 6:
+    case 8:
+    case 10:
+    case 12:
+    case 16:
+    case 20:
+    case 24:
+    case 30:
+    case 32:
+    case 40:
+    case 48:
+    case 50:
+    case 60:
+    case 64:
+    case 80:
+    case 96:
+    case 100:
+    case 120:
+    case 128:
+    case 150:
+    case 160:
+    case 192:
+    case 200:
+    case 240:
+    case 256:
+    case 300:
+    case 320:
+    case 384:
+    case 400:
+    case 480:
+    case 512:
         return true;
     default:
         return false;

@@ -191,7 +191,7 @@ xo_c xo_plane_wave_h(const xo_section *s, double omega, double depth)
 }
 
 /* ======================================================================
- * FFT: mixed-radix (4, 2, 3) Stockham, decimation in frequency, natural
+ * FFT: mixed-radix (4, 2, 3, 5) Stockham, decimation in frequency, natural
  * order output.  Textbook DFT X[k] = sum_j x[j] exp(sign 2 pi i jk/n).
  * ====================================================================== */
 
@@ -201,6 +201,7 @@ static int fft_factor(int n, int *radix)
     while (n % 4 == 0) { radix[m++] = 4; n /= 4; }
     while (n % 2 == 0) { radix[m++] = 2; n /= 2; }
     while (n % 3 == 0) { radix[m++] = 3; n /= 3; }
+    while (n % 5 == 0) { radix[m++] = 5; n /= 5; }
     return n == 1 ? m : -1;
 }
 
@@ -213,6 +214,8 @@ static void fft1_work(xo_c *x, int n, int sign, xo_c *y, const xo_c *tw)
     xo_c *src = x, *dst = y;
     int len = n, s = 1;
     const xo_c w3 = cexp(sign * 2.0 * XO_PI * I / 3.0);
+    xo_c w5[5]; /* exp(sign 2 pi i k / 5) */
+    for (int k = 0; k < 5; k++) w5[k] = cexp(sign * 2.0 * XO_PI * I * k / 5.0);
     for (int st = 0; st < nr; st++) {
         int r = radix[st];
         int m = len / r;
@@ -236,6 +239,14 @@ static void fft1_work(xo_c *x, int n, int sign, xo_c *y, const xo_c *tw)
                     dst[q + s * (4 * p + 1)] = y1 * tw[(int64_t)p * 1 * tstep % n];
                     dst[q + s * (4 * p + 2)] = y2 * tw[(int64_t)p * 2 * tstep % n];
                     dst[q + s * (4 * p + 3)] = y3 * tw[(int64_t)p * 3 * tstep % n];
+                } else if (r == 5) { /* y_u = sum_t a_t w5^(t u), textbook */
+                    xo_c a[5];
+                    for (int t = 0; t < 5; t++) a[t] = src[q + s * (p + t * m)];
+                    for (int u = 0; u < 5; u++) {
+                        xo_c yv = 0;
+                        for (int t = 0; t < 5; t++) yv += a[t] * w5[(t * u) % 5];
+                        dst[q + s * (5 * p + u)] = u == 0 ? yv : yv * tw[(int64_t)p * u * tstep % n];
+                    }
                 } else { /* r == 3 */
                     xo_c a0 = src[q + s * (p + 0 * m)];
                     xo_c a1 = src[q + s * (p + 1 * m)];

@@ -21,10 +21,12 @@ def _three_layer_small(nx, ny, nz):
     return cm.CartesianModel(nx, ny, 300.0, 250.0, depth, thick, sigma, sec, "three-layer", [0.3])
 
 
-@pytest.mark.parametrize("which", ["small", "three-layer", "c1"])
+@pytest.mark.parametrize("which", ["small", "three-layer", "c1", "radix5"])
 def test_green_matches_oracle(oracle, torch_cuda, which):
     if which == "small":
         m = cm.small_model(4, 4, 3, dx=150.0, dz=80.0)
+    elif which == "radix5":
+        m = cm.small_model(5, 10, 3, dx=150.0, dz=80.0)        # FFT lengths 10 and 20
     elif which == "three-layer":
         m = _three_layer_small(8, 6, 4)
     else:

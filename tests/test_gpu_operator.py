@@ -21,7 +21,8 @@ def _setup(oracle, model, precision="fp64", max_rhs=2):
     return w, G, op, oop
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 2), (8, 12, 3), (16, 8, 5), (4, 4, 24), (4, 4, 33), (4, 4, 64)])
+@pytest.mark.parametrize("shape", [(4, 4, 2), (8, 12, 3), (16, 8, 5), (4, 4, 24), (4, 4, 33), (4, 4, 64),
+                                   (5, 4, 3), (25, 15, 2), (40, 40, 2)])
 @pytest.mark.parametrize("kind", ["A", "Chi0"])
 def test_apply_small_grids(oracle, torch_cuda, shape, kind):
     nx, ny, nz = shape

@@ -16,7 +16,7 @@ from extreme_amd import cartesian_model as cm
 
 # ---------------------------------------------------------------------------- DFT (A4, A6, G13)
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 9, 12, 16, 24, 48, 64, 96, 256, 384])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 16, 24, 25, 48, 50, 64, 80, 96, 150, 256, 384, 400])
 def test_fft1_matches_textbook_dft(oracle, n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -27,7 +27,7 @@ def test_fft1_matches_textbook_dft(oracle, n):
     assert rel(oracle.fft1(x, -1), np.fft.fft(x)) < 1e-13
 
 
-@pytest.mark.parametrize("n1,n2,nc", [(8, 6, 5), (16, 16, 3), (4, 24, 2)])
+@pytest.mark.parametrize("n1,n2,nc", [(8, 6, 5), (16, 16, 3), (4, 24, 2), (10, 80, 2), (50, 12, 3)])
 def test_fft2_batch_matches_numpy(oracle, n1, n2, nc):
     rng = np.random.default_rng(n1 * 100 + n2)
     x = rng.standard_normal((n1, n2, nc)) + 1j * rng.standard_normal((n1, n2, nc))
