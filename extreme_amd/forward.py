@@ -185,17 +185,20 @@ class FrequencySolve:
         op.calculate_green_atoo(level_z, electric=True)
         op.calculate_green_atoo(level_z, electric=False)
         normals = level_normal_fields(self.model.section, self.omega, level_z)
-        E, H = [], []
+        E, H, Ea, Ha = [], [], [], []
         for pol in (0, 1):
             js = torch.from_numpy(self.js[pol]).to("cuda")
             jq = op.jq(self.chi[pol], js)
             ea = op.level_field(jq, electric=True).cpu().numpy()
             ha = op.level_field(jq, electric=False).cpu().numpy()
+            Ea.append(ea)
+            Ha.append(ha)
             E.append(normals[pol][0][None, None, :] + ea)
             H.append(normals[pol][1][None, None, :] + ha)
         z = impedance(E[0], E[1], H[0], H[1])
         rho, phase = apparent_resistivity_phase(z, self.omega)
-        return {"E": E, "H": H, "Z": z, "rho": rho, "phase": phase}
+        return {"E": E, "H": H, "Z": z, "rho": rho, "phase": phase, "E_anomalous": Ea, "H_anomalous": Ha,
+                "E_normal": [normals[p][0] for p in (0, 1)], "H_normal": [normals[p][1] for p in (0, 1)]}
 
     def solve_mixed(self, inner_floor=1e-8, max_refinements=10):
         """The same two solves in mixed precision (needs precision="mixed"): chi <- chi0, then FP64

@@ -1,0 +1,146 @@
+"""The reference's file formats (project, model, anomaly layers, result tables) on the CPU, with the
+reference's COMMEMI 3D-2 example project as the fixture."""
+import os
+
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import forward as F
+from extreme_amd import project_io as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+COMMEMI = os.path.join(HERE, "golden", "commemi3d2")
+
+
+def test_commemi_project():
+    p = P.read_project(os.path.join(COMMEMI, "forward.xproj"))
+    assert p.frequencies == [0.1, 0.01, 0.001]
+    assert p.model_file == "commemi.xmodel" and p.results_path == "results_forward"
+    s = p.settings
+    assert (s.residual, s.inner_buffer_length, s.outer_buffer_length, s.max_repeats_number,
+            s.number_of_hankels) == (1e-10, 10, 15, 100, 5)
+    assert [(l.x_shift, l.y_shift, l.z) for l in p.levels] == [(-20000.0, -20000.0, 0.0), (-20000.0, 20000.0, 0.0),
+                                                               (20000.0, -20000.0, 0.0), (20000.0, 20000.0, 0.0)]
+    assert p.sites == [] and p.sources == []
+
+
+def test_commemi_model():
+    m = P.read_model(os.path.join(COMMEMI, "commemi.xmodel"))
+    assert (m.nx, m.ny, m.nz, m.dx, m.dy) == (40, 40, 40, 2000.0, 2000.0)
+    assert m.section.zero_level == -200000.0
+    assert m.section.thickness == [0.0, 200000.0, 10000.0, 20000.0, 0.0]
+    assert m.section.sigma == [0.0, 0.0, 0.1, 0.01, 10.0]
+    assert np.array_equal(m.depth, 250.0 * np.arange(40)) and np.all(m.thick == 250.0)
+    # every layer: the 10 ohm-m host with a 1 ohm-m block (x 20-40 km) and a 100 ohm-m block
+    # (x 40-60 km), both at y 20-60 km
+    ref = np.full((40, 40), 0.1)
+    ref[10:20, 10:30] = 1.0
+    ref[20:30, 10:30] = 0.01
+    for k in range(40):
+        assert np.array_equal(m.sigma[:, :, k], ref), k
+    # every anomaly layer nests in the 10 km background layer
+    assert set(m.background_indices()) == {2}
+
+
+def test_settings_reader_defaults(tmp_path):
+    """ForwardSettingsReader's own fallbacks (not the class defaults) for missing elements."""
+    f = tmp_path / "p.xproj"
+    f.write_text('<?xml version="1.0"?><ExtremeProject><Frequencies><F>1</F></Frequencies>'
+                 '<Settings><Forward><Residual>1E-8</Residual></Forward></Settings></ExtremeProject>')
+    s = P.read_project(str(f)).settings
+    assert (s.residual, s.inner_buffer_length, s.outer_buffer_length, s.max_repeats_number,
+            s.number_of_hankels) == (1e-8, 10, 10, 10, 10)
+
+
+def test_applique_and_plain_text(tmp_path):
+    """Applique runs fill sigma[j, i, k] (the reference's order) with float32 values; plain-text
+    layers read nx lines of ny values with Fortran D exponents."""
+    sigma = np.zeros((4, 3, 1))
+    P.parse_applique("\n 2LINES: 1*0.5 3*1.5\n\n1lines: 4*2e-1\n", sigma, 0)
+    want = np.zeros((4, 3))
+    want[0, 0:2] = np.float32(0.5)
+    want[1:4, 0:2] = np.float32(1.5)
+    want[:, 2] = np.float32(0.2)
+    assert np.array_equal(sigma[:, :, 0], want)
+    with pytest.raises(ValueError):
+        P.parse_applique("4LINES: 1*1", np.zeros((4, 3, 1)), 0)
+    path = tmp_path / "layer.dat"
+    path.write_text("1 2.5D-1 3\n4\t5 6E0\n")
+    assert np.array_equal(P.read_plain_text_layer(str(path), 2, 3), [[1, 0.25, 3], [4, 5, 6]])
+    with pytest.raises(ValueError):
+        P.read_plain_text_layer(str(path), 2, 4)
+
+
+def test_model_round_trip(tmp_path):
+    m = cm.small_model(5, 3, 4, seed=8)
+    P.write_model(m, str(tmp_path / "m.xmodel"))
+    back = P.read_model(str(tmp_path / "m.xmodel"))
+    assert (back.nx, back.ny, back.dx, back.dy) == (m.nx, m.ny, m.dx, m.dy)
+    assert np.array_equal(back.depth, m.depth) and np.array_equal(back.thick, m.thick)
+    assert np.array_equal(back.sigma, m.sigma)                 # shortest round-trip text
+    assert back.section.thickness == list(map(float, m.section.thickness))
+    assert back.section.sigma == list(map(float, m.section.sigma))
+
+
+def test_net_number_formats():
+    assert P.net_e(1234.56) == "1.234560E+003"
+    assert P.net_e(-0.5) == "-5.000000E-001"
+    assert P.net_e(0.0) == "0.000000E+000"
+    assert P.net_e(1e-100) == "1.000000E-100"
+    assert P.net_e(float("nan")) == "NaN"
+    assert P.frequency_to_string(0.1) == "0000000000.1000000000"
+    assert P.fields_file_name(1000.0) == "fields_freq0000001000.0000000000"
+    assert P.responses_file_name(0.001) == "responses_freq0000000000.0010000000"
+
+
+def _synthetic_level(m, z, seed):
+    rng = np.random.default_rng(seed)
+
+    def field():
+        return rng.standard_normal((m.nx, m.ny, 3)) + 1j * rng.standard_normal((m.nx, m.ny, 3))
+
+    lev = P.ObservationLevel(0.0, 0.0, z)
+    return P.LevelFields(lev, (field(), field()), (field(), field()), (field(), field()), (field(), field()))
+
+
+def test_result_tables(tmp_path):
+    """Column layout, coordinates and values of fields_* and responses_* (the 6-decimal E format)."""
+    m = cm.small_model(3, 2, 2, dx=500.0)
+    levels = [_synthetic_level(m, 0.0, 1), _synthetic_level(m, -10.0, 2)]
+    freq = 0.25
+    P.export_raw_fields(str(tmp_path / "f"), m, levels, freq)
+    names, rows = P.read_result_table(str(tmp_path / "f"))
+    assert names[:4] == ["x", "y", "z", "freq"] and len(names) == 4 + 48
+    assert names[4:8] == ["N_polX_Ex_re", "N_polX_Ex_im", "N_polX_Ey_re", "N_polX_Ey_im"]
+    assert names[-2:] == ["A_polY_Hz_re", "A_polY_Hz_im"]
+    assert rows.shape == (2 * m.nx * m.ny, 52)
+    assert np.allclose(rows[:6, 0], [250, 250, 750, 750, 1250, 1250]) and np.allclose(rows[:2, 1], [250, 750])
+    assert np.all(rows[:6, 2] == 0) and np.all(rows[6:, 2] == -10) and np.all(rows[:, 3] == freq)
+    lf = levels[0]
+    got = rows[1, 4:].reshape(8, 3, 2)                             # site (0, 1)
+    vecs = [lf.normal_e[0], lf.normal_h[0], lf.normal_e[1], lf.normal_h[1],
+            lf.anomaly_e[0], lf.anomaly_h[0], lf.anomaly_e[1], lf.anomaly_h[1]]
+    want = np.array([[[v[0, 1, c].real, v[0, 1, c].imag] for c in range(3)] for v in vecs])
+    assert np.allclose(got, want, rtol=1e-6, atol=0)
+    P.export_mt_responses(str(tmp_path / "r"), m, levels, freq)
+    names, rows = P.read_result_table(str(tmp_path / "r"))
+    assert len(names) == 4 + 8 + 8 + 8 and names[4] == "Z_XX_re" and names[12] == "Rho_XX" and names[-1] == "EW_ZY_im"
+    e1, e2 = lf.normal_e[0] + lf.anomaly_e[0], lf.normal_e[1] + lf.anomaly_e[1]
+    h1, h2 = lf.normal_h[0] + lf.anomaly_h[0], lf.normal_h[1] + lf.anomaly_h[1]
+    z = F.impedance(e1, e2, h1, h2)                               # the solver side's own Z
+    rho, phs = F.apparent_resistivity_phase(z, cm.omega_of(freq))
+    zz = rows[1, 4:12].reshape(4, 2)
+    assert np.allclose(zz[:, 0] + 1j * zz[:, 1], z[0, 1].reshape(-1), rtol=1e-6)
+    assert np.allclose(rows[1, 12:16], rho[0, 1].reshape(-1), rtol=1e-6)
+    assert np.allclose(rows[1, 16:20], phs[0, 1].reshape(-1), rtol=1e-6, atol=1e-6)
+    # tipper: [Wzx Wzy] [H1 H2]_xy = [h1z h2z]
+    w = rows[1, 20:24].reshape(2, 2)
+    wzx, wzy = w[0, 0] + 1j * w[0, 1], w[1, 0] + 1j * w[1, 1]
+    for hh in (h1, h2):
+        assert abs(wzx * hh[0, 1, 0] + wzy * hh[0, 1, 1] - hh[0, 1, 2]) < 1e-5 * abs(hh[0, 1]).max()
+
+
+def test_run_project_refuses_shifted_levels():
+    with pytest.raises(NotImplementedError):
+        P.run_project(os.path.join(COMMEMI, "forward.xproj"))
