@@ -484,18 +484,19 @@ double log10_rho_step()
 
 struct HostPlan {
     std::vector<std::vector<double>> rho, lam;
-    std::vector<double> merged;
+    std::vector<double> merged;           // with a zero-rho plan, merged[0] = 0 and m_item[0] = -1
     std::vector<int> m_item, m_idx;
+    bool zero_rho = false;
 };
 
-HostPlan make_plan(const GreenInput &in)
+// ScalarPlansCreater.CreateFor (:102-110) over [rmin, rmax]; zero_rho puts rho = 0 in front of the
+// merged radii (ScalarPlan.GetSortedRho :56-65, GreenScalarCalculator.CalculateZeroRho :147-157)
+HostPlan make_plan(int number_of_hankels, double rmin, double rmax, bool zero_rho)
 {
     HostPlan hp;
-    const int nh = std::abs(in.number_of_hankels);
-    const double h1 = in.dx / 2, h2 = in.dy / 2;
-    const double rmin = std::min(h1, h2);                                   // ScalarPlansCreater.cs:360-366
-    const double mx = in.g.nx * in.dx - h1, my = in.g.ny * in.dy - h2;
-    const double rmax = std::sqrt(mx * mx + my * my);                       // :368-380
+    hp.zero_rho = zero_rho;
+    const int nh = std::abs(number_of_hankels);
+    if (nh < 1 || !(rmin > 0)) throw hip_error("scalar plan needs NumberOfHankels >= 1 and rho_min > 0");
     const double step = log10_rho_step();
     std::vector<double> r1;                                                 // CreateRhoFromMin :142-157
     for (double nr = rmin; nr <= rmax; nr *= step) r1.push_back(nr);
@@ -527,15 +528,63 @@ HostPlan make_plan(const GreenInput &in)
     for (int it = 0; it < nh; ++it)
         for (int ix = 0; ix < (int)hp.rho[it].size(); ++ix) all.push_back({hp.rho[it][ix], it, ix});
     std::stable_sort(all.begin(), all.end(), [](const E &a, const E &b) { return a.r < b.r; });
+    if (zero_rho) {
+        hp.merged.push_back(0.0);
+        hp.m_item.push_back(-1);
+        hp.m_idx.push_back(0);
+    }
     for (auto &e : all) {
-        if (!hp.merged.empty() && hp.merged.back() == e.r) continue;
+        if ((int)hp.merged.size() > (int)zero_rho && hp.merged.back() == e.r) continue;
         hp.merged.push_back(e.r);
         hp.m_item.push_back(e.it);
         hp.m_idx.push_back(e.ix);
     }
-    if (hp.merged.size() != all.size())
+    if (hp.merged.size() != all.size() + (zero_rho ? 1 : 0))
         throw hip_error("scalar plan has duplicate rho values; the reference's radii and scalars would misalign");
     return hp;
+}
+
+// the anomaly's own range, CreateForAnomalyToAnomaly (ScalarPlansCreater.cs:95-100, 360-380)
+HostPlan make_plan(const GreenInput &in)
+{
+    const double h1 = in.dx / 2, h2 = in.dy / 2;
+    const double mx = in.g.nx * in.dx - h1, my = in.g.ny * in.dy - h2;
+    return make_plan(in.number_of_hankels, std::min(h1, h2), std::sqrt(mx * mx + my * my), false);
+}
+
+// FindRhoMin along one axis (ScalarPlansCreater.cs:245-266) for a level shifted by `shift` against
+// the cell centres 0, h, ..., (n - 1) h. The reference's System.Decimal arithmetic and these doubles
+// agree whenever h and the shift are whole multiples of a power-of-two fraction of a metre.
+double level_rho_min_axis(int n, double h, double shift)
+{
+    const double a0 = -h / 2, a1 = n * h - h / 2;
+    const double y0 = shift - h / 2, y1 = y0 + h * n;
+    if ((y0 >= a0 && y0 <= a1) || (y1 >= a0 && y1 <= a1)) {
+        const double rem = std::fmod(std::fabs(a0 - y0), h);
+        return rem == 0 ? h / 2 : std::fabs(h / 2 - rem);
+    }
+    const double c1 = h * (n - 1);
+    return std::min(std::min(std::fabs(y0), std::fabs(y0 - c1)), std::min(std::fabs(y1), std::fabs(y1 - c1)));
+}
+
+// CreateForAnomalyToObservationLevels (:51-66) for one level shifted by (sx, sy): FindRhoMax over
+// the anomaly's corner cells and the level footprint's corners (:200-236), FindRhoMin (:238-243),
+// and a zero-rho plan from 0.001 min(dx, dy) when the minimum is 0.
+HostPlan make_level_plan(const GreenInput &in, double sx, double sy)
+{
+    const double ex = in.dx * (in.g.nx - 1), ey = in.dy * (in.g.ny - 1);
+    const double x1 = sx - in.dx / 2, y1 = sy - in.dy / 2;
+    const double x2 = x1 + in.dx * in.g.nx, y2 = y1 + in.dy * in.g.ny;
+    const double ax[4] = {0, 0, ex, ex}, ay[4] = {0, ey, ey, 0};
+    const double cx[4] = {x1, x2, x2, x1}, cy[4] = {y1, y1, y2, y2};
+    double rmax = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c)
+            rmax = std::max(rmax, std::sqrt((cx[c] - ax[a]) * (cx[c] - ax[a]) + (cy[c] - ay[a]) * (cy[c] - ay[a])));
+    double rmin = std::min(level_rho_min_axis(in.g.nx, in.dx, sx), level_rho_min_axis(in.g.ny, in.dy, sy));
+    const bool zero = rmin <= 0;
+    if (zero) rmin = std::min(in.dx, in.dy) * 0.001;
+    return make_plan(in.number_of_hankels, rmin, rmax, zero);
 }
 
 template <typename T>
@@ -816,19 +865,14 @@ __global__ void point_u_kernel(LayerModel lm, int nz, const PointInfo *pts, int 
 // NormalizeAfterHankelForE/H (ScalarMathUtils.cs:131-155): comp c uses J1 or J0 and the factor
 // sign / (den rho). electric: J1 -, J1 +, J0 +, J0 +, J1 + with den 2 pi; magnetic: J1 -, J1 -,
 // J0 -, J0 + with den 4 pi.
-__global__ void atoo_hankel_kernel(int nz, int nitems, int ncomp, int nlam_max, const double2 *U, int nmerged,
-                                   const int *m_item, const int *m_idx, const double *radii, const double *hank0,
-                                   const double *hank1, int electric, double2 *I)
+__device__ inline double2 atoo_hankel_at(int nitems, int ncomp, int nlam_max, const double2 *U, int tr, int it,
+                                         int idx, int c, double rho, const double *hank0, const double *hank1,
+                                         int electric)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tr = blockIdx.y;
-    const int c = blockIdx.z;
-    if (k >= nmerged || tr >= nz || c >= ncomp) return;
-    const int it = m_item[k], idx = m_idx[k];
     const bool j0 = c == 2 || c == 3;
     const double *hank = j0 ? hank0 : hank1;
     const double sgn = electric ? (c == 0 ? -1.0 : 1.0) : (c == 3 ? 1.0 : -1.0);
-    const double den = (electric ? 2 * kPi : 4 * kPi) * radii[k];
+    const double den = (electric ? 2 * kPi : 4 * kPi) * rho;
     const double2 *f = U + (((int64_t)tr * nitems + it) * ncomp + c) * nlam_max + idx;
     double re = 0, im = 0;
     for (int j = 0; j < kTaps; ++j) {
@@ -837,12 +881,39 @@ __global__ void atoo_hankel_kernel(int nz, int nitems, int ncomp, int nlam_max, 
         re += v.x * h;
         im += v.y * h;
     }
-    I[((int64_t)c * nz + tr) * nmerged + k] = make_double2(sgn * re / den, sgn * im / den);
+    return make_double2(sgn * re / den, sgn * im / den);
+}
+
+// A merged radius with item -1 is the zero-rho point (CalculateZeroRhoForE/H, ScalarMathUtils.cs:33-81):
+// I1, I2 (and I5) are 0 there, I3 and I4 are (f[1] a^2 - f[2]) / (a^2 - 1) from the first item's 2nd and
+// 3rd radii rho01, rho02, a the rho step.
+__global__ void atoo_hankel_kernel(int nz, int nitems, int ncomp, int nlam_max, const double2 *U, int nmerged,
+                                   const int *m_item, const int *m_idx, const double *radii, const double *hank0,
+                                   const double *hank1, int electric, double rho01, double rho02, double alpha2,
+                                   double2 *I)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tr = blockIdx.y;
+    const int c = blockIdx.z;
+    if (k >= nmerged || tr >= nz || c >= ncomp) return;
+    const int it = m_item[k], idx = m_idx[k];
+    double2 v;
+    if (it >= 0) {
+        v = atoo_hankel_at(nitems, ncomp, nlam_max, U, tr, it, idx, c, radii[k], hank0, hank1, electric);
+    } else if (c == 2 || c == 3) {
+        const double2 f1 = atoo_hankel_at(nitems, ncomp, nlam_max, U, tr, 0, 1, c, rho01, hank0, hank1, electric);
+        const double2 f2 = atoo_hankel_at(nitems, ncomp, nlam_max, U, tr, 0, 2, c, rho02, hank0, hank1, electric);
+        v = make_double2((f1.x * alpha2 - f2.x) / (alpha2 - 1), (f1.y * alpha2 - f2.y) / (alpha2 - 1));
+    } else {
+        v = make_double2(0, 0);
+    }
+    I[((int64_t)c * nz + tr) * nmerged + k] = v;
 }
 
 struct AoArgs {
     int nx, ny, nz, nr, electric, ncomp, ntc;
     double dx, dy;
+    double xs, ys;                // minus the level's shift (AtoOLevelGreenTensorCalculator.cs:119-125)
     const double2 *segA, *segB;   // [seg][comp][tr]
     const double *radii;
     double2 *G;                   // [2Nx][2Ny][ntc][nz]
@@ -935,10 +1006,10 @@ __global__ __launch_bounds__(64) void atoo_lateral_kernel(AoArgs a)
     const double dx = a.dx, dy = a.dy;
     Q9 G;
     for (int t = 0; t < AO_N; ++t) G.q[t] = c_(0);
-    for (int e = 0; e < 2; ++e) {     // edges along x at y = dy (jj - 1/2): jj = j (-1), j + 1 (+1)
+    for (int e = 0; e < 2; ++e) {     // edges along x at y = dy (jj - 1/2) - ys: jj = j (-1), j + 1 (+1)
         const int jj = j + e, sign = e == 0 ? -1 : 1;
-        const double y = dy * (jj - 0.5);
-        const double x1 = dx * (i - 0.5), x2 = dx * (i + 0.5);
+        const double y = dy * (jj - 0.5) - a.ys;
+        const double x1 = dx * (i - 0.5) - a.xs, x2 = dx * (i + 0.5) - a.xs;
         const Knot kl = make_knot(a.radii, a.nr, x1, y), kr = make_knot(a.radii, a.nr, x2, y);
         if (x1 > 0 && x2 > 0) {
             ao_fill(a, G, ao_integrate(a, tr, kl, kr, x1, x2, y, false), sign, 0);
@@ -950,10 +1021,10 @@ __global__ __launch_bounds__(64) void atoo_lateral_kernel(AoArgs a)
             ao_fill(a, G, ao_integrate(a, tr, kz, kr, 0, x2, y, false), sign, 0);
         }
     }
-    for (int e = 0; e < 2; ++e) {     // edges along y at x = dx (ii - 1/2): ii = i (-1), i + 1 (+1)
+    for (int e = 0; e < 2; ++e) {     // edges along y at x = dx (ii - 1/2) - xs: ii = i (-1), i + 1 (+1)
         const int ii = i + e, sign = e == 0 ? -1 : 1;
-        const double x = dx * (ii - 0.5);
-        const double y1 = dy * (j - 0.5), y2 = dy * (j + 0.5);
+        const double x = dx * (ii - 0.5) - a.xs;
+        const double y1 = dy * (j - 0.5) - a.ys, y2 = dy * (j + 0.5) - a.ys;
         const Knot kl = make_knot(a.radii, a.nr, x, y1), kr = make_knot(a.radii, a.nr, x, y2);
         if (y1 > 0 && y2 > 0) {
             ao_fill(a, G, ao_integrate(a, tr, kl, kr, y1, y2, x, true), sign, 1);
@@ -982,8 +1053,8 @@ int atoo_components(int electric, int *slot)
     return electric ? 8 : 7;
 }
 
-void build_green_atoo_level(const GreenInput &in, double level_z, int electric, double2 *G, const double2 *tw_x,
-                            const double2 *tw_y, hipStream_t s)
+void build_green_atoo_level(const GreenInput &in, double level_z, double shift_x, double shift_y, int electric,
+                            double2 *G, const double2 *tw_x, const double2 *tw_y, hipStream_t s)
 {
     const Geom &g = in.g;
     const int nz = g.nz;
@@ -1001,8 +1072,9 @@ void build_green_atoo_level(const GreenInput &in, double level_z, int electric, 
         d += in.section.thickness[k];
     }
     lm.depth[nl] = d;
-    // a level without lateral shift has the anomaly's rho range (ScalarPlansCreater.cs:51-66, 180-250)
-    const HostPlan hp = make_plan(in);
+    // the level's own scalar plan (AtoOCalculator.cs:129, 151)
+    const HostPlan hp = make_level_plan(in, shift_x, shift_y);
+    if (hp.zero_rho && hp.rho[0].size() < 3) throw hip_error("zero-rho plan needs three radii in its first item");
     const int nitems = (int)hp.rho.size();
     int nlam_max = 0;
     std::vector<int> nlam(nitems);
@@ -1056,7 +1128,9 @@ void build_green_atoo_level(const GreenInput &in, double level_z, int electric, 
     {
         dim3 grid((unsigned)ceil_div(nmerged, 128), nz, ncomp);
         hipLaunchKernelGGL(atoo_hankel_kernel, grid, dim3(128), 0, s, nz, nitems, ncomp, nlam_max, U.p, nmerged,
-                           d_mitem.p, d_midx.p, d_radii.p, d_h0.p, d_h1.p, electric, I.p);
+                           d_mitem.p, d_midx.p, d_radii.p, d_h0.p, d_h1.p, electric,
+                           hp.zero_rho ? hp.rho[0][1] : 0.0, hp.zero_rho ? hp.rho[0][2] : 0.0,
+                           log10_rho_step() * log10_rho_step(), I.p);
         XTR_LAUNCH_CHECK("atoo_hankel_kernel");
     }
     const int nseg = nmerged - 1;
@@ -1076,6 +1150,8 @@ void build_green_atoo_level(const GreenInput &in, double level_z, int electric, 
     a.ntc = atoo_components(electric, a.slot);
     a.dx = in.dx;
     a.dy = in.dy;
+    a.xs = -shift_x;
+    a.ys = -shift_y;
     a.segA = segA.p;
     a.segB = segB.p;
     a.radii = d_radii.p;

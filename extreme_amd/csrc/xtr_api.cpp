@@ -1184,7 +1184,7 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
 }
 
 
-int xtr_op_build_green_atoo(xtr_op *op, double level_z, int electric)
+int xtr_op_build_green_atoo(xtr_op *op, double level_z, double x_shift, double y_shift, int electric)
 {
     return guarded([&] {
         require(op, "null argument");
@@ -1195,7 +1195,9 @@ int xtr_op_build_green_atoo(xtr_op *op, double level_z, int electric)
         const int ncomp = xtr::atoo_components(e, slot);
         op->go_ready[e] = false;
         op->Go[e].alloc((size_t)(op->g.L() * ncomp * op->g.nz));
-        xtr::build_green_atoo_level(green_input(op), level_z, e, op->Go[e].p, op->tw64_x.p, op->tw64_y.p, op->stream);
+        require(std::isfinite(x_shift) && std::isfinite(y_shift), "level shift must be finite");
+        xtr::build_green_atoo_level(green_input(op), level_z, x_shift, y_shift, e, op->Go[e].p, op->tw64_x.p,
+                                    op->tw64_y.p, op->stream);
         op->go_ready[e] = true;
     });
 }

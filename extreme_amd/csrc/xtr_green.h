@@ -28,11 +28,12 @@ struct GreenInput {
 // into G (L * gblk complex128), ordered on stream s.
 void build_green_atoa(const GreenInput &in, double2 *G, const double2 *tw_x, const double2 *tw_y, hipStream_t s);
 
-// AtoO level tensor for the observation level at depth level_z (no lateral shift): device layout
-// [L][component][nz] with components, in order, xx yy zz xy xz yz zx zy (electric = 1) or
-// xx xy yx xz yz zx zy (electric = 0), FFT'd; G holds L * ncomp * nz complex128.
+// AtoO level tensor for the observation level at depth level_z whose sites are the cell centres
+// shifted by (shift_x, shift_y) metres: device layout [L][component][nz] with components, in order,
+// xx yy zz xy xz yz zx zy (electric = 1) or xx xy yx xz yz zx zy (electric = 0), FFT'd; G holds
+// L * ncomp * nz complex128.
 int atoo_components(int electric, int *slot /* [9]: xx yy zz xy yx xz yz zx zy -> compact or -1 */);
-void build_green_atoo_level(const GreenInput &in, double level_z, int electric, double2 *G, const double2 *tw_x,
-                            const double2 *tw_y, hipStream_t s);
+void build_green_atoo_level(const GreenInput &in, double level_z, double shift_x, double shift_y, int electric,
+                            double2 *G, const double2 *tw_x, const double2 *tw_y, hipStream_t s);
 
 }  // namespace xtr

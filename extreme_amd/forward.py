@@ -176,14 +176,15 @@ class FrequencySolve:
         self.solver.history.clear()
         return self.solver.solve_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]])
 
-    def observe(self, level_z: float = 0.0) -> dict:
+    def observe(self, level_z: float = 0.0, x_shift: float = 0.0, y_shift: float = 0.0) -> dict:
         """Fields and responses at the observation level after solve(): level tensors on the GPU,
-        jQ from chi, anomalous E and H by the AtoO convolution (AtoOCalculator.cs:63-137), plus the
-        level's normal field; then Z, rho_a and phase per site (cell centres, no shift)."""
+        jQ from chi, anomalous E and H by the AtoO convolution (AtoOCalculator.cs:63-160), plus the
+        level's normal field; then Z, rho_a and phase per site (the cell centres shifted by
+        (x_shift, y_shift) metres)."""
         torch = self.torch
         op = self.op
-        op.calculate_green_atoo(level_z, electric=True)
-        op.calculate_green_atoo(level_z, electric=False)
+        op.calculate_green_atoo(level_z, electric=True, x_shift=x_shift, y_shift=y_shift)
+        op.calculate_green_atoo(level_z, electric=False, x_shift=x_shift, y_shift=y_shift)
         normals = level_normal_fields(self.model.section, self.omega, level_z)
         E, H, Ea, Ha = [], [], [], []
         for pol in (0, 1):

@@ -414,14 +414,12 @@ def read_result_table(path: str):
 def run_project(path: str, precision: str = "fp64", results_path: str | None = None, log=print) -> list:
     """ExtremeMtSolverRunner.Run (EndUser/ExtremeMt/ExtremeMtSolverRunner.cs:47-76) on the GPU:
     for each frequency, solve both polarisations, compute every observation level's fields and
-    write fields_freq* and responses_freq* into the results path. Levels with a lateral shift are
-    not built here (their tensors need the shifted-offset lateral integration), nor are sites."""
+    write fields_freq* and responses_freq* into the results path. Observation sites (single points
+    off the level grid) are not built here."""
     from . import forward as F
     project = read_project(path)
     if project.model_file is None:
         raise ValueError("the project names no model file")
-    if any(l.x_shift != 0 or l.y_shift != 0 for l in project.levels):
-        raise NotImplementedError("observation levels with a lateral shift are not supported")
     if project.sites:
         raise NotImplementedError("observation sites are not supported")
     model = read_model(project.resolve(project.model_file), project.frequencies)
@@ -434,7 +432,7 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
         log(f"frequency {freq}: {[getattr(i, 'number_of_iterations', getattr(i, 'refinements', None)) for i in infos]}")
         levels = []
         for lev in project.levels:
-            ob = fs.observe(lev.z)
+            ob = fs.observe(lev.z, lev.x_shift, lev.y_shift)
             shape = (model.nx, model.ny, 3)
             ne = tuple(np.broadcast_to(v, shape) for v in ob["E_normal"])
             nh = tuple(np.broadcast_to(v, shape) for v in ob["H_normal"])

@@ -98,7 +98,7 @@ def lib():
         L.xtr_fgmres_solve_models_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_int, C.POINTER(C.c_int),
                                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                      C.POINTER(_Info), ITER_FN, C.c_void_p]
-        L.xtr_op_build_green_atoo.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.xtr_op_build_green_atoo.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int]
         L.xtr_op_get_green_atoo.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.xtr_op_jq_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.xtr_op_level_field_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -297,10 +297,13 @@ class ConvolutionOperator:
     AO_ELECTRIC = ("xx", "yy", "zz", "xy", "xz", "yz", "zx", "zy")
     AO_MAGNETIC = ("xx", "xy", "yx", "xz", "yz", "zx", "zy")
 
-    def calculate_green_atoo(self, level_z: float = 0.0, electric: bool = True):
-        """AtoOCalculator.CalculateGreenTensorAtoOLevelElectric / ...Magnetic (AtoOCalculator.cs:101-137):
-        the level tensor, assembled and FFT'd on the GPU and kept in the handle."""
-        _check(lib().xtr_op_build_green_atoo(self._h, float(level_z), 1 if electric else 0))
+    def calculate_green_atoo(self, level_z: float = 0.0, electric: bool = True, x_shift: float = 0.0,
+                             y_shift: float = 0.0):
+        """AtoOCalculator.CalculateGreenTensorAtoOLevelElectric / ...Magnetic (AtoOCalculator.cs:124-160):
+        the tensor of the level at depth level_z whose sites are the cell centres shifted by
+        (x_shift, y_shift) metres, assembled and FFT'd on the GPU and kept in the handle."""
+        _check(lib().xtr_op_build_green_atoo(self._h, float(level_z), float(x_shift), float(y_shift),
+                                             1 if electric else 0))
 
     def green_atoo(self, electric: bool = True) -> dict:
         """The level tensor copied back, component -> [2Nx*2Ny, Nz]."""

@@ -205,10 +205,12 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
 /* ---- observation level (AtoO) ----------------------------------------------------------------- */
 
 /* Replaces: AtoOCalculator.CalculateGreenTensorAtoOLevelElectric / ...Magnetic
- * (Extreme.Cartesian/Forward/ToObs/AtoOCalculator.cs:101-137): the FP64 level tensor of the
- * observation level at depth level_z (cell centres, no lateral shift), assembled and FFT'd on the
- * device and kept in the handle. electric = 1: E (xx yy zz xy xz yz zx zy); 0: H (xx xy yx xz yz zx zy). */
-int xtr_op_build_green_atoo(xtr_op *op, double level_z, int electric);
+ * (Extreme.Cartesian/Forward/ToObs/AtoOCalculator.cs:124-160): the FP64 level tensor of the
+ * ObservationLevel at depth level_z whose sites are the cell centres shifted by (x_shift, y_shift)
+ * metres (ObservationLevel.ShiftAlongX/Y), with the level's own scalar plan, assembled and FFT'd on
+ * the device and kept in the handle. electric = 1: E (xx yy zz xy xz yz zx zy); 0: H (xx xy yx xz
+ * yz zx zy). */
+int xtr_op_build_green_atoo(xtr_op *op, double level_z, double x_shift, double y_shift, int electric);
 
 /* Copies a level tensor back: comps[9] in the order xx yy zz xy yx xz yz zx zy, each [2Nx*2Ny][Nz]
  * (the reference's AlongVertical layout, AtoOLevelGreenTensorCalculator.cs:52-57); NULL entries

@@ -90,7 +90,17 @@ def lib():
         L.xo_set_threads.argtypes = [C.c_int]
         L.xo_mu0.restype = C.c_double
         L.xo_green_atoo_level.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, C.c_int, f64p, f64p,
-                                          C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+                                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.POINTER(C.c_void_p)]
+        L.xo_green_atoo_level_plan.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, C.c_int, f64p,
+                                               f64p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                               C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.xo_atoo_level_rho_range.argtypes = [C.POINTER(Grid), C.c_double, C.c_double, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
+        L.xo_atoo_level_scalars.argtypes = [C.POINTER(Section), C.POINTER(Grid), C.c_double, C.c_int, f64p, f64p,
+                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, c128p, C.c_int,
+                                            f64p]
+        L.xo_plan_create_range.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p]
         L.xo_atoo_field.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_int, c128p, c128p]
         L.xo_log10_rho_step.restype = C.c_double
         L.xo_hankel_lambdas.restype = C.c_int
@@ -190,15 +200,21 @@ class _Plan(C.Structure):
     _fields_ = [("nitems", C.c_int), ("nrho", C.POINTER(C.c_int)), ("rho", C.POINTER(C.POINTER(C.c_double))),
                 ("nlam", C.POINTER(C.c_int)), ("lam", C.POINTER(C.POINTER(C.c_double))),
                 ("nrho_merged", C.c_int), ("rho_merged", C.POINTER(C.c_double)),
-                ("merge_item", C.POINTER(C.c_int)), ("merge_index", C.POINTER(C.c_int))]
+                ("merge_item", C.POINTER(C.c_int)), ("merge_index", C.POINTER(C.c_int)), ("zero_rho", C.c_int)]
 
 
-def scalar_plan(model, number_of_hankels=10):
-    """ScalarPlansCreater.CreateForAnomalyToAnomaly: returns (list of rho, list of lambda, merged rho)."""
+def scalar_plan(model, number_of_hankels=10, rho_range=None):
+    """ScalarPlansCreater.CreateForAnomalyToAnomaly, or CreateFor with rho_range = (rho_min,
+    rho_max, zero_rho): returns the items' rho and lambda and the merged rho (rho = 0 first, item
+    -1, with zero_rho)."""
     k = _Keep()
     g = make_grid(model, k)
     p = _Plan()
-    rc = lib().xo_plan_create(C.byref(g), number_of_hankels, C.byref(p))
+    if rho_range is None:
+        rc = lib().xo_plan_create(C.byref(g), number_of_hankels, C.byref(p))
+    else:
+        rc = lib().xo_plan_create_range(number_of_hankels, float(rho_range[0]), float(rho_range[1]),
+                                        1 if rho_range[2] else 0, C.byref(p))
     if rc:
         raise RuntimeError(f"xo_plan_create failed: {rc}")
     rho = [np.ctypeslib.as_array(p.rho[i], (p.nrho[i],)).copy() for i in range(p.nitems)]
@@ -257,9 +273,42 @@ AO_ELECTRIC = ("xx", "yy", "zz", "xy", "xz", "yz", "zx", "zy")
 AO_MAGNETIC = ("xx", "xy", "yx", "xz", "yz", "zx", "zy")
 
 
-def green_atoo_level(model, omega, level_z=0.0, electric=True, number_of_hankels=10, fft=True):
-    """AtoO level tensor (AtoOCalculator.CalculateGreenTensorAtoOLevel{Electric,Magnetic}):
-    dict comp -> [2Nx*2Ny, Nz] for the components the reference computes."""
+def atoo_level_rho_range(model, x_shift=0.0, y_shift=0.0):
+    """rho range of the level's scalar plan (ScalarPlansCreater.CreateForAnomalyToObservationLevels,
+    :51-66): (rho_min, rho_max, zero_rho)."""
+    k = _Keep()
+    g = make_grid(model, k)
+    rmin, rmax = C.c_double(), C.c_double()
+    zero = lib().xo_atoo_level_rho_range(C.byref(g), float(x_shift), float(y_shift), C.byref(rmin), C.byref(rmax))
+    return rmin.value, rmax.value, bool(zero)
+
+
+def atoo_level_scalars(model, omega, level_z, rho_range, electric=True, number_of_hankels=10):
+    """The level's scalars (PlanCalculator.CalculateForE/H with GreenScalarCalculator.CalculateZeroRho)
+    on the merged rho grid of the plan rho_range = (rho_min, rho_max, zero_rho): (rho [n],
+    I [ncomp, nz, n])."""
+    h0, h1 = hankel_table()
+    k = _Keep()
+    s = make_section(model.section, k)
+    g = make_grid(model, k)
+    ncomp = 5 if electric else 4
+    cap = 8192
+    out = np.zeros(ncomp * model.nz * cap, dtype=np.complex128)
+    rho = np.zeros(cap)
+    n = lib().xo_atoo_level_scalars(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1, float(level_z),
+                                    float(rho_range[0]), float(rho_range[1]), 1 if rho_range[2] else 0,
+                                    1 if electric else 0, out, cap, rho)
+    if n <= 0:
+        raise RuntimeError(f"xo_atoo_level_scalars failed: {n}")
+    return rho[:n].copy(), out[:ncomp * model.nz * n].reshape(ncomp, model.nz, n)
+
+
+def green_atoo_level(model, omega, level_z=0.0, electric=True, number_of_hankels=10, fft=True, x_shift=0.0,
+                     y_shift=0.0, rho_range=None):
+    """AtoO level tensor (AtoOCalculator.CalculateGreenTensorAtoOLevel{Electric,Magnetic}) of the
+    level at depth level_z whose sites are the cell centres shifted by (x_shift, y_shift) metres:
+    dict comp -> [2Nx*2Ny, Nz] for the components the reference computes. rho_range =
+    (rho_min, rho_max, zero_rho) replaces the level's own scalar plan range."""
     h0, h1 = hankel_table()
     k = _Keep()
     s = make_section(model.section, k)
@@ -268,8 +317,14 @@ def green_atoo_level(model, omega, level_z=0.0, electric=True, number_of_hankels
     comps = AO_ELECTRIC if electric else AO_MAGNETIC
     out = {c: np.zeros(L * model.nz, dtype=np.complex128) for c in comps}
     ptrs = (C.c_void_p * 9)(*[out[c].ctypes.data if c in out else None for c in AO_COMPONENTS])
-    rc = lib().xo_green_atoo_level(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1, float(level_z),
-                                   1 if electric else 0, 1 if fft else 0, ptrs)
+    if rho_range is None:
+        rc = lib().xo_green_atoo_level(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1, float(level_z),
+                                       float(x_shift), float(y_shift), 1 if electric else 0, 1 if fft else 0, ptrs)
+    else:
+        rc = lib().xo_green_atoo_level_plan(C.byref(s), C.byref(g), omega, number_of_hankels, h0, h1,
+                                            float(level_z), float(x_shift), float(y_shift), float(rho_range[0]),
+                                            float(rho_range[1]), 1 if rho_range[2] else 0, 1 if electric else 0,
+                                            1 if fft else 0, ptrs)
     if rc:
         raise RuntimeError(f"xo_green_atoo_level failed: {rc}")
     return {c: v.reshape(L, model.nz) for c, v in out.items()}

@@ -93,14 +93,76 @@ static int cmp_rho(const void *a, const void *b)
 
 int xo_plan_create(const xo_grid *g, int number_of_hankels, xo_plan *plan)
 {
-    memset(plan, 0, sizeof(*plan));
-    int nh = abs(number_of_hankels);
-    if (nh < 1) return -1;
     /* FindRhoMin / FindRhoMax for the anomaly (ScalarPlansCreater.cs:360-380) */
     double h1 = g->dx / 2, h2 = g->dy / 2;
     double rmin = h1 < h2 ? h1 : h2;
     double mx = g->nx * g->dx - h1, my = g->ny * g->dy - h2;
     double rmax = sqrt(mx * mx + my * my);
+    return xo_plan_create_range(number_of_hankels, rmin, rmax, 0, plan);
+}
+
+/* FindRhoMin (ScalarPlansCreater.cs:245-266) along one axis of a level shifted by `shift` against
+ * the anomaly's cell centres 0, h, ..., (n - 1) h. The reference works in System.Decimal; these
+ * doubles give the same values whenever the cell size and the shift are whole multiples of a power
+ * of two fraction of a metre (fmod and the sums are then exact). */
+static double level_rho_min_axis(int n, double h, double shift)
+{
+    double a0 = -h / 2, a1 = n * h - h / 2;                  /* anomaly edges */
+    double y0 = shift - h / 2, y1 = y0 + h * n;              /* level edges */
+    if ((y0 >= a0 && y0 <= a1) || (y1 >= a0 && y1 <= a1)) {
+        double rem = fmod(fabs(a0 - y0), h);
+        if (rem == 0) return h / 2;
+        return fabs(h / 2 - rem);
+    }
+    double c0 = 0, c1 = h * (n - 1);                         /* first and last cell centres */
+    double r = fabs(y0 - c0);
+    if (fabs(y0 - c1) < r) r = fabs(y0 - c1);
+    if (fabs(y1 - c0) < r) r = fabs(y1 - c0);
+    if (fabs(y1 - c1) < r) r = fabs(y1 - c1);
+    return r;
+}
+
+/* FindRhoMaxByFourPoints (:220-236): farthest corner of the level's footprint from (x0, y0) */
+static double level_rho_max_four(const xo_grid *g, double sx, double sy, double x0, double y0)
+{
+    double x1 = sx - g->dx / 2, y1 = sy - g->dy / 2;
+    double x2 = x1 + g->dx * g->nx, y2 = y1 + g->dy * g->ny;
+    double c[4][2] = {{x1, y1}, {x2, y1}, {x2, y2}, {x1, y2}};
+    double m = 0;
+    for (int k = 0; k < 4; k++) {
+        double r = sqrt((c[k][0] - x0) * (c[k][0] - x0) + (c[k][1] - y0) * (c[k][1] - y0));
+        if (r > m) m = r;
+    }
+    return m;
+}
+
+int xo_atoo_level_rho_range(const xo_grid *g, double sx, double sy, double *rho_min, double *rho_max)
+{
+    /* FindRhoMax (:207-218): the four anomaly corner cells' centres */
+    double ex = g->dx * (g->nx - 1), ey = g->dy * (g->ny - 1);
+    double pts[4][2] = {{0, 0}, {0, ey}, {ex, ey}, {ex, 0}};
+    double m = 0;
+    for (int k = 0; k < 4; k++) {
+        double r = level_rho_max_four(g, sx, sy, pts[k][0], pts[k][1]);
+        if (r > m) m = r;
+    }
+    *rho_max = m;
+    /* FindRhoMin (:238-243) and CreateForAnomalyToObservationLevels (:51-66) */
+    double rx = level_rho_min_axis(g->nx, g->dx, sx), ry = level_rho_min_axis(g->ny, g->dy, sy);
+    double r = rx < ry ? rx : ry;
+    if (r <= 0) {
+        *rho_min = (g->dx < g->dy ? g->dx : g->dy) * 0.001;
+        return 1;
+    }
+    *rho_min = r;
+    return 0;
+}
+
+int xo_plan_create_range(int number_of_hankels, double rmin, double rmax, int zero_rho, xo_plan *plan)
+{
+    memset(plan, 0, sizeof(*plan));
+    int nh = abs(number_of_hankels);
+    if (nh < 1 || !(rmin > 0)) return -1;
     double step = log10_rho_step();
     /* AddLog10Plans (:123-140) */
     plan->nitems = nh;
@@ -129,12 +191,19 @@ int xo_plan_create(const xo_grid *g, int number_of_hankels, xo_plan *plan)
             all[3 * c] = plan->rho[i][j]; all[3 * c + 1] = i; all[3 * c + 2] = j; c++;
         }
     qsort(all, tot, 3 * sizeof(double), cmp_rho);
-    plan->rho_merged = malloc(sizeof(double) * tot);
-    plan->merge_item = malloc(sizeof(int) * tot);
-    plan->merge_index = malloc(sizeof(int) * tot);
+    plan->rho_merged = malloc(sizeof(double) * (tot + 1));
+    plan->merge_item = malloc(sizeof(int) * (tot + 1));
+    plan->merge_index = malloc(sizeof(int) * (tot + 1));
     int u = 0;
+    plan->zero_rho = zero_rho != 0;
+    if (plan->zero_rho) {     /* GreenScalarCalculator.CalculateZeroRho (:147-157): rho = 0 first */
+        plan->rho_merged[0] = 0;
+        plan->merge_item[0] = -1;
+        plan->merge_index[0] = 0;
+        u = 1;
+    }
     for (int t = 0; t < tot; t++) {
-        if (u > 0 && all[3 * t] == plan->rho_merged[u - 1]) continue;
+        if (u > plan->zero_rho && all[3 * t] == plan->rho_merged[u - 1]) continue;
         plan->rho_merged[u] = all[3 * t];
         plan->merge_item[u] = (int)all[3 * t + 1];
         plan->merge_index[u] = (int)all[3 * t + 2];
@@ -144,7 +213,7 @@ int xo_plan_create(const xo_grid *g, int number_of_hankels, xo_plan *plan)
     free(all);
     /* ScalarSegments takes its radii from ScalarPlan.GetSortedRho (all values, duplicates kept);
      * the two grids coincide only without duplicates, which is what the reference assumes. */
-    if (u != tot) {
+    if (u != tot + plan->zero_rho) {
         fprintf(stderr, "xo_plan_create: duplicate rho values (%d of %d unique)\n", u, tot);
         return -2;
     }
@@ -935,12 +1004,35 @@ static int atoo_scalars(const xo_section *sec, const xo_grid *g, double omega, c
                 if (plan->merge_item[k] == it)
                     for (int c = 0; c < ncomp; c++)
                         out[((int64_t)c * nz + tr) * nrho + k] = Ir[c * nr + plan->merge_index[k]];
+            /* CalculateZeroRhoForE/H (ScalarMathUtils.cs:33-81) on the first item's results: I1, I2
+             * (and I5) are 0 at rho = 0, I3 and I4 the value a + b rho^2 through its 2nd and 3rd
+             * radii gives there: (f[1] a^2 - f[2]) / (a^2 - 1), a the rho step. */
+            if (plan->zero_rho && it == 0) {
+                double a2 = log10_rho_step() * log10_rho_step();
+                for (int c = 0; c < ncomp; c++)
+                    out[((int64_t)c * nz + tr) * nrho + 0] =
+                        (c == 2 || c == 3) ? (Ir[c * nr + 1] * a2 - Ir[c * nr + 2]) / (a2 - 1) : 0;
+            }
             free(Ir);
         }
     }
     for (int it = 0; it < plan->nitems; it++) item_aux_free(&aux[it]);
     free(aux);
     return err ? -1 : 0;
+}
+
+int xo_atoo_level_scalars(const xo_section *sec, const xo_grid *g, double omega, int number_of_hankels,
+                          const double *hank0, const double *hank1, double zl, double rho_min, double rho_max,
+                          int zero_rho, int electric, xo_c *out, int nrho_cap, double *rho_out)
+{
+    xo_plan plan;
+    if (xo_plan_create_range(number_of_hankels, rho_min, rho_max, zero_rho, &plan)) return -1;
+    int n = plan.nrho_merged;
+    if (n > nrho_cap) { xo_plan_free(&plan); return -n; }
+    memcpy(rho_out, plan.rho_merged, sizeof(double) * n);
+    int rc = atoo_scalars(sec, g, omega, hank0, hank1, &plan, zl, electric, out);
+    xo_plan_free(&plan);
+    return rc ? -1 : n;
 }
 
 /* component order of the AtoO tensors */
@@ -1052,13 +1144,24 @@ static void ao_fill_y(actx *c, int i, int j, int left, int right, int qsel)
 }
 
 int xo_green_atoo_level(const xo_section *sec, const xo_grid *g, double omega, int number_of_hankels,
-                        const double *hank0, const double *hank1, double zl, int electric, int fft, xo_c **out)
+                        const double *hank0, const double *hank1, double zl, double sx, double sy, int electric,
+                        int fft, xo_c **out)
+{
+    /* CreateForAnomalyToObservationLevels (ScalarPlansCreater.cs:51-66) for this level alone
+     * (AtoOCalculator.cs:129, 151) */
+    double rmin, rmax;
+    int zero = xo_atoo_level_rho_range(g, sx, sy, &rmin, &rmax);
+    return xo_green_atoo_level_plan(sec, g, omega, number_of_hankels, hank0, hank1, zl, sx, sy, rmin, rmax, zero,
+                                    electric, fft, out);
+}
+
+int xo_green_atoo_level_plan(const xo_section *sec, const xo_grid *g, double omega, int number_of_hankels,
+                             const double *hank0, const double *hank1, double zl, double sx, double sy,
+                             double rho_min, double rho_max, int zero_rho, int electric, int fft, xo_c **out)
 {
     int nx = g->nx, ny = g->ny, nz = g->nz;
-    /* CreateForAnomalyToObservationLevels (ScalarPlansCreater.cs:51-66) for a level with no lateral
-     * shift: rho from min(dx, dy)/2 to the diagonal, the AtoA range (FindRhoMin/Max :180-250) */
     xo_plan plan;
-    if (xo_plan_create(g, number_of_hankels, &plan)) return -1;
+    if (xo_plan_create_range(number_of_hankels, rho_min, rho_max, zero_rho, &plan)) return -1;
     int nrho = plan.nrho_merged, nseg = nrho - 1;
     int ncomp = electric ? 5 : 4;
     xo_c *Isc = malloc(sizeof(xo_c) * (size_t)ncomp * nz * nrho);
@@ -1087,15 +1190,17 @@ int xo_green_atoo_level(const xo_section *sec, const xo_grid *g, double omega, i
     for (int t = 0; t < AO_N; t++)
         if (out[t]) memset(out[t], 0, sizeof(xo_c) * csize);
     /* knots over offsets [-Nx+1, Nx] x [-Ny+1, Ny] (PrepareKnotsAtoO, TensorPlan serial case:
-     * DistributedUtils.cs:54-61) */
+     * DistributedUtils.cs:54-61); the edge coordinates are dx (i - 1/2) - xShift with
+     * xShift = -ShiftAlongX (AtoOLevelGreenTensorCalculator.cs:119-125, TensorPlan.cs:44-54) */
+    const double xs = -sx, ys = -sy;
     int lx = -nx + 1, rx = nx, ly = -ny + 1, ry = ny;
     int kx = rx - lx + 1, ky = ry - ly + 1;
     knot *K = malloc(sizeof(knot) * kx * ky), *ZX = malloc(sizeof(knot) * ky), *ZY = malloc(sizeof(knot) * kx);
     for (int i = lx; i <= rx; i++)
         for (int j = ly; j <= ry; j++)
-            K[(i - lx) * ky + (j - ly)] = make_knot(radii, nrho, g->dx * (i - 0.5), g->dy * (j - 0.5));
-    for (int j = ly; j <= ry; j++) ZX[j - ly] = make_knot(radii, nrho, 0, g->dy * (j - 0.5));
-    for (int i = lx; i <= rx; i++) ZY[i - lx] = make_knot(radii, nrho, g->dx * (i - 0.5), 0);
+            K[(i - lx) * ky + (j - ly)] = make_knot(radii, nrho, g->dx * (i - 0.5) - xs, g->dy * (j - 0.5) - ys);
+    for (int j = ly; j <= ry; j++) ZX[j - ly] = make_knot(radii, nrho, 0, g->dy * (j - 0.5) - ys);
+    for (int i = lx; i <= rx; i++) ZY[i - lx] = make_knot(radii, nrho, g->dx * (i - 0.5) - xs, 0);
 #define KN(i, j) K[((i) - lx) * ky + ((j) - ly)]
     actx c;
     memset(&c, 0, sizeof(c));
@@ -1109,9 +1214,9 @@ int xo_green_atoo_level(const xo_section *sec, const xo_grid *g, double omega, i
     double dx = g->dx, dy = g->dy;
     /* RunAlongX (GreenTensorCalculator.cs:192-230) */
     for (int j = ly; j <= ry; j++) {
-        double y = dy * (j - 0.5);
+        double y = dy * (j - 0.5) - ys;
         for (int i = lx; i <= rx - 1; i++) {
-            double x1 = dx * (i - 0.5), x2 = dx * (i + 0.5);
+            double x1 = dx * (i - 0.5) - xs, x2 = dx * (i + 0.5) - xs;
             if (x1 > 0 && x2 > 0) {
                 ao_integrate(&c, KN(i, j), KN(i + 1, j), x1, x2, y, 0);
                 ao_fill_x(&c, i, j, ly, ry, 0);
@@ -1128,9 +1233,9 @@ int xo_green_atoo_level(const xo_section *sec, const xo_grid *g, double omega, i
     }
     /* RunAlongY (:246-281) */
     for (int i = lx; i <= rx; i++) {
-        double x = dx * (i - 0.5);
+        double x = dx * (i - 0.5) - xs;
         for (int j = ly; j <= ry - 1; j++) {
-            double y1 = dy * (j - 0.5), y2 = dy * (j + 0.5);
+            double y1 = dy * (j - 0.5) - ys, y2 = dy * (j + 0.5) - ys;
             if (y1 > 0 && y2 > 0) {
                 ao_integrate(&c, KN(i, j), KN(i, j + 1), y1, y2, x, 1);
                 ao_fill_y(&c, i, j, lx, rx, 1);

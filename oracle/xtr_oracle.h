@@ -68,12 +68,20 @@ typedef struct {
     double *rho_merged;   /* sorted unique */
     int *merge_item;      /* [nrho_merged] first-wins source item */
     int *merge_index;     /* [nrho_merged] index inside that item */
+    int zero_rho;         /* rho_merged[0] = 0, merge_item[0] = -1 (ScalarPlan.CalculateZeroRho) */
 } xo_plan;
 
 /* ScalarPlansCreater.CreateForAnomalyToAnomaly + ScalarPlan.CalculateLambdasForLog10
  * + InnerResultsCombiner.CreateFromSample (Green/Scalar/ScalarPlansCreater.cs:102-176,
  * Green/Scalar/ScalarPlan.cs:76-97, Green/Scalar/Impl/InnerResultsCombiner.cs:19-33). */
 int xo_plan_create(const xo_grid *g, int number_of_hankels, xo_plan *plan);
+/* ScalarPlansCreater.CreateFor (:102-110) for an explicit rho range; zero_rho adds rho = 0 in front
+ * of the merged radii (ScalarPlan.GetSortedRho :56-65). */
+int xo_plan_create_range(int number_of_hankels, double rho_min, double rho_max, int zero_rho, xo_plan *plan);
+/* rho range of one observation level shifted by (sx, sy) metres (ScalarPlansCreater.cs:51-66,
+ * 180-266): returns 1 when the minimum is zero (then *rho_min = 0.001 min(dx, dy) and the plan
+ * takes the rho = 0 point), else 0. */
+int xo_atoo_level_rho_range(const xo_grid *g, double sx, double sy, double *rho_min, double *rho_max);
 void xo_plan_free(xo_plan *plan);
 
 /* Green scalars of the AtoA tensor on the merged rho grid.
@@ -102,12 +110,25 @@ int xo_green_atoa(const xo_section *s, const xo_grid *g, double omega, int numbe
                   const double *hank0, const double *hank1, int fft,
                   xo_c *xx, xo_c *xy, xo_c *xz, xo_c *yy, xo_c *yz, xo_c *zz);
 
-/* AtoO level tensors (P7): anomaly layers to the observation level at depth zl (no lateral shift).
- * electric = 1: components xx yy zz xy - xz yz zx zy; electric = 0 (magnetic): xx - - xy yx xz yz
- * zx zy (AtoOLevelGreenTensorCalculator.cs:21-75). out[9] in the order xx yy zz xy yx xz yz zx zy,
- * each [2Nx][2Ny][nz] (NULL for an absent component); fft = 0 stops before the FFT. */
+/* AtoO level tensors (P7): anomaly layers to the observation level at depth zl whose sites sit at
+ * the cell centres shifted by (sx, sy) metres, with the level's own scalar plan
+ * (AtoOCalculator.cs:124-160). electric = 1: components xx yy zz xy - xz yz zx zy; electric = 0
+ * (magnetic): xx - - xy yx xz yz zx zy (AtoOLevelGreenTensorCalculator.cs:21-75). out[9] in the
+ * order xx yy zz xy yx xz yz zx zy, each [2Nx][2Ny][nz] (NULL for an absent component); fft = 0
+ * stops before the FFT. */
 int xo_green_atoo_level(const xo_section *s, const xo_grid *g, double omega, int number_of_hankels,
-                        const double *hank0, const double *hank1, double zl, int electric, int fft, xo_c **out);
+                        const double *hank0, const double *hank1, double zl, double sx, double sy, int electric,
+                        int fft, xo_c **out);
+/* the level's scalars I1..I5 (electric) or I1..I4 (magnetic) of every anomaly layer on the merged
+ * rho grid of the plan (rho_min, rho_max, zero_rho): out[comp][layer][rho], rho_out[rho]; returns the
+ * number of radii (or minus the number needed when it exceeds nrho_cap, -1 on error). */
+int xo_atoo_level_scalars(const xo_section *s, const xo_grid *g, double omega, int number_of_hankels,
+                          const double *hank0, const double *hank1, double zl, double rho_min, double rho_max,
+                          int zero_rho, int electric, xo_c *out, int nrho_cap, double *rho_out);
+/* the same with the scalar plan's rho range given (rho_min, rho_max, zero_rho) */
+int xo_green_atoo_level_plan(const xo_section *s, const xo_grid *g, double omega, int number_of_hankels,
+                             const double *hank0, const double *hank1, double zl, double sx, double sy,
+                             double rho_min, double rho_max, int zero_rho, int electric, int fft, xo_c **out);
 
 /* Anomalous field at the level from jQ [Nx][Ny][3][Nz] (ToOCalculator.CalculateAnomalyField,
  * ToOCalculator.cs:21-45): field [Nx][Ny][3]. */

@@ -47,6 +47,23 @@ def test_level_tensor_matches_oracle(oracle, torch_cuda, which, electric):
                   f"{which} {'E' if electric else 'H'}")
 
 
+@pytest.mark.parametrize("shift", [(2.0, -1.0), (0.3, 0.0), (0.5, 0.25), (-13.0, 9.5)],
+                         ids=["whole-cells", "fraction", "cell-edges-zero-rho", "outside"])
+def test_shifted_level_tensor_matches_oracle(oracle, torch_cuda, shift):
+    """Levels whose sites are the cell centres shifted by (sx dx, sy dy): whole cells, a fraction of
+    a cell, receivers on cell edges (the zero-rho plan), and a level beside the anomaly."""
+    m = cm.small_model(6, 4, 3, seed=2)
+    w = cm.omega_of(m.frequencies[0])
+    sx, sy = shift[0] * m.dx, shift[1] * m.dy
+    print(f"shift {sx} {sy}: plan {oracle.atoo_level_rho_range(m, sx, sy)}", flush=True)
+    op = X.ConvolutionOperator(m, w)
+    for electric in (True, False):
+        op.calculate_green_atoo(0.0, electric, x_shift=sx, y_shift=sy)
+        _check_tensor(op.green_atoo(electric),
+                      oracle.green_atoo_level(m, w, 0.0, electric, x_shift=sx, y_shift=sy),
+                      f"shift {shift} {'E' if electric else 'H'}")
+
+
 def test_level_tensor_inside_the_anomaly_matches_oracle(oracle, torch_cuda):
     """A level inside an anomaly layer takes the receiver-inside-transmitter branches."""
     m = _three_layer(4, 4, 4)

@@ -60,3 +60,42 @@ def test_commemi_end_to_end(torch_cuda, tmp_path):
     # the fields table carries the blocks' anomalous fields (the last 24 columns)
     names_f, fields = P.read_result_table(files[0])
     assert fields.shape == (1600, 52) and np.abs(fields[:, 28:]).max() > 0
+
+
+def test_commemi_reference_project(torch_cuda, tmp_path):
+    """The example's own project: four tablets shifted by (+-20 km, +-20 km), three frequencies.
+    Where tablets overlap they observe the same sites with the same scalar plan, so their responses
+    agree to the written digits; tablets (x, -20 km) and (x, +20 km) are y-mirror images."""
+    logs = []
+    files = P.run_project(os.path.join(HERE, "golden", "commemi3d2", "forward.xproj"),
+                          results_path=str(tmp_path), log=logs.append)
+    print(logs, flush=True)
+    assert len(files) == 6
+    proj = P.read_project(os.path.join(HERE, "golden", "commemi3d2", "forward.xproj"))
+    shifts = [(l.x_shift, l.y_shift) for l in proj.levels]
+    for f in files[1::2]:
+        _, rows = P.read_result_table(f)
+        assert rows.shape == (4 * 1600, 28) and np.all(np.isfinite(rows))
+        tab = {s: rows[k * 1600:(k + 1) * 1600].reshape(40, 40, 28) for k, s in enumerate(shifts)}
+        a, b = tab[(-20000.0, -20000.0)], tab[(20000.0, -20000.0)]
+        assert np.array_equal(a[20:, :, 0:3], b[:20, :, 0:3])              # same site coordinates
+        za, zb = a[20:, :, 4:12], b[:20, :, 4:12]
+        overlap = np.abs(za - zb).max() / np.abs(za).max()
+        c, d = tab[(-20000.0, -20000.0)], tab[(-20000.0, 20000.0)]
+        z1 = c[..., 4:12:2] + 1j * c[..., 5:12:2]
+        z2 = (d[..., 4:12:2] + 1j * d[..., 5:12:2])[:, ::-1]
+        scale = np.abs(z1[..., 1]).max()
+        even = max(np.abs(z1[..., k] - z2[..., k]).max() for k in (1, 2)) / scale
+        odd = max(np.abs(z1[..., k] + z2[..., k]).max() for k in (0, 3)) / scale
+        print(f"{os.path.basename(f)}: overlap {overlap:.2e}, y-mirror even {even:.2e} odd {odd:.2e}", flush=True)
+        assert overlap < 1e-5 and even < 1e-5 and odd < 1e-5
+    # 0.1 Hz, the far corner of tablet (-20 km, -20 km) (>= 39 km from the blocks): the 1-D earth
+    _, rows = P.read_result_table(files[1])
+    m = P.read_model(MODEL)
+    w = cm.omega_of(0.1)
+    (e1, h1), _ = F.level_normal_fields(m.section, w, 0.0)
+    rho1d = abs(e1[0] / h1[1]) ** 2 / (w * cm.MU0)
+    corner = rows[0, 12:16]
+    dev = max(abs(corner[1] / rho1d - 1), abs(corner[2] / rho1d - 1))
+    print(f"far corner rho_xy, rho_yx vs 1-D: {dev:.3%}", flush=True)
+    assert dev < 5e-3

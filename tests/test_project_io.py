@@ -141,6 +141,10 @@ def test_result_tables(tmp_path):
         assert abs(wzx * hh[0, 1, 0] + wzy * hh[0, 1, 1] - hh[0, 1, 2]) < 1e-5 * abs(hh[0, 1]).max()
 
 
-def test_run_project_refuses_shifted_levels():
+def test_run_project_refuses_sites(tmp_path):
+    f = tmp_path / "p.xproj"
+    f.write_text('<?xml version="1.0"?><ExtremeProject><ModelFile>m.xmodel</ModelFile>'
+                 '<Frequencies><F>1</F></Frequencies><Observations><Sites><S x="1" y="2" z="0"/></Sites>'
+                 '</Observations></ExtremeProject>')
     with pytest.raises(NotImplementedError):
-        P.run_project(os.path.join(COMMEMI, "forward.xproj"))
+        P.run_project(str(f))
