@@ -66,8 +66,10 @@ void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, d
     XTR_LAUNCH_CHECK("block_dots_kernel");
 }
 
-// out[k] = sum_b partial[k][b], one workgroup per k, fixed order
-__global__ __launch_bounds__(kThreads) void reduce_c_kernel(const double2 *partial, double2 *out)
+// out[k] = sum_b partial[k][b], one workgroup per k, fixed order; also into host_out[k] (pinned,
+// device-mapped host memory) when given, which the host reads once the stream has passed this kernel
+__global__ __launch_bounds__(kThreads) void reduce_c_kernel(const double2 *partial, double2 *out,
+                                                            double2 *host_out)
 {
     __shared__ double sh[kThreads / 64];
     const int k = blockIdx.x;
@@ -78,12 +80,15 @@ __global__ __launch_bounds__(kThreads) void reduce_c_kernel(const double2 *parti
     }
     re = block_sum(re, sh);
     im = block_sum(im, sh);
-    if (threadIdx.x == 0) out[k] = make_double2(re, im);
+    if (threadIdx.x == 0) {
+        out[k] = make_double2(re, im);
+        if (host_out) host_out[k] = make_double2(re, im);
+    }
 }
 
-void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s)
+void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s, double2 *host_out)
 {
-    hipLaunchKernelGGL(reduce_c_kernel, dim3(cnt), dim3(kThreads), 0, s, partial, out);
+    hipLaunchKernelGGL(reduce_c_kernel, dim3(cnt), dim3(kThreads), 0, s, partial, out, host_out);
     XTR_LAUNCH_CHECK("reduce_c_kernel");
 }
 
@@ -136,18 +141,21 @@ void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s)
     XTR_LAUNCH_CHECK("norm2_kernel");
 }
 
-__global__ __launch_bounds__(kThreads) void reduce_r_kernel(const double *partial, double *out)
+__global__ __launch_bounds__(kThreads) void reduce_r_kernel(const double *partial, double *out, double *host_out)
 {
     __shared__ double sh[kThreads / 64];
     double v = 0;
     for (int b = threadIdx.x; b < kRedBlocks; b += kThreads) v += partial[b];
     v = block_sum(v, sh);
-    if (threadIdx.x == 0) *out = v;
+    if (threadIdx.x == 0) {
+        *out = v;
+        if (host_out) *host_out = v;
+    }
 }
 
-void launch_reduce_r(const double *partial, double *out, hipStream_t s)
+void launch_reduce_r(const double *partial, double *out, hipStream_t s, double *host_out)
 {
-    hipLaunchKernelGGL(reduce_r_kernel, dim3(1), dim3(kThreads), 0, s, partial, out);
+    hipLaunchKernelGGL(reduce_r_kernel, dim3(1), dim3(kThreads), 0, s, partial, out, host_out);
     XTR_LAUNCH_CHECK("reduce_r_kernel");
 }
 

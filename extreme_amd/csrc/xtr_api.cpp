@@ -83,17 +83,35 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
 };
 
+// Pinned host memory that kernels write directly (mapped into the device address space,
+// fine-grained coherent): small results the host reads after the stream has passed the writer.
 template <typename T>
-struct PinBuf {
-    T *p = nullptr;
-    explicit PinBuf(size_t n) { XTR_CHECK(hipHostMalloc((void **)&p, n * sizeof(T), hipHostMallocDefault)); }
-    ~PinBuf()
+struct MappedBuf {
+    T *p = nullptr;     // host address
+    T *dev = nullptr;   // device address of the same memory
+    explicit MappedBuf(size_t n)
+    {
+        XTR_CHECK(hipHostMalloc((void **)&p, n * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+        XTR_CHECK(hipHostGetDevicePointer((void **)&dev, p, 0));
+    }
+    ~MappedBuf()
     {
         if (p) (void)hipHostFree(p);
     }
-    PinBuf(const PinBuf &) = delete;
-    PinBuf &operator=(const PinBuf &) = delete;
+    MappedBuf(const MappedBuf &) = delete;
+    MappedBuf &operator=(const MappedBuf &) = delete;
 };
+
+// Waits for everything submitted to `s` so far by polling an event: the host resumes as soon as
+// the GPU passes the marker instead of being woken from a blocking wait.
+inline void spin_wait(hipStream_t s, hipEvent_t ev)
+{
+    XTR_CHECK(hipEventRecord(ev, s));
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    XTR_CHECK(e);
+}
 
 }  // namespace
 
@@ -258,10 +276,22 @@ struct SolveCtx {
     // reduction scratch shared by the outer and inner solvers of one solve
     DevBuf<double2> partial_c;   // [32][kRedBlocks]
     DevBuf<double> partial_r;    // [kRedBlocks]
-    DevBuf<double2> h_dev;       // [32] projections, then [32] norm slot
+    DevBuf<double2> h_dev;       // [32] projections
     DevBuf<double> norm_dev;     // [1]
-    PinBuf<double2> h_host{33};
-    SolveCtx() : partial_c(32 * xtr::kRedBlocks), partial_r(xtr::kRedBlocks), h_dev(32), norm_dev(1) {}
+    MappedBuf<double2> h_host{32};   // the projections, written by the reduction kernel
+    MappedBuf<double> nn_host{1};    // the squared norm, likewise
+    hipEvent_t ev = nullptr;
+    SolveCtx() : partial_c(32 * xtr::kRedBlocks), partial_r(xtr::kRedBlocks), h_dev(32), norm_dev(1)
+    {
+        XTR_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    ~SolveCtx()
+    {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    SolveCtx(const SolveCtx &) = delete;
+    SolveCtx &operator=(const SolveCtx &) = delete;
+    void wait() { spin_wait(stream, ev); }
 };
 
 struct Gmres {
@@ -299,14 +329,10 @@ struct Gmres {
     double norm(const double2 *v)
     {
         xtr::launch_norm2(v, n, ctx->partial_r.p, ctx->stream);
-        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream);
-        double nn = 0;
-        XTR_CHECK(hipMemcpyAsync(&ctx->h_host.p[32], ctx->norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        XTR_CHECK(hipStreamSynchronize(ctx->stream));
-        std::memcpy(&nn, &ctx->h_host.p[32], sizeof(double));
+        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream, ctx->nn_host.dev);
+        ctx->wait();
         ctx->dots++;
-        return std::sqrt(nn);
+        return std::sqrt(ctx->nn_host.p[0]);
     }
 
     // ClassicalGramSchmidtIteration (Gmres.cs:302-313) fused with the norm of the new w.
@@ -314,14 +340,10 @@ struct Gmres {
     {
         const int cnt = jH + 1;
         xtr::launch_block_dots(V.p, cnt, w.p, n, ctx->partial_c.p, ctx->stream);
-        xtr::launch_reduce_c(ctx->partial_c.p, cnt, ctx->h_dev.p, ctx->stream);
+        xtr::launch_reduce_c(ctx->partial_c.p, cnt, ctx->h_dev.p, ctx->stream, ctx->h_host.dev);
         xtr::launch_cgs_update(V.p, cnt, ctx->h_dev.p, w.p, n, ctx->partial_r.p, ctx->stream);
-        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream);
-        XTR_CHECK(hipMemcpyAsync(ctx->h_host.p, ctx->h_dev.p, cnt * sizeof(double2), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        XTR_CHECK(hipMemcpyAsync(&ctx->h_host.p[32], ctx->norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        XTR_CHECK(hipStreamSynchronize(ctx->stream));
+        xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream, ctx->nn_host.dev);
+        ctx->wait();
         double s = 0;
         for (int k = 0; k < cnt; ++k) {
             const cd d(ctx->h_host.p[k].x, ctx->h_host.p[k].y);
@@ -329,9 +351,7 @@ struct Gmres {
             s += d.real() * d.real() + d.imag() * d.imag();
         }
         dloo = s;                                       // Dznrm2^2
-        double nn;
-        std::memcpy(&nn, &ctx->h_host.p[32], sizeof(double));
-        dnormw = std::sqrt(nn);
+        dnormw = std::sqrt(ctx->nn_host.p[0]);
         ctx->dots += cnt + 1;
     }
 
@@ -1112,13 +1132,9 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
         }
         auto norm = [&](const double2 *v) {
             xtr::launch_norm2(v, n, ctx.partial_r.p, ctx.stream);
-            xtr::launch_reduce_r(ctx.partial_r.p, ctx.norm_dev.p, ctx.stream);
-            XTR_CHECK(hipMemcpyAsync(&ctx.h_host.p[32], ctx.norm_dev.p, sizeof(double), hipMemcpyDeviceToHost,
-                                     ctx.stream));
-            XTR_CHECK(hipStreamSynchronize(ctx.stream));
-            double nn = 0;
-            std::memcpy(&nn, &ctx.h_host.p[32], sizeof(double));
-            return std::sqrt(nn);
+            xtr::launch_reduce_r(ctx.partial_r.p, ctx.norm_dev.p, ctx.stream, ctx.nn_host.dev);
+            ctx.wait();
+            return std::sqrt(ctx.nn_host.p[0]);
         };
         std::vector<double> bn(nrhs), rn(nrhs, 0.0);
         std::vector<int> live;

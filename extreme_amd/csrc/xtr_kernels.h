@@ -143,13 +143,13 @@ constexpr int kRedBlocks = 512;   // partial-sum blocks of the reductions
 // partial[k][blk] of conj(V_k) . w, k < cnt (V columns contiguous, stride n)
 void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s);
 // out[k] = sum_blk partial[k][blk]
-void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s);
+void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s, double2 *host_out = nullptr);
 // w -= sum_k V_k h_k (h on device), partial norms of the new w
 void launch_cgs_update(const double2 *V, int cnt, const double2 *h, double2 *w, int64_t n, double *partial,
                        hipStream_t s);
 // partial sums of |w|^2 ; out = sum of partials
 void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s);
-void launch_reduce_r(const double *partial, double *out, hipStream_t s);
+void launch_reduce_r(const double *partial, double *out, hipStream_t s, double *host_out = nullptr);
 // dst = src / d  (v_{j+1} = w / |w|, Gmres.cs:267-268)
 void launch_div(const double2 *src, double d, double2 *dst, int64_t n, hipStream_t s);
 // dst = a * src  (V_0 = (1/beta) w, Gmres.cs:117)
