@@ -1092,69 +1092,99 @@ __device__ __forceinline__ void fold32_issue(const Fold32Copy<T> &cp, int wave, 
 #pragma clang diagnostic pop
 
 // A element (row 16 TILE + i, column 4 STEP + kl) of the 96 x 96 block matrix
-// [xx xy xz; xy yy yz; -xz^T -yz^T zz].
-template <typename T, int TILE, int STEP>
-__device__ __forceinline__ cplx<T> fold32_a(const cplx<T> *__restrict__ blk, int i, int kl)
+// [xx xy xz; xy yy yz; -xz^T -yz^T zz]: its index in the folded block (complex elements), and whether
+// the stored value enters negated (the -xz^T, -yz^T part).
+template <int TILE, int STEP>
+__device__ __forceinline__ int fold32_a_idx(int i, int kl)
 {
     using namespace fold32;
     constexpr int o = TILE / 2, ci = STEP / 8;
     const int r = 16 * (TILE % 2) + i, c = 4 * (STEP % 8) + kl;
     if constexpr (o == ci || (o < 2 && ci < 2)) {
         constexpr int base = o != ci ? S : o == 0 ? 0 : o == 1 ? 2 * S : 3 * S;   // xy | xx, yy, zz
-        const int p = c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2;
-        return blk[base + p];
+        return base + (c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2);
     } else if constexpr (ci == 2) {
         constexpr int base = o == 0 ? 4 * S : 4 * S + A;                       // xz, yz (r, c)
-        return blk[base + r * NZ + (c ^ (r & MASK))];
+        return base + r * NZ + (c ^ (r & MASK));
     } else {
         constexpr int base = ci == 0 ? 4 * S : 4 * S + A;                      // -xz^T, -yz^T
-        const cplx<T> v = blk[base + c * NZ + (r ^ (c & MASK))];
-        return mk<T>(-v.x, -v.y);
+        return base + c * NZ + (r ^ (c & MASK));
     }
 }
+template <int TILE, int STEP>
+constexpr bool fold32_a_neg = TILE / 2 == 2 && STEP / 8 < 2;
 
-// Part PART of A element (row 16 TILE + i, column 4 STEP + kl).
-template <typename T, int TILE, int STEP, int PART>
-__device__ __forceinline__ T fold32_a_part(const cplx<T> *__restrict__ blk, int i, int kl)
+// D += (NEG ? -a : a) b. The FP64 instruction negates its A operand itself (the neg modifier);
+// the FP32 one has no such modifier, the sign goes on the value.
+template <bool NEG>
+__device__ __forceinline__ mfma_d4 mfma16s(double a, double b, mfma_d4 c)
 {
-    const cplx<T> v = fold32_a<T, TILE, STEP>(blk, i, kl);
-    return PART ? v.y : v.x;
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, NEG ? 1 : 0);
+}
+template <bool NEG>
+__device__ __forceinline__ mfma_f4 mfma16s(float a, float b, mfma_f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(NEG ? -a : a, b, c, 0, 0, 0);
+}
+
+// Per-lane byte offsets, within a slot, of the A values a wave reads at each k-step: the whole
+// complex element of tile TF and part PH of tile TH. They depend only on the lane, so a wave
+// computes them once and keeps them in registers for all its blocks.
+template <typename T, int TF, int TH, int PH, int... Ss>
+__device__ __forceinline__ void fold32_offsets(std::integer_sequence<int, Ss...>, int i, int kl,
+                                               unsigned (&of)[fold32::KSTEPS], unsigned (&oh)[fold32::KSTEPS])
+{
+    constexpr unsigned E = sizeof(cplx<T>);
+    ((of[Ss] = E * (unsigned)fold32_a_idx<TF, Ss>(i, kl)), ...);
+    ((oh[Ss] = E * (unsigned)fold32_a_idx<TH, Ss>(i, kl) + (unsigned)(PH * sizeof(T))), ...);
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> lds_cplx(const cplx<T> *blk, unsigned off)
+{
+    return *reinterpret_cast<const cplx<T> *>(reinterpret_cast<const char *>(blk) + off);
+}
+template <typename T>
+__device__ __forceinline__ T lds_part(const cplx<T> *blk, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(blk) + off);
 }
 
 // One k-step: the A values of the next step are read before this step's MFMAs, so the LDS latency
 // is covered by three MFMAs; two wave-loads of the next block's copy go out over the first steps.
 template <typename T, int TF, int TH, int PH, int STEP>
-__device__ __forceinline__ void fold32_step(const cplx<T> *__restrict__ blk, const T (&bf)[fold32::KSTEPS], int i,
-                                            int kl, typename MfmaAcc<T>::type &f1, typename MfmaAcc<T>::type &f2,
-                                            typename MfmaAcc<T>::type &hh, cplx<T> &gf, T &gh,
-                                            const Fold32Copy<T> &cp, int wave, int lane)
+__device__ __forceinline__ void fold32_step(const cplx<T> *__restrict__ blk, const T (&bf)[fold32::KSTEPS],
+                                            const unsigned (&of)[fold32::KSTEPS],
+                                            const unsigned (&oh)[fold32::KSTEPS], typename MfmaAcc<T>::type &f1,
+                                            typename MfmaAcc<T>::type &f2, typename MfmaAcc<T>::type &hh,
+                                            cplx<T> &gf, T &gh, const Fold32Copy<T> &cp, int wave, int lane)
 {
     cplx<T> nf = gf;
     T nh = gh;
     if constexpr (STEP + 1 < fold32::KSTEPS) {
-        nf = fold32_a<T, TF, STEP + 1>(blk, i, kl);
-        nh = fold32_a_part<T, TH, STEP + 1, PH>(blk, i, kl);
+        nf = lds_cplx<T>(blk, of[STEP + 1]);
+        nh = lds_part<T>(blk, oh[STEP + 1]);
     }
     if constexpr (2 * STEP < fold32::Cfg<T>::PW_HI) fold32_issue_one<T>(cp, 2 * STEP, wave, lane);
     if constexpr (2 * STEP + 1 < fold32::Cfg<T>::PW_HI) fold32_issue_one<T>(cp, 2 * STEP + 1, wave, lane);
-    f1 = mfma16(gf.x, bf[STEP], f1);
-    f2 = mfma16(gf.y, bf[STEP], f2);
-    hh = mfma16(gh, bf[STEP], hh);
+    f1 = mfma16s<fold32_a_neg<TF, STEP>>(gf.x, bf[STEP], f1);
+    f2 = mfma16s<fold32_a_neg<TF, STEP>>(gf.y, bf[STEP], f2);
+    hh = mfma16s<fold32_a_neg<TH, STEP>>(gh, bf[STEP], hh);
     gf = nf;
     gh = nh;
 }
 
-// Wave's three accumulators: tile TF (Re G and Im G parts) and part PH of tile TH.
+// Wave's three accumulators over one block: tile TF (Re G and Im G parts) and part PH of tile TH.
 template <typename T, int TF, int TH, int PH, int... Ss>
 __device__ __forceinline__ void fold32_wave(std::integer_sequence<int, Ss...>, const cplx<T> *__restrict__ blk,
-                                            const T (&bf)[fold32::KSTEPS], int lane,
-                                            typename MfmaAcc<T>::type &f1, typename MfmaAcc<T>::type &f2,
-                                            typename MfmaAcc<T>::type &hh, const Fold32Copy<T> &cp, int wave)
+                                            const T (&bf)[fold32::KSTEPS], const unsigned (&of)[fold32::KSTEPS],
+                                            const unsigned (&oh)[fold32::KSTEPS], typename MfmaAcc<T>::type &f1,
+                                            typename MfmaAcc<T>::type &f2, typename MfmaAcc<T>::type &hh,
+                                            const Fold32Copy<T> &cp, int wave, int lane)
 {
-    const int i = lane & 15, kl = lane >> 4;
-    cplx<T> gf = fold32_a<T, TF, 0>(blk, i, kl);
-    T gh = fold32_a_part<T, TH, 0, PH>(blk, i, kl);
-    (fold32_step<T, TF, TH, PH, Ss>(blk, bf, i, kl, f1, f2, hh, gf, gh, cp, wave, lane), ...);
+    cplx<T> gf = lds_cplx<T>(blk, of[0]);
+    T gh = lds_part<T>(blk, oh[0]);
+    (fold32_step<T, TF, TH, PH, Ss>(blk, bf, of, oh, f1, f2, hh, gf, gh, cp, wave, lane), ...);
 }
 
 // Y of tile TILE from D1 (Re G . B) and D2 (Im G . B), stored with the image signs.
@@ -1179,19 +1209,21 @@ __device__ __forceinline__ void fold32_store(typename MfmaAcc<T>::type d1, typen
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
-                                                                 int64_t Lq, int64_t M, int nrhs, int nx, int ny)
+// The block loop of wave `wave` (tiles TF, TH): every wave runs the same loop and barriers.
+template <typename T, int TF, int TH, int PH>
+__device__ __forceinline__ void fold32_run(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t Lq, int64_t M,
+                                           int nrhs, int nx, int ny, cplx<T> *ring, unsigned base, int wave,
+                                           int lane)
 {
     using namespace fold32;
     using K = Cfg<T>;
     using acc_t = typename MfmaAcc<T>::type;
-    __shared__ __attribute__((aligned(32))) cplx<T> ring[K::NBUF * K::SLOT];   // 32: exchange vectors
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
     const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= Lq
     const int64_t b0 = blockIdx.x;
     const int64_t cnt = (Lq - 1 - b0) / g + 1;
+    using Seq = std::make_integer_sequence<int, KSTEPS>;
+    unsigned of[KSTEPS], oh[KSTEPS];
+    fold32_offsets<T, TF, TH, PH>(Seq{}, lane & 15, lane >> 4, of, oh);
     Fold32Copy<T> cp{G, buf, M, 0, FoldImages{}, nrhs, 0u, false};
     if (K::NBUF == 2) {                                          // block 0 now, block it + 1 during block it
         cp.b = b0;
@@ -1238,13 +1270,7 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
             }
         }
         acc_t f1 = {0, 0, 0, 0}, f2 = f1, hh = f1;
-        using Seq = std::make_integer_sequence<int, KSTEPS>;
-        switch (wave) {
-        case 0: fold32_wave<T, 0, 1, 0>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
-        case 1: fold32_wave<T, 2, 1, 1>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
-        case 2: fold32_wave<T, 3, 4, 0>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
-        default: fold32_wave<T, 5, 4, 1>(Seq{}, blk, bf, lane, f1, f2, hh, cp, wave); break;
-        }
+        fold32_wave<T, TF, TH, PH>(Seq{}, blk, bf, of, oh, f1, f2, hh, cp, wave, lane);
         // tiles 1 and 4: the Im G part (waves 1, 3) joins the Re G part (waves 0, 2), through the slot
         acc_t (*exch)[64] = reinterpret_cast<acc_t (*)[64]>(blk);
         XTR_BARRIER();                                           // every wave has read the slot
@@ -1252,21 +1278,27 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
         XTR_BARRIER();
         acc_t h2 = f1;
         if (!(wave & 1)) h2 = exch[wave >> 1][lane];
-        switch (wave) {
-        case 0:
-            fold32_store<T, 0>(f1, f2, buf, M, im, nrhs, lane);
-            fold32_store<T, 1>(hh, h2, buf, M, im, nrhs, lane);
-            break;
-        case 1: fold32_store<T, 2>(f1, f2, buf, M, im, nrhs, lane); break;
-        case 2:
-            fold32_store<T, 3>(f1, f2, buf, M, im, nrhs, lane);
-            fold32_store<T, 4>(hh, h2, buf, M, im, nrhs, lane);
-            break;
-        default: fold32_store<T, 5>(f1, f2, buf, M, im, nrhs, lane); break;
-        }
+        fold32_store<T, TF>(f1, f2, buf, M, im, nrhs, lane);
+        if constexpr (PH == 0) fold32_store<T, TH>(hh, h2, buf, M, im, nrhs, lane);
         XTR_BARRIER();                                           // slot free (exchange read)
     }
     wait_vm<0>();                                                // no copy may land after exit
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
+                                                                 int64_t Lq, int64_t M, int nrhs, int nx, int ny)
+{
+    using K = fold32::Cfg<T>;
+    __shared__ __attribute__((aligned(32))) cplx<T> ring[K::NBUF * K::SLOT];   // 32: exchange vectors
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
+    switch (wave) {
+    case 0: fold32_run<T, 0, 1, 0>(G, buf, Lq, M, nrhs, nx, ny, ring, base, wave, lane); break;
+    case 1: fold32_run<T, 2, 1, 1>(G, buf, Lq, M, nrhs, nx, ny, ring, base, wave, lane); break;
+    case 2: fold32_run<T, 3, 4, 0>(G, buf, Lq, M, nrhs, nx, ny, ring, base, wave, lane); break;
+    default: fold32_run<T, 5, 4, 1>(G, buf, Lq, M, nrhs, nx, ny, ring, base, wave, lane); break;
+    }
 }
 
 // ---- folded contraction on MFMA, nz = 64 -------------------------------------------------------------
