@@ -195,6 +195,7 @@ def main():
         "apply_gbs": apply_gbs,
         "apply_frac_of_hbm_peak": apply_gbs / HBM_PEAK_GBS,
         "stage_ms_per_apply": {k: v / max(rounds, 1) for k, v in stage_ms.items()},
+        "fft_tiles": problems[0].op.fft_tiles(),
         "applies": rounds,
         "rhs_applies": rhs_applies,
         "iterations": iters,
@@ -338,7 +339,8 @@ def large_config(model, args):
                "fp64_iterations": [i.number_of_iterations for i in infos],
                "fp64_matvecs": [i.matvecs for i in infos],
                "fp64_true_residual": [i.backward_error for i in infos],
-               "apply_rounds": rounds, "stage_ms_per_apply": {k: v / max(rounds, 1) for k, v in st.items()}}
+               "apply_rounds": rounds, "stage_ms_per_apply": {k: v / max(rounds, 1) for k, v in st.items()},
+               "fft_tiles": p.op.fft_tiles()}
         if not args.no_mixed:
             torch.cuda.synchronize()
             t0 = time.perf_counter()

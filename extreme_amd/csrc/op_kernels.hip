@@ -242,22 +242,42 @@ __global__ __launch_bounds__(kFftThreads) void pencil_fft_kernel(IO io, const cp
     }
 }
 
-// Column tile per pencil length: the LDS tile stays at or below 32 KB in FP64.
+// Column tiles per pencil length. The default tile has 16 columns while P x 16 stays within 32 KB,
+// else 8. For lengths from 64 there are two more: the narrow tile (half the columns, more workgroups
+// per CU) and the wide one (twice the columns, longer contiguous runs per row, within 128 KB). Which
+// one a pass uses is measured on the device (the operator's tile tuning); the choice does not change
+// the arithmetic, only how the columns are grouped.
 template <typename T, int P>
 constexpr int tile_cols()
 {
     return (P * 16 * (int)sizeof(cplx<T>) <= 32768) ? 16 : 8;
 }
+template <typename T, int P>
+constexpr int tile_cols_of(int tile)
+{
+    constexpr int d = tile_cols<T, P>();
+    if (P < 64) return d;
+    if (tile == FFT_TILE_WIDE) return P * 2 * d * (int)sizeof(cplx<T>) <= 131072 ? 2 * d : d;
+    if (tile == FFT_TILE_NARROW) return d / 2 >= 4 ? d / 2 : d;
+    return d;
+}
 
 template <typename T, int DIR, class IO>
-void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw, hipStream_t s)
+void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw, hipStream_t s, int tile)
 {
     auto go = [&](auto pc) {
         constexpr int PP = decltype(pc)::value;
-        constexpr int TC = tile_cols<T, PP>();
-        dim3 grid((unsigned)ceil_div(ncols, TC), nrhs);
-        hipLaunchKernelGGL((pencil_fft_kernel<T, PP, TC, DIR, IO>), grid, dim3(kFftThreads), 0, s, io,
-                           (const cplx<T> *)tw);
+        constexpr int TD = tile_cols_of<T, PP>(FFT_TILE_DEFAULT), TW = tile_cols_of<T, PP>(FFT_TILE_WIDE),
+                      TN = tile_cols_of<T, PP>(FFT_TILE_NARROW);
+        auto run = [&](auto tc) {
+            constexpr int TC = decltype(tc)::value;
+            dim3 grid((unsigned)ceil_div(ncols, TC), nrhs);
+            hipLaunchKernelGGL((pencil_fft_kernel<T, PP, TC, DIR, IO>), grid, dim3(kFftThreads), 0, s, io,
+                               (const cplx<T> *)tw);
+        };
+        if (tile == FFT_TILE_WIDE && TW != TD) run(std::integral_constant<int, TW>{});
+        else if (tile == FFT_TILE_NARROW && TN != TD) run(std::integral_constant<int, TN>{});
+        else run(std::integral_constant<int, TD>{});
     };
     switch (P) {
     case 4: go(std::integral_constant<int, 4>{}); break;
@@ -298,6 +318,14 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
 }
 
 // even lengths 2^a m, m in {1, 3, 5, 15, 25, 75}, up to 512 (Nx, Ny <= 256)
+int fft_tile_options(int p, int precision)
+{
+    if (p < 64) return 1;
+    const int e = precision ? 8 : 16;
+    const int d = (p * 16 * e <= 32768) ? 16 : 8;
+    return 2 + (p * 2 * d * e <= 131072 ? 1 : 0);   // default, narrow (d / 2 >= 4 always), wide if it fits
+}
+
 bool fft_length_supported(int p)
 {
     switch (p) {
@@ -354,7 +382,7 @@ template <typename T>
 static void fwd_y(const ApplyArgs &a, hipStream_t s)
 {
     IoFwdY<T> io{with_r(a), a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
-    launch_pencil<T, -1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
+    launch_pencil<T, -1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s, a.fft_tile[0]);
 }
 
 template <typename T>
@@ -362,7 +390,7 @@ static void fwd_x(const ApplyArgs &a, hipStream_t s)
 {
     const int64_t ncols = 2LL * a.g.ny * a.g.C();
     IoPlain<T> io{(cplx<T> *)a.buf, a.g.M(), ncols, ncols, 0, ncols, a.g.nx, 2 * a.g.nx};
-    launch_pencil<T, -1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s);
+    launch_pencil<T, -1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s, a.fft_tile[1]);
 }
 
 template <typename T>
@@ -370,14 +398,14 @@ static void inv_x(const ApplyArgs &a, hipStream_t s)
 {
     const int64_t ncols = 2LL * a.g.ny * a.g.C();
     IoPlain<T> io{(cplx<T> *)a.buf, a.g.M(), ncols, ncols, 0, ncols, 2 * a.g.nx, a.g.nx};
-    launch_pencil<T, +1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s);
+    launch_pencil<T, +1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s, a.fft_tile[2]);
 }
 
 template <typename T>
 static void inv_y(const ApplyArgs &a, hipStream_t s)
 {
     IoInvY<T> io{with_r(a), a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
-    launch_pencil<T, +1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s);
+    launch_pencil<T, +1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s, a.fft_tile[3]);
 }
 
 void launch_fwd_y(const ApplyArgs &a, hipStream_t s) { a.precision ? fwd_y<float>(a, s) : fwd_y<double>(a, s); }
@@ -393,11 +421,11 @@ void launch_fft2_plain(double2 *data, int n1, int n2, int64_t ncols, const doubl
     // along n1: columns (i2, c) flattened, pencil stride n2*ncols
     IoPlain<double> x{data, 0, (int64_t)n2 * ncols, (int64_t)n2 * ncols, 0, (int64_t)n2 * ncols, n1, n1};
     if (sign < 0) {
-        launch_pencil<double, -1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s);
-        launch_pencil<double, -1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s);
+        launch_pencil<double, -1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s, 0);
+        launch_pencil<double, -1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s, 0);
     } else {
-        launch_pencil<double, +1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s);
-        launch_pencil<double, +1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s);
+        launch_pencil<double, +1>(n2, y, (int64_t)n1 * ncols, 1, tw2, s, 0);
+        launch_pencil<double, +1>(n1, x, (int64_t)n2 * ncols, 1, tw1, s, 0);
     }
 }
 

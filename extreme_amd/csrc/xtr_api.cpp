@@ -13,6 +13,8 @@
 #include "../../include/xtreme_hip.h"
 
 #include <array>
+#include <map>
+#include <tuple>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -102,6 +104,23 @@ struct MappedBuf {
     MappedBuf &operator=(const MappedBuf &) = delete;
 };
 
+// FFT tile choices measured per process: (precision, nx, ny, nz) -> tile of each pass (the number
+// of right-hand sides only repeats the same workgroups along the grid's y).
+struct TileKey {
+    int p, nx, ny, nz;
+    bool operator<(const TileKey &o) const { return std::tie(p, nx, ny, nz) < std::tie(o.p, o.nx, o.ny, o.nz); }
+};
+std::map<TileKey, std::array<int, 4>> &tile_cache()
+{
+    static std::map<TileKey, std::array<int, 4>> m;
+    return m;
+}
+std::mutex &tile_cache_mu()
+{
+    static std::mutex m;
+    return m;
+}
+
 // Waits for everything submitted to `s` so far by polling an event: the host resumes as soon as
 // the GPU passes the marker instead of being woken from a blocking wait.
 inline void spin_wait(hipStream_t s, hipEvent_t ev)
@@ -155,6 +174,7 @@ struct xtr_op {
 
     // Per-stage timing: one set of six events per apply while profiling is on, summed on demand.
     bool profiling = false;
+    std::array<int, 4> last_tiles{{-1, -1, -1, -1}};   // FFT column tiles of the latest apply
     std::vector<std::array<hipEvent_t, 6>> ev_sets;
     size_t ev_used = 0;
     std::mutex apply_mu;
@@ -180,6 +200,66 @@ struct xtr_op {
     size_t elem() const { return precision == XTR_FP32 ? sizeof(float2) : sizeof(double2); }
     int arith() const { return precision == XTR_MIXED ? active : precision; }
 
+    // Column tiles of the four FFT passes for this apply's precision and grid: the
+    // XTR_FFT_TILES override (four letters d/n/w: default, narrow, wide), else the choice measured
+    // once per process and shape. Measuring runs each pass with each of its tiles on this apply's
+    // own buffers (the passes rewrite the spectrum and y, which the apply then recomputes from x).
+    std::array<int, 4> fft_tiles(xtr::ApplyArgs &a, hipStream_t s)
+    {
+        const int P[4] = {2 * g.ny, 2 * g.nx, 2 * g.nx, 2 * g.ny};
+        if (const char *e = std::getenv("XTR_FFT_TILES")) {
+            std::array<int, 4> t{};
+            for (int k = 0; k < 4; ++k) {
+                const char c = e[0] && std::strlen(e) == 4 ? e[k] : 'd';
+                t[k] = c == 'w' ? xtr::FFT_TILE_WIDE : c == 'n' ? xtr::FFT_TILE_NARROW : xtr::FFT_TILE_DEFAULT;
+            }
+            return t;
+        }
+        const TileKey key{a.precision, g.nx, g.ny, g.nz};
+        {
+            std::lock_guard<std::mutex> lk(tile_cache_mu());
+            auto it = tile_cache().find(key);
+            if (it != tile_cache().end()) return it->second;
+        }
+        using Pass = void (*)(const xtr::ApplyArgs &, hipStream_t);
+        const Pass pass[4] = {xtr::launch_fwd_y, xtr::launch_fwd_x, xtr::launch_inv_x, xtr::launch_inv_y};
+        hipEvent_t e0, e1;
+        XTR_CHECK(hipEventCreate(&e0));
+        XTR_CHECK(hipEventCreate(&e1));
+        std::array<int, 4> best{};
+        for (int k = 0; k < 4; ++k) a.fft_tile[k] = xtr::FFT_TILE_DEFAULT;
+        try {
+            for (int k = 0; k < 4; ++k) {
+                const int nopt = xtr::fft_tile_options(P[k], a.precision);
+                float best_ms = 0;
+                for (int t = 0; t < nopt; ++t) {
+                    a.fft_tile[k] = t;
+                    pass[k](a, s);                           // warm-up
+                    XTR_CHECK(hipEventRecord(e0, s));
+                    for (int r = 0; r < 3; ++r) pass[k](a, s);
+                    XTR_CHECK(hipEventRecord(e1, s));
+                    XTR_CHECK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    XTR_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (t == 0 || ms < best_ms) {
+                        best_ms = ms;
+                        best[k] = t;
+                    }
+                }
+                a.fft_tile[k] = xtr::FFT_TILE_DEFAULT;
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(tile_cache_mu());
+        tile_cache()[key] = best;
+        return best;
+    }
+
     // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr).
     void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s)
     {
@@ -203,6 +283,8 @@ struct xtr_op {
         a.tw_x = p == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
         a.tw_y = p == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
         a.v = v;
+        last_tiles = fft_tiles(a, s);
+        for (int k = 0; k < 4; ++k) a.fft_tile[k] = last_tiles[k];
         hipEvent_t *ev = profiling ? next_events() : nullptr;
         if (ev) XTR_CHECK(hipEventRecord(ev[0], s));
         xtr::launch_fwd_y(a, s);
@@ -1025,6 +1107,15 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies)
             }
         }
         if (applies) *applies = (int64_t)op->ev_used;
+    });
+}
+
+int xtr_op_fft_tiles(xtr_op *op, int *tiles4)
+{
+    return guarded([&] {
+        require(op && tiles4, "null argument");
+        std::lock_guard<std::mutex> lk(op->apply_mu);
+        for (int k = 0; k < 4; ++k) tiles4[k] = op->last_tiles[k];
     });
 }
 

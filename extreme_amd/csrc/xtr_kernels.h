@@ -103,8 +103,13 @@ struct ApplyArgs {
     void *buf;                // nrhs * M spectrum in apply precision
     const void *tw_x;         // twiddles for P = 2 nx
     const void *tw_y;         // twiddles for P = 2 ny
+    int fft_tile[4];          // column tile of fwd_y, fwd_x, inv_x, inv_y (FFT_TILE_*)
     RhsPtrs v;
 };
+
+enum { FFT_TILE_DEFAULT = 0, FFT_TILE_NARROW = 1, FFT_TILE_WIDE = 2 };
+// number of column tiles a pencil length offers (1: only the default; 2: + narrow; 3: + wide)
+int fft_tile_options(int p, int precision);
 
 // Stage launchers (used separately for per-stage timing).
 void launch_fwd_y(const ApplyArgs &a, hipStream_t s);

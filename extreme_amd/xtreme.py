@@ -104,6 +104,7 @@ def lib():
         L.xtr_op_level_field_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.xtr_op_fft_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
@@ -344,6 +345,17 @@ class ConvolutionOperator:
         n = C.c_int64()
         _check(lib().xtr_op_stage_times(self._h, out, C.byref(n)))
         return dict(zip(self.STAGES, list(out))), n.value
+
+    FFT_TILES = ("default", "narrow", "wide")
+
+    def fft_tiles(self):
+        """Column tile of each FFT pass (fwd_y, fwd_x, inv_x, inv_y) in the latest apply: chosen
+        by measurement once per process and shape (or fixed by XTR_FFT_TILES); None before the
+        first apply."""
+        out = (C.c_int * 4)()
+        _check(lib().xtr_op_fft_tiles(self._h, out))
+        return None if out[0] < 0 else dict(zip(("fwd_y", "fwd_x", "inv_x", "inv_y"),
+                                                (self.FFT_TILES[t] for t in out)))
 
     def sizes(self):
         n, gb, bb = C.c_int64(), C.c_int64(), C.c_int64()

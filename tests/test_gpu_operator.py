@@ -204,3 +204,37 @@ def test_apply_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
     assert rel(y2, ref) < TOL_FP32
     assert rel(y1, ref[1]) < TOL_FP32
     assert rel(y2, yg) < 1e-6
+
+
+@pytest.mark.parametrize("tiles", ["nnnn", "wwww", None], ids=["narrow", "wide", "tuned"])
+def test_fft_tiles_give_the_same_apply(oracle, torch_cuda, monkeypatch, tiles):
+    """Every FFT column tile (forced, or the measured choice) gives the same apply as the oracle;
+    the tile only regroups columns, so FP64 results stay within rounding of the default tile."""
+    m = cm.small_model(32, 32, 4, seed=21)                 # pencils of 64: all three tiles exist
+    w = cm.omega_of(m.frequencies[0])
+    G = oracle.green_atoa(m, w)
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    oop = oracle.Operator(m, G)
+    oop.prepare("A", z, z0)
+    rng = np.random.default_rng(3)
+    xs = rng.standard_normal((2, m.n)) + 1j * rng.standard_normal((2, m.n))
+    ref = np.stack([oop.apply(x) for x in xs])
+    monkeypatch.setenv("XTR_FFT_TILES", "dddd")
+    op = X.ConvolutionOperator(m, w, max_rhs=2)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", z)
+    y_default = op.apply(xs)
+    if tiles is None:
+        monkeypatch.delenv("XTR_FFT_TILES")
+    else:
+        monkeypatch.setenv("XTR_FFT_TILES", tiles)
+    for rep in range(3):                                    # the first apply of a shape measures
+        y = op.apply(xs)
+        e, d = rel(y, ref), rel(y, y_default)
+        print(f"tiles {tiles} rep {rep}: {op.fft_tiles()} vs oracle {e:.2e}, vs default tile {d:.2e}", flush=True)
+        assert e < 1e-12 and d < 1e-14
+    got = op.fft_tiles()
+    assert set(got) == {"fwd_y", "fwd_x", "inv_x", "inv_y"}
+    if tiles is not None:
+        want = {"n": "narrow", "w": "wide"}[tiles[0]]
+        assert all(v == want for v in got.values())
