@@ -79,19 +79,19 @@ __device__ inline void dft_small(cplx<T> *a)
     }
 }
 
-// One Stockham stage over the LDS tile s[p*TC + c] and the recursion to the next stage.
-template <typename T, int P, int TC, int DIR, int LEN, int ST>
+// One Stockham stage over the LDS tile s[p*TC + c] (NT threads) and the recursion to the next stage.
+template <typename T, int P, int TC, int DIR, int LEN, int ST, int NT>
 __device__ inline void fft_stage(cplx<T> *s, const cplx<T> *tw, int tid)
 {
     constexpr int R = radix_of(LEN);
     constexpr int M = LEN / R;
     constexpr int NB = (P / R) * TC;
-    constexpr int NPT = (NB + kFftThreads - 1) / kFftThreads;
+    constexpr int NPT = (NB + NT - 1) / NT;
     cplx<T> y[NPT][R];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
-        const int b = tid + k * kFftThreads;
-        if (NB % kFftThreads == 0 || b < NB) {
+        const int b = tid + k * NT;
+        if (NB % NT == 0 || b < NB) {
             const int cl = b % TC, bb = b / TC;
             const int p = bb / ST, q = bb % ST;
             cplx<T> a[R];
@@ -107,8 +107,8 @@ __device__ inline void fft_stage(cplx<T> *s, const cplx<T> *tw, int tid)
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
-        const int b = tid + k * kFftThreads;
-        if (NB % kFftThreads == 0 || b < NB) {
+        const int b = tid + k * NT;
+        if (NB % NT == 0 || b < NB) {
             const int cl = b % TC, bb = b / TC;
             const int p = bb / ST, q = bb % ST;
 #pragma unroll
@@ -116,7 +116,7 @@ __device__ inline void fft_stage(cplx<T> *s, const cplx<T> *tw, int tid)
         }
     }
     __syncthreads();
-    if constexpr (M > 1) fft_stage<T, P, TC, DIR, M, ST * R>(s, tw, tid);
+    if constexpr (M > 1) fft_stage<T, P, TC, DIR, M, ST * R, NT>(s, tw, tid);
 }
 
 // ---- load/store policies -------------------------------------------------------------------
@@ -209,33 +209,33 @@ struct IoInvY {
     }
 };
 
-template <typename T, int P, int TC, int DIR, class IO>
-__global__ __launch_bounds__(kFftThreads) void pencil_fft_kernel(IO io, const cplx<T> *__restrict__ twg)
+template <typename T, int P, int TC, int DIR, class IO, int NT>
+__global__ __launch_bounds__(NT) void pencil_fft_kernel(IO io, const cplx<T> *__restrict__ twg)
 {
     __shared__ cplx<T> s[P * TC];
     __shared__ cplx<T> tw[P];
     const int tid = threadIdx.x;
-    for (int k = tid; k < P; k += kFftThreads) {
+    for (int k = tid; k < P; k += NT) {
         cplx<T> w = twg[k];
         tw[k] = DIR < 0 ? w : cconj<T>(w);
     }
     const int64_t c0 = (int64_t)blockIdx.x * TC;
     const int rhs = blockIdx.y;
-    constexpr int NE = (P * TC + kFftThreads - 1) / kFftThreads;
+    constexpr int NE = (P * TC + NT - 1) / NT;
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
-        const int e = tid + k * kFftThreads;
-        if ((P * TC) % kFftThreads == 0 || e < P * TC) {
+        const int e = tid + k * NT;
+        if ((P * TC) % NT == 0 || e < P * TC) {
             const int p = e / TC, cl = e % TC;
             s[p * TC + cl] = io.load(rhs, c0 + cl, p);
         }
     }
     __syncthreads();
-    fft_stage<T, P, TC, DIR, P, 1>(s, tw, tid);
+    fft_stage<T, P, TC, DIR, P, 1, NT>(s, tw, tid);
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
-        const int e = tid + k * kFftThreads;
-        if ((P * TC) % kFftThreads == 0 || e < P * TC) {
+        const int e = tid + k * NT;
+        if ((P * TC) % NT == 0 || e < P * TC) {
             const int p = e / TC, cl = e % TC;
             io.store(rhs, c0 + cl, p, s[p * TC + cl]);
         }
@@ -244,9 +244,10 @@ __global__ __launch_bounds__(kFftThreads) void pencil_fft_kernel(IO io, const cp
 
 // Column tiles per pencil length. The default tile has 16 columns while P x 16 stays within 32 KB,
 // else 8. For lengths from 64 there are two more: the narrow tile (half the columns, more workgroups
-// per CU) and the wide one (twice the columns, longer contiguous runs per row, within 128 KB). Which
+// per CU) and the wide one (twice the columns, longer contiguous runs per row, within 128 KB); from
+// 128, the narrow tile also runs with 512 threads (twice the waves per CU for the same LDS). Which
 // one a pass uses is measured on the device (the operator's tile tuning); the choice does not change
-// the arithmetic, only how the columns are grouped.
+// the arithmetic, only how the columns are grouped and how many threads share a tile.
 template <typename T, int P>
 constexpr int tile_cols()
 {
@@ -258,8 +259,13 @@ constexpr int tile_cols_of(int tile)
     constexpr int d = tile_cols<T, P>();
     if (P < 64) return d;
     if (tile == FFT_TILE_WIDE) return P * 2 * d * (int)sizeof(cplx<T>) <= 131072 ? 2 * d : d;
-    if (tile == FFT_TILE_NARROW) return d / 2 >= 4 ? d / 2 : d;
+    if (tile == FFT_TILE_NARROW || tile == FFT_TILE_NARROW_T512) return d / 2 >= 4 ? d / 2 : d;
     return d;
+}
+template <int P>
+constexpr int tile_threads_of(int tile)
+{
+    return tile == FFT_TILE_NARROW_T512 && P >= 128 ? 512 : kFftThreads;
 }
 
 template <typename T, int DIR, class IO>
@@ -269,15 +275,19 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
         constexpr int PP = decltype(pc)::value;
         constexpr int TD = tile_cols_of<T, PP>(FFT_TILE_DEFAULT), TW = tile_cols_of<T, PP>(FFT_TILE_WIDE),
                       TN = tile_cols_of<T, PP>(FFT_TILE_NARROW);
-        auto run = [&](auto tc) {
-            constexpr int TC = decltype(tc)::value;
+        constexpr int NW = tile_threads_of<PP>(FFT_TILE_NARROW_T512);
+        auto run = [&](auto tc, auto nt) {
+            constexpr int TC = decltype(tc)::value, NT = decltype(nt)::value;
             dim3 grid((unsigned)ceil_div(ncols, TC), nrhs);
-            hipLaunchKernelGGL((pencil_fft_kernel<T, PP, TC, DIR, IO>), grid, dim3(kFftThreads), 0, s, io,
+            hipLaunchKernelGGL((pencil_fft_kernel<T, PP, TC, DIR, IO, NT>), grid, dim3(NT), 0, s, io,
                                (const cplx<T> *)tw);
         };
-        if (tile == FFT_TILE_WIDE && TW != TD) run(std::integral_constant<int, TW>{});
-        else if (tile == FFT_TILE_NARROW && TN != TD) run(std::integral_constant<int, TN>{});
-        else run(std::integral_constant<int, TD>{});
+        using T256 = std::integral_constant<int, kFftThreads>;
+        if (tile == FFT_TILE_WIDE && TW != TD) run(std::integral_constant<int, TW>{}, T256{});
+        else if (tile == FFT_TILE_NARROW && TN != TD) run(std::integral_constant<int, TN>{}, T256{});
+        else if (tile == FFT_TILE_NARROW_T512 && TN != TD && NW != kFftThreads)
+            run(std::integral_constant<int, TN>{}, std::integral_constant<int, NW>{});
+        else run(std::integral_constant<int, TD>{}, T256{});
     };
     switch (P) {
     case 4: go(std::integral_constant<int, 4>{}); break;
@@ -318,12 +328,17 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
 }
 
 // even lengths 2^a m, m in {1, 3, 5, 15, 25, 75}, up to 512 (Nx, Ny <= 256)
-int fft_tile_options(int p, int precision)
+int fft_tile_options(int p, int precision, int *tiles)
 {
-    if (p < 64) return 1;
+    int n = 0;
+    tiles[n++] = FFT_TILE_DEFAULT;
+    if (p < 64) return n;
     const int e = precision ? 8 : 16;
     const int d = (p * 16 * e <= 32768) ? 16 : 8;
-    return 2 + (p * 2 * d * e <= 131072 ? 1 : 0);   // default, narrow (d / 2 >= 4 always), wide if it fits
+    tiles[n++] = FFT_TILE_NARROW;                    // d / 2 >= 4 always
+    if (p * 2 * d * e <= 131072) tiles[n++] = FFT_TILE_WIDE;
+    if (p >= 128) tiles[n++] = FFT_TILE_NARROW_T512;
+    return n;
 }
 
 bool fft_length_supported(int p)

@@ -201,7 +201,8 @@ struct xtr_op {
     int arith() const { return precision == XTR_MIXED ? active : precision; }
 
     // Column tiles of the four FFT passes for this apply's precision and grid: the
-    // XTR_FFT_TILES override (four letters d/n/w: default, narrow, wide), else the choice measured
+    // XTR_FFT_TILES override (four letters d/n/w/m: default, narrow, wide, narrow with 512 threads),
+    // else the choice measured
     // once per process and shape. Measuring runs each pass with each of its tiles on this apply's
     // own buffers (the passes rewrite the spectrum and y, which the apply then recomputes from x).
     std::array<int, 4> fft_tiles(xtr::ApplyArgs &a, hipStream_t s)
@@ -211,7 +212,10 @@ struct xtr_op {
             std::array<int, 4> t{};
             for (int k = 0; k < 4; ++k) {
                 const char c = e[0] && std::strlen(e) == 4 ? e[k] : 'd';
-                t[k] = c == 'w' ? xtr::FFT_TILE_WIDE : c == 'n' ? xtr::FFT_TILE_NARROW : xtr::FFT_TILE_DEFAULT;
+                t[k] = c == 'w'   ? xtr::FFT_TILE_WIDE
+                       : c == 'n' ? xtr::FFT_TILE_NARROW
+                       : c == 'm' ? xtr::FFT_TILE_NARROW_T512
+                                  : xtr::FFT_TILE_DEFAULT;
             }
             return t;
         }
@@ -230,9 +234,11 @@ struct xtr_op {
         for (int k = 0; k < 4; ++k) a.fft_tile[k] = xtr::FFT_TILE_DEFAULT;
         try {
             for (int k = 0; k < 4; ++k) {
-                const int nopt = xtr::fft_tile_options(P[k], a.precision);
+                int opts[4];
+                const int nopt = xtr::fft_tile_options(P[k], a.precision, opts);
                 float best_ms = 0;
-                for (int t = 0; t < nopt; ++t) {
+                for (int o = 0; o < nopt; ++o) {
+                    const int t = opts[o];
                     a.fft_tile[k] = t;
                     pass[k](a, s);                           // warm-up
                     XTR_CHECK(hipEventRecord(e0, s));
@@ -241,7 +247,7 @@ struct xtr_op {
                     XTR_CHECK(hipEventSynchronize(e1));
                     float ms = 0;
                     XTR_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                    if (t == 0 || ms < best_ms) {
+                    if (o == 0 || ms < best_ms) {
                         best_ms = ms;
                         best[k] = t;
                     }

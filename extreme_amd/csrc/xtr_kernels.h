@@ -107,9 +107,9 @@ struct ApplyArgs {
     RhsPtrs v;
 };
 
-enum { FFT_TILE_DEFAULT = 0, FFT_TILE_NARROW = 1, FFT_TILE_WIDE = 2 };
-// number of column tiles a pencil length offers (1: only the default; 2: + narrow; 3: + wide)
-int fft_tile_options(int p, int precision);
+enum { FFT_TILE_DEFAULT = 0, FFT_TILE_NARROW = 1, FFT_TILE_WIDE = 2, FFT_TILE_NARROW_T512 = 3 };
+// the column tiles a pencil length offers in this precision (FFT_TILE_*), into tiles[4]; returns how many
+int fft_tile_options(int p, int precision, int *tiles);
 
 // Stage launchers (used separately for per-stage timing).
 void launch_fwd_y(const ApplyArgs &a, hipStream_t s);

@@ -204,10 +204,11 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
 
 /* No counterpart in the reference (its FFTs are FFTW plans): the column tile each of the four FFT
  * passes (y forward, x forward, x inverse, y inverse) used in the latest apply, 0 = default,
- * 1 = narrow (half the columns), 2 = wide (twice the columns), -1 before the first apply. The tile
- * is measured once per process for each precision and grid, on the first apply of that shape (a
- * few milliseconds at 256 x 256 x 64); the environment variable XTR_FFT_TILES (four letters d, n, w) fixes it
- * instead. The tile changes only how columns are grouped into workgroups, not the arithmetic. */
+ * 1 = narrow (half the columns), 2 = wide (twice the columns), 3 = narrow with 512 threads per
+ * workgroup, -1 before the first apply. The tile is measured once per process for each precision
+ * and grid, on the first apply of that shape (a few milliseconds at 256 x 256 x 64); the environment
+ * variable XTR_FFT_TILES (four letters d, n, w, m) fixes it instead. The tile changes only how
+ * columns are grouped into workgroups, not the arithmetic. */
 int xtr_op_fft_tiles(xtr_op *op, int *tiles4);
 
 /* ---- observation level (AtoO) ----------------------------------------------------------------- */

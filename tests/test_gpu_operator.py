@@ -206,11 +206,11 @@ def test_apply_fp32_streaming(oracle, torch_cuda, monkeypatch, shape):
     assert rel(y2, yg) < 1e-6
 
 
-@pytest.mark.parametrize("tiles", ["nnnn", "wwww", None], ids=["narrow", "wide", "tuned"])
+@pytest.mark.parametrize("tiles", ["nnnn", "wwww", "mmmm", None], ids=["narrow", "wide", "narrow512", "tuned"])
 def test_fft_tiles_give_the_same_apply(oracle, torch_cuda, monkeypatch, tiles):
     """Every FFT column tile (forced, or the measured choice) gives the same apply as the oracle;
     the tile only regroups columns, so FP64 results stay within rounding of the default tile."""
-    m = cm.small_model(32, 32, 4, seed=21)                 # pencils of 64: all three tiles exist
+    m = cm.small_model(64, 64, 2, seed=21)                 # pencils of 128: all four tiles exist
     w = cm.omega_of(m.frequencies[0])
     G = oracle.green_atoa(m, w)
     z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
@@ -236,5 +236,5 @@ def test_fft_tiles_give_the_same_apply(oracle, torch_cuda, monkeypatch, tiles):
     got = op.fft_tiles()
     assert set(got) == {"fwd_y", "fwd_x", "inv_x", "inv_y"}
     if tiles is not None:
-        want = {"n": "narrow", "w": "wide"}[tiles[0]]
+        want = {"n": "narrow", "w": "wide", "m": "narrow512"}[tiles[0]]
         assert all(v == want for v in got.values())
