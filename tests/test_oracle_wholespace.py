@@ -129,3 +129,30 @@ def test_atoa_tensor_equals_the_whole_space_green_function(oracle):
         err = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-3 * scale)
         print(f"AtoA {c}: relative error {err:.2e}")
         assert err < 5e-5, c
+
+
+def test_splitting_a_background_layer_changes_nothing(oracle):
+    """A background layer split in two layers of the same conductivity (at 600 m, between anomaly
+    layers, so cells on either side switch from the same-layer to the cross-layer branches of the
+    vertical functions) must leave every tensor unchanged: the layer recursions then run through a
+    transparent interface, with the real interfaces' reflections present on both sides."""
+    nx, ny, nz = 4, 3, 4
+    depth = 500.0 + np.arange(nz) * 50.0
+    thick = np.full(nz, 50.0)
+    sigma = np.full((nx, ny, nz), 0.05)
+    whole = cm.Section1D(0.0, [0.0, 400.0, 900.0, 0.0], [0.0, 0.02, 0.005, 0.1])
+    split = cm.Section1D(0.0, [0.0, 400.0, 200.0, 700.0, 0.0], [0.0, 0.02, 0.005, 0.005, 0.1])
+    ma = cm.CartesianModel(nx, ny, 100.0, 120.0, depth, thick, sigma, whole, "whole layer", [1.0])
+    mb = cm.CartesianModel(nx, ny, 100.0, 120.0, depth, thick, sigma, split, "split layer", [1.0])
+    w = cm.omega_of(3.0)
+    ga, gb = oracle.green_atoa(ma, w, fft=False), oracle.green_atoa(mb, w, fft=False)
+    for c in ga:
+        assert np.abs(ga[c] - gb[c]).max() <= 1e-13 * np.abs(ga[c]).max(), c
+    for zl in (0.0, 560.0, 620.0, 800.0):
+        for electric in (True, False):
+            a = oracle.green_atoo_level(ma, w, zl, electric, fft=False)
+            b = oracle.green_atoo_level(mb, w, zl, electric, fft=False)
+            scale = max(np.abs(v).max() for v in a.values())
+            err = max(np.abs(a[c] - b[c]).max() for c in a) / scale
+            print(f"level {zl} {'E' if electric else 'H'}: {err:.1e}")
+            assert err <= 1e-12
