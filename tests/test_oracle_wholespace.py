@@ -1,4 +1,4 @@
-"""Pins the oracle's Green tensors to the physics, without the reference: in a whole space (every
+"""Pins the oracle's Green tensors to the physics, without the reference. In a whole space (every
 background layer the same conductivity, so the layer recursions carry no reflections) the tensors
 must equal direct quadrature of the closed-form dyadic Green's function of an electric current
 element,
@@ -156,3 +156,41 @@ def test_splitting_a_background_layer_changes_nothing(oracle):
             err = max(np.abs(a[c] - b[c]).max() for c in a) / scale
             print(f"level {zl} {'E' if electric else 'H'}: {err:.1e}")
             assert err <= 1e-12
+
+
+@pytest.mark.parametrize("s2", [1e4, 1e-9], ids=["conductor below", "insulator below"])
+def test_interface_limits(oracle, s2):
+    """Currents in a layer over a half-space of very different conductivity, fields at the interface
+    (just inside the lower medium), against the same currents in a whole space. Image theory fixes
+    the limits. Over a near-perfect conductor: tangential E -> 0, tangential H doubles, and
+    E_z = 2 (s1 / s2) of the whole-space value (normal current continuity on the doubled field).
+    Over an insulator: tangential E doubles. These are reflections at a contrast for every lambda,
+    with the sign of each mode's coefficient fixed."""
+    nx, ny, nz = 4, 3, 2
+    s1 = 0.01
+    depth = 1000.0 + np.arange(nz) * 50.0
+    thick = np.full(nz, 50.0)
+    w = cm.omega_of(1.0)
+    rng = np.random.default_rng(3)
+    jq = (rng.standard_normal((nx, ny, 3, nz)) + 1j * rng.standard_normal((nx, ny, 3, nz))).reshape(-1)
+
+    def fields(lower):
+        sec = cm.Section1D(0.0, [0.0, 1200.0, 1000.0, 0.0], [s1, s1, lower, lower])
+        m = cm.CartesianModel(nx, ny, 100.0, 120.0, depth, thick, np.full((nx, ny, nz), s1), sec, "interface", [1.0])
+        e = oracle.atoo_field(m, oracle.green_atoo_level(m, w, 1200.0, True), True, jq)
+        h = oracle.atoo_field(m, oracle.green_atoo_level(m, w, 1200.0, False), False, jq)
+        return e, h
+
+    e0, h0 = fields(s1)
+    e, h = fields(s2)
+    exy = np.abs(e[..., :2]).max() / np.abs(e0[..., :2]).max()
+    print(f"lower {s2:g}: |E_xy| ratio {exy:.3e}")
+    if s2 > s1:
+        hxy = np.abs(h[..., :2] - 2 * h0[..., :2]).max() / np.abs(2 * h0[..., :2]).max()
+        ez = np.abs(e[..., 2] - 2 * (s1 / s2) * e0[..., 2]).max() / np.abs(2 * (s1 / s2) * e0[..., 2]).max()
+        print(f"  H_xy vs 2 H0: {hxy:.2e}, E_z vs 2 (s1/s2) E0: {ez:.2e}")
+        # the conductor is not perfect: its skin depth (5 m at 1 Hz) against ~100 m source distances
+        # leaves a few per cent on the doubled H; a wrong reflection sign would cancel H instead
+        assert exy < 2e-4 and hxy < 5e-2 and ez < 1e-3
+    else:
+        assert np.abs(e[..., :2] - 2 * e0[..., :2]).max() <= 1e-3 * np.abs(2 * e0[..., :2]).max()
