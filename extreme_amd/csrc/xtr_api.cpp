@@ -19,6 +19,7 @@
 #include <cmath>
 #include <complex>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -119,6 +120,18 @@ std::mutex &tile_cache_mu()
 {
     static std::mutex m;
     return m;
+}
+
+// Shortest decimal that reads back as the same double (what .NET's default double formatting
+// prints in the reference's messages, up to the exponent's spelling).
+std::string shortest(double v)
+{
+    char b[40];
+    for (int p = 1; p <= 17; ++p) {
+        std::snprintf(b, sizeof(b), "%.*g", p, v);
+        if (std::strtod(b, nullptr) == v) break;
+    }
+    return b;
 }
 
 // Waits for everything submitted to `s` so far by polling an event: the host resumes as soon as
@@ -502,7 +515,8 @@ struct Gmres {
         matvec(x, w.p);
         xtr::launch_sub(rhs, w.p, w.p, n, ctx->stream);
         const double be = norm(w.p) / bn;
-        if (be >= tol && backward_check == 2) throw std::runtime_error("be = " + std::to_string(be) + ", tolerance = " + std::to_string(tol));
+        if (be >= tol && backward_check == 2)
+            throw std::runtime_error("be = " + shortest(be) + ", tolerance = " + shortest(tol));
         return be;
     }
 
@@ -562,9 +576,10 @@ struct Gmres {
         matvec(x0, r0.p);
         xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
         double beta = norm(w.p);
-        if (beta == 0) {
+        if (beta == 0) {   // CheckForExactSolution: ResultInfo.Converged(0, 0, 1) (Gmres.cs:104-107)
             if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
             bea_out = 0; be_out = 0; conv = 1;
+            iter_counter = 1;
             return;
         }
         double bea = -1, be = -1;

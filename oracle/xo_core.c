@@ -735,9 +735,10 @@ static int gm_solve(gm *s, const xo_c *rhs, const xo_c *x0, xo_c *result,
     for (int64_t t = 0; t < n; t++) s->r0[t] = rhs[t] - s->r0[t];
     memcpy(s->w, s->r0, sizeof(xo_c) * n);
     double beta = gm_norm(s, s->w);
-    if (beta == 0) {
+    if (beta == 0) {   /* CheckForExactSolution: ResultInfo.Converged(0, 0, 1) (Gmres.cs:104-107) */
         if (result != x0) memcpy(result, x0, sizeof(xo_c) * n);
         *out_bea = 0; *out_be = 0; *out_conv = 1;
+        s->iter_counter = 1;
         return 0;
     }
     double bea = -1, be = -1;

@@ -92,3 +92,70 @@ def test_zero_rhs_returns_immediately(oracle, torch_cuda):
     info = solver.solve(b, x)
     assert info.convergence_achieved and info.number_of_iterations == 0 and info.matvecs == 0
     assert np.all(x == 1)
+
+
+def test_non_convergence_matches_oracle(oracle, torch_cuda):
+    """FGMRES(2) / GMRES(1) with one repeat cannot reach 1e-12: ResultInfo.NonConverged with the
+    oracle's iterations, matvecs, Arnoldi and true backward errors (Gmres.cs:131-141), and the same
+    iterate."""
+    m = cm.config_c1()
+    w = cm.omega_of(m.frequencies[0])
+    G = oracle.green_atoa(m, w)
+    oop, chi0 = _rhs(oracle, m, w, G, 1)
+    x_ref, info_ref, _ = oracle.fgmres(oop.raw_apply, chi0, chi0, 2, 1, 1, 1e-12)
+    assert not info_ref["converged"]
+    s = X.ForwardSettings(residual=1e-12, inner_buffer_length=1, outer_buffer_length=2, max_repeats_number=1)
+    op = X.ConvolutionOperator(m, w, s)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    x = chi0.copy()
+    info = X.AnomalyCurrentFgmresSolver(op, s).solve(chi0, x)
+    print(f"non-converged: it {info.number_of_iterations} mv {info.matvecs} bea {info.arnoldi_backward_error:.4e} "
+          f"be {info.backward_error:.4e}; oracle {info_ref}", flush=True)
+    assert not info.convergence_achieved
+    assert (info.number_of_iterations, info.matvecs) == (info_ref["iterations"], info_ref["matvecs"])
+    assert abs(info.arnoldi_backward_error - info_ref["bea"]) <= 1e-9 * info_ref["bea"]
+    assert abs(info.backward_error - info_ref["be"]) <= 1e-9 * info_ref["be"]
+    assert rel(x, x_ref) < TOL_SOLUTION
+
+
+def test_exact_initial_guess_returns_at_once(oracle, torch_cuda):
+    """b = A x0 with the same device apply: the first residual is exactly zero, so the solve returns
+    x0 with ResultInfo.Converged(0, 0, 1) after one matvec (Gmres.cs:95-107)."""
+    m = cm.small_model(4, 4, 2, seed=9)
+    w = cm.omega_of(1.0)
+    G = oracle.green_atoa(m, w)
+    op = X.ConvolutionOperator(m, w)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    rng = np.random.default_rng(2)
+    x0 = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    b = op.apply(x0)
+    x = x0.copy()
+    info = X.AnomalyCurrentFgmresSolver(op).solve(b, x)
+    print(f"exact guess: converged {info.convergence_achieved} it {info.number_of_iterations} mv {info.matvecs}",
+          flush=True)
+    assert info.convergence_achieved and info.number_of_iterations == 1 and info.matvecs == 1
+    assert info.arnoldi_backward_error == 0 and info.backward_error == 0
+    assert np.array_equal(x, x0)
+
+
+def test_backward_error_exception(oracle, torch_cuda):
+    """BackwardErrorChecking.CheckWithException: a solve that stops above the tolerance raises
+    BackwardErrorException with "be = ..., tolerance = ..." (Gmres.cs:217-223), here an error
+    status whose message carries the same text."""
+    m = cm.small_model(4, 4, 2, seed=9)
+    w = cm.omega_of(1.0)
+    G = oracle.green_atoa(m, w)
+    s = X.ForwardSettings(residual=1e-14, inner_buffer_length=1, outer_buffer_length=1, max_repeats_number=1)
+    op = X.ConvolutionOperator(m, w, s)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    x = b.copy()
+    with pytest.raises(RuntimeError, match=r"be = .*, tolerance = 1e-14"):
+        X.AnomalyCurrentFgmresSolver(op, s, backward_error_checking=2).solve(b, x)
+    x = b.copy()
+    info = X.AnomalyCurrentFgmresSolver(op, s, backward_error_checking=1).solve(b, x)   # warning only
+    assert not info.convergence_achieved and info.backward_error >= 1e-14
