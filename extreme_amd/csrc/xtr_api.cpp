@@ -539,6 +539,8 @@ struct Gmres {
             do {
                 ++north;
                 cgs(jH, dloo, dnormw);
+                if (!std::isfinite(dnormw) || !std::isfinite(dloo))
+                    throw std::runtime_error("a Krylov vector holds NaN or Inf: the operator produced non-finite values");
             } while ((2.0 * dnormw <= std::sqrt(dloo)) && (north < 3));
             h(jH + 1, jH) = dnormw;
             if (jH < m - 1) xtr::launch_div(w.p, dnormw, col(V, jH + 1), n, ctx->stream);
@@ -569,6 +571,8 @@ struct Gmres {
     {
         iter_counter = 0;
         const double bn = norm(rhs);
+        // NaN or Inf would end the iteration as if converged (bea > tol is false): refuse instead
+        if (!std::isfinite(bn)) throw std::runtime_error("the right-hand side holds NaN or Inf");
         if (bn == 0) {
             bea_out = 0; be_out = 0; conv = 1;
             return;
@@ -576,6 +580,7 @@ struct Gmres {
         matvec(x0, r0.p);
         xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
         double beta = norm(w.p);
+        if (!std::isfinite(beta)) throw std::runtime_error("the initial residual holds NaN or Inf (operator or initial guess)");
         if (beta == 0) {   // CheckForExactSolution: ResultInfo.Converged(0, 0, 1) (Gmres.cs:104-107)
             if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
             bea_out = 0; be_out = 0; conv = 1;
@@ -1273,6 +1278,7 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
                 info[k].matvecs64++;
                 xtr::launch_sub((const double2 *)b[k], r[k].p, r[k].p, n, op->stream);
                 rn[k] = bn[k] > 0 ? norm(r[k].p) : 0.0;
+                if (!std::isfinite(rn[k])) throw std::runtime_error("the FP64 residual holds NaN or Inf");
                 info[k].residual = bn[k] > 0 ? rn[k] / bn[k] : 0.0;
                 if (info[k].residual <= params->tolerance) info[k].converged = 1;
                 else if (step < params->max_refinements) next.push_back(k);

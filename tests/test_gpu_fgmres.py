@@ -159,3 +159,22 @@ def test_backward_error_exception(oracle, torch_cuda):
     x = b.copy()
     info = X.AnomalyCurrentFgmresSolver(op, s, backward_error_checking=1).solve(b, x)   # warning only
     assert not info.convergence_achieved and info.backward_error >= 1e-14
+
+
+def test_non_finite_operator_is_refused(oracle, torch_cuda):
+    """A Green tensor holding NaN (what the layer recursions give when exp(eta h) overflows) makes
+    every Krylov vector NaN; the solve must raise rather than stop as if converged (NaN > tol is
+    false in the stopping test)."""
+    m = cm.small_model(4, 4, 2, seed=9)
+    w = cm.omega_of(1.0)
+    G = oracle.green_atoa(m, w)
+    G["xx"] = G["xx"].copy()
+    G["xx"][3, 0] = np.nan
+    op = X.ConvolutionOperator(m, w)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    x = b.copy()
+    with pytest.raises(RuntimeError, match="NaN or Inf"):
+        X.AnomalyCurrentFgmresSolver(op).solve(b, x)
