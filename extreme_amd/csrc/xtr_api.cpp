@@ -9,6 +9,7 @@
 // test (Gmres.cs:297-300) needs to decide on a second pass.
 #include "xtr_green.h"
 #include "xtr_kernels.h"
+#include "xtr_learned.h"
 #include "xtr_common.h"
 #include "../../include/xtreme_hip.h"
 
@@ -379,6 +380,10 @@ struct SolveCtx {
     DevBuf<double> partial_r;    // [kRedBlocks]
     DevBuf<double2> h_dev;       // [32] projections
     DevBuf<double> norm_dev;     // [1]
+    // pairs (v_j, z_j) of the flexible preconditioner, copied out while count < cap
+    double2 *const *harvest_v = nullptr;
+    double2 *const *harvest_z = nullptr;
+    int harvest_cap = 0, harvest_count = 0;
     MappedBuf<double2> h_host{32};   // the projections, written by the reduction kernel
     MappedBuf<double> nn_host{1};    // the squared norm, likewise
     hipEvent_t ev = nullptr;
@@ -395,6 +400,73 @@ struct SolveCtx {
     void wait() { spin_wait(stream, ev); }
 };
 
+}  // namespace
+
+// Learned right preconditioner (no counterpart in the reference): the network's device state.
+struct xtr_net {
+    xtr::NetGeom g{};
+    hipStream_t stream = nullptr;
+    DevBuf<unsigned short> x0, h, dyb, dpb, W1, W2, WT2;
+    DevBuf<float> pre, y, tgt, dy, dh, dp, part;
+    DevBuf<float> w1, w2, m1, m2, s1, s2, g1, g2;   // FP32 master weights, Adam moments, gradients
+    DevBuf<double> partial_r, nn, lpart;
+    int step = 0;
+    std::mutex mu;
+
+    ~xtr_net()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    xtr::NetBuffers bufs()
+    {
+        return xtr::NetBuffers{x0.p, h.p, pre.p, y.p, tgt.p, dy.p, dyb.p, dh.p, dp.p, dpb.p, part.p, W1.p, W2.p, WT2.p};
+    }
+};
+
+namespace {
+
+// z = the network applied to v (complex n-vectors on the device), on stream s
+void net_precondition(xtr_net *net, const double2 *v, double2 *z, hipStream_t s)
+{
+    const int64_t n = net->g.cells * 3;
+    xtr::launch_norm2(v, n, net->partial_r.p, s);
+    xtr::launch_reduce_r(net->partial_r.p, net->nn.p, s);
+    const xtr::NetBuffers b = net->bufs();
+    xtr::net_pack(net->g, v, nullptr, net->nn.p, b, s);
+    xtr::net_forward(net->g, b, s);
+    xtr::net_unpack(net->g, v, net->nn.p, b, z, s);
+}
+
+// Gradient of the mean relative loss over the samples into net->g1, net->g2; returns the mean loss.
+double net_gradients(xtr_net *net, int ns, const double2 *const *v, const double2 *const *z)
+{
+    const int64_t n = net->g.cells * 3;
+    const hipStream_t s = net->stream;
+    const xtr::NetBuffers b = net->bufs();
+    const int nb = xtr::net_loss_blocks(net->g);
+    std::vector<double> lp(2 * (size_t)nb);
+    double total = 0;
+    for (int k = 0; k < ns; ++k) {
+        xtr::launch_norm2(v[k], n, net->partial_r.p, s);
+        xtr::launch_reduce_r(net->partial_r.p, net->nn.p, s);
+        xtr::net_pack(net->g, v[k], z[k], net->nn.p, b, s);
+        xtr::net_forward(net->g, b, s);
+        xtr::net_loss(net->g, b, net->lpart.p, s);
+        XTR_CHECK(hipMemcpyAsync(lp.data(), net->lpart.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        XTR_CHECK(hipStreamSynchronize(s));
+        double se = 0, st = 0;
+        for (int i = 0; i < nb; ++i) {
+            se += lp[2 * i];
+            st += lp[2 * i + 1];
+        }
+        const float scale = st > 0 ? (float)(1.0 / (st * ns)) : 0.0f;
+        xtr::net_backward(net->g, b, net->g1.p, net->g2.p, scale, k > 0, s);
+        total += st > 0 ? se / st : 0.0;
+    }
+    XTR_CHECK(hipStreamSynchronize(s));
+    return total / ns;
+}
+
 struct Gmres {
     int64_t n;
     int m;
@@ -404,6 +476,7 @@ struct Gmres {
     int backward_check = 0;
     SolveCtx *ctx;
     Gmres *inner = nullptr;
+    xtr_net *net = nullptr;       // learned right preconditioner in place of the inner GMRES
     DevBuf<double2> V, Z, w, r0;
     std::vector<cd> H;      // (m+1) x (m+1), column-major (FortranMatrix.cs:11-52)
     std::vector<double> gc;
@@ -485,9 +558,20 @@ struct Gmres {
     void precond_and_apply(int jH)
     {
         if (flexible) {
-            double bea, be;
-            int conv;
-            inner->solve(col(V, jH), col(V, jH), col(Z, jH), bea, be, conv);   // FlexibleGmresWithGmresPreconditioner.cs:57
+            if (net) {
+                net_precondition(net, col(V, jH), col(Z, jH), ctx->stream);
+            } else {
+                double bea, be;
+                int conv;
+                inner->solve(col(V, jH), col(V, jH), col(Z, jH), bea, be, conv);   // FlexibleGmresWithGmresPreconditioner.cs:57
+            }
+            if (ctx->harvest_count < ctx->harvest_cap) {
+                const int k = ctx->harvest_count++;
+                XTR_CHECK(hipMemcpyAsync(ctx->harvest_v[k], col(V, jH), n * sizeof(double2), hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+                XTR_CHECK(hipMemcpyAsync(ctx->harvest_z[k], col(Z, jH), n * sizeof(double2), hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+            }
             matvec(col(Z, jH), w.p);
         } else {
             matvec(col(V, jH), w.p);
@@ -612,21 +696,25 @@ struct Gmres {
 
 // FlexibleGmresWithGmresPreconditioner (:29-58) on one right-hand side.
 void run_fgmres(int64_t n, const xtr_gmres_params &p, SolveCtx &ctx, const double2 *b, double2 *x,
-                xtr_result_info &info)
+                xtr_result_info &info, xtr_net *net = nullptr)
 {
     require(p.outer_buffer_length >= 1 && p.outer_buffer_length <= 31, "OuterBufferLength must be 1..31");
     require(p.inner_buffer_length >= 1 && p.inner_buffer_length <= 31, "InnerBufferLength must be 1..31");
     require(p.max_repeat_number >= 1, "MaxRepeatsNumber must be >= 1");
     auto t0 = std::chrono::steady_clock::now();
-    Gmres inner(n, p.inner_buffer_length, false, &ctx);
-    inner.tol = p.tolerance;
-    inner.max_repeat = 1;
-    inner.backward_check = 0;
+    std::unique_ptr<Gmres> inner;
+    if (!net) {
+        inner = std::make_unique<Gmres>(n, p.inner_buffer_length, false, &ctx);
+        inner->tol = p.tolerance;
+        inner->max_repeat = 1;
+        inner->backward_check = 0;
+    }
     Gmres outer(n, p.outer_buffer_length, true, &ctx);
     outer.tol = p.tolerance;
     outer.max_repeat = p.max_repeat_number;
     outer.backward_check = p.backward_error_checking;
-    outer.inner = &inner;
+    outer.inner = inner.get();
+    outer.net = net;
     double bea = -1, be = -1;
     int conv = 0;
     outer.solve(b, x, x, bea, be, conv);
@@ -1383,3 +1471,183 @@ int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *fi
 }
 
 }  // extern "C"
+
+// ---- learned right preconditioner ----------------------------------------------------------------
+
+int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **out)
+{
+    return guarded([&] {
+        require(out && nx >= 1 && ny >= 1 && nz >= 1, "invalid network shape");
+        *out = nullptr;
+        auto net = std::make_unique<xtr_net>();
+        net->g = xtr::NetGeom{nx, ny, nz, (int64_t)nx * ny * nz};
+        const int64_t c = net->g.cells;
+        XTR_CHECK(hipStreamCreateWithFlags(&net->stream, hipStreamDefault));
+        net->x0.alloc(c * 8);
+        net->h.alloc(c * 16);
+        net->dyb.alloc(c * 8);
+        net->dpb.alloc(c * 16);
+        net->pre.alloc(c * 16);
+        net->y.alloc(c * 16);
+        net->tgt.alloc(c * 8);
+        net->dy.alloc(c * 8);
+        net->dh.alloc(c * 16);
+        net->dp.alloc(c * 16);
+        net->part.alloc((size_t)xtr::net_wgrad_chunks(net->g) * xtr::kNetW2);
+        net->W1.alloc(xtr::kNetKpad8 * 16);
+        net->W2.alloc(27 * 16 * 16);
+        net->WT2.alloc(xtr::kNetKpad8 * 16);
+        for (auto *d : {&net->W1, &net->WT2})
+            XTR_CHECK(hipMemset(d->p, 0, xtr::kNetKpad8 * 16 * sizeof(unsigned short)));   // rows 216..223 stay 0
+        for (auto *d : {&net->w1, &net->m1, &net->s1, &net->g1}) {
+            d->alloc(xtr::kNetW1);
+            XTR_CHECK(hipMemset(d->p, 0, xtr::kNetW1 * sizeof(float)));
+        }
+        for (auto *d : {&net->w2, &net->m2, &net->s2, &net->g2}) {
+            d->alloc(xtr::kNetW2);
+            XTR_CHECK(hipMemset(d->p, 0, xtr::kNetW2 * sizeof(float)));
+        }
+        net->partial_r.alloc(xtr::kRedBlocks);
+        net->nn.alloc(1);
+        net->lpart.alloc(2 * (size_t)xtr::net_loss_blocks(net->g));
+        // layer 1: He-scaled normal values (6 live input channels x 27 taps) from SplitMix64 +
+        // Box-Muller; layer 2: zero, so the untrained network is the identity preconditioner
+        std::vector<float> w1(xtr::kNetW1, 0.0f);
+        uint64_t st = seed;
+        auto next = [&st]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+        };
+        const double sd = std::sqrt(2.0 / (27.0 * 6.0));
+        for (int e = 0; e < xtr::kNetW1; ++e) {
+            const double u1 = next(), u2 = next();
+            const int ci = (e / 16) % 8;
+            w1[e] = ci < 6 ? (float)(sd * std::sqrt(-2.0 * std::log(u1 > 0 ? u1 : 1e-300)) * std::cos(2 * M_PI * u2)) : 0.0f;
+        }
+        XTR_CHECK(hipMemcpy(net->w1.p, w1.data(), w1.size() * sizeof(float), hipMemcpyHostToDevice));
+        xtr::net_weights(net->w1.p, net->w2.p, net->bufs(), net->stream);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+        *out = net.release();
+    });
+}
+
+void xtr_net_destroy(xtr_net *net) { delete net; }
+
+int xtr_net_set_weights(xtr_net *net, const float *w1, const float *w2)
+{
+    return guarded([&] {
+        require(net && w1 && w2, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipMemcpy(net->w1.p, w1, xtr::kNetW1 * sizeof(float), hipMemcpyHostToDevice));
+        XTR_CHECK(hipMemcpy(net->w2.p, w2, xtr::kNetW2 * sizeof(float), hipMemcpyHostToDevice));
+        xtr::net_weights(net->w1.p, net->w2.p, net->bufs(), net->stream);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+    });
+}
+
+int xtr_net_get_weights(xtr_net *net, float *w1, float *w2)
+{
+    return guarded([&] {
+        require(net && w1 && w2, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+        XTR_CHECK(hipMemcpy(w1, net->w1.p, xtr::kNetW1 * sizeof(float), hipMemcpyDeviceToHost));
+        XTR_CHECK(hipMemcpy(w2, net->w2.p, xtr::kNetW2 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_net_apply_device(xtr_net *net, const void *v, void *z)
+{
+    return guarded([&] {
+        require(net && v && z, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
+        net_precondition(net, (const double2 *)v, (double2 *)z, net->stream);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+    });
+}
+
+int xtr_net_gradients_device(xtr_net *net, int nsamples, const void *const *v, const void *const *z, float *g1,
+                             float *g2, double *loss)
+{
+    return guarded([&] {
+        require(net && v && z && g1 && g2 && nsamples >= 1, "invalid arguments");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipDeviceSynchronize());
+        const double l = net_gradients(net, nsamples, (const double2 *const *)v, (const double2 *const *)z);
+        XTR_CHECK(hipMemcpy(g1, net->g1.p, xtr::kNetW1 * sizeof(float), hipMemcpyDeviceToHost));
+        XTR_CHECK(hipMemcpy(g2, net->g2.p, xtr::kNetW2 * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) *loss = l;
+    });
+}
+
+int xtr_net_train_device(xtr_net *net, int nsamples, const void *const *v, const void *const *z, float lr,
+                         double *loss)
+{
+    return guarded([&] {
+        require(net && v && z && nsamples >= 1 && lr > 0, "invalid arguments");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipDeviceSynchronize());
+        const double l = net_gradients(net, nsamples, (const double2 *const *)v, (const double2 *const *)z);
+        ++net->step;
+        xtr::net_adam(net->w1.p, net->m1.p, net->s1.p, net->g1.p, xtr::kNetW1, 1.0f, lr, net->step, net->stream);
+        xtr::net_adam(net->w2.p, net->m2.p, net->s2.p, net->g2.p, xtr::kNetW2, 1.0f, lr, net->step, net->stream);
+        xtr::net_weights(net->w1.p, net->w2.p, net->bufs(), net->stream);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+        if (loss) *loss = l;
+    });
+}
+
+int xtr_fgmres_solve_learned_device(xtr_op *op, const xtr_gmres_params *params, xtr_net *net, const void *b, void *x,
+                                    xtr_result_info *info)
+{
+    return guarded([&] {
+        require(op && params && net && b && x && info, "null argument");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        require(net->g.nx == op->g.nx && net->g.ny == op->g.ny && net->g.nz == op->g.nz,
+                "the network was made for another grid");
+        std::lock_guard<std::mutex> lk(net->mu);
+        XTR_CHECK(hipDeviceSynchronize());
+        SolveCtx ctx;
+        ctx.stream = op->stream;
+        xtr_op *o = op;
+        ctx.apply = [o](const double2 *xx, double2 *yy) {
+            xtr::RhsPtrs v{};
+            v.x[0] = xx;
+            v.y[0] = yy;
+            o->apply_raw(v, 1, o->stream);
+        };
+        std::memset(info, 0, sizeof(*info));
+        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info, net);
+    });
+}
+
+int xtr_fgmres_solve_harvest_device(xtr_op *op, const xtr_gmres_params *params, const void *b, void *x,
+                                    xtr_result_info *info, int capacity, void *const *v_out, void *const *z_out,
+                                    int *count)
+{
+    return guarded([&] {
+        require(op && params && b && x && info && count && capacity >= 0, "invalid arguments");
+        require(capacity == 0 || (v_out && z_out), "null harvest buffers");
+        require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
+        XTR_CHECK(hipDeviceSynchronize());
+        SolveCtx ctx;
+        ctx.stream = op->stream;
+        ctx.harvest_v = (double2 *const *)v_out;
+        ctx.harvest_z = (double2 *const *)z_out;
+        ctx.harvest_cap = capacity;
+        xtr_op *o = op;
+        ctx.apply = [o](const double2 *xx, double2 *yy) {
+            xtr::RhsPtrs v{};
+            v.x[0] = xx;
+            v.y[0] = yy;
+            o->apply_raw(v, 1, o->stream);
+        };
+        std::memset(info, 0, sizeof(*info));
+        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+        *count = ctx.harvest_count;
+    });
+}

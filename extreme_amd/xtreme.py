@@ -108,6 +108,20 @@ def lib():
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
+        f32p = C.POINTER(C.c_float)
+        vpp = C.POINTER(C.c_void_p)
+        L.xtr_net_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp]
+        L.xtr_net_destroy.argtypes = [C.c_void_p]
+        L.xtr_net_destroy.restype = None
+        L.xtr_net_set_weights.argtypes = [C.c_void_p, f32p, f32p]
+        L.xtr_net_get_weights.argtypes = [C.c_void_p, f32p, f32p]
+        L.xtr_net_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xtr_net_gradients_device.argtypes = [C.c_void_p, C.c_int, vpp, vpp, f32p, f32p, C.POINTER(C.c_double)]
+        L.xtr_net_train_device.argtypes = [C.c_void_p, C.c_int, vpp, vpp, C.c_float, C.POINTER(C.c_double)]
+        L.xtr_fgmres_solve_learned_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p,
+                                                      C.c_void_p, C.POINTER(_Info)]
+        L.xtr_fgmres_solve_harvest_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p,
+                                                      C.POINTER(_Info), C.c_int, vpp, vpp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -404,6 +418,27 @@ class AnomalyCurrentFgmresSolver:
                                           cb, None))
         return ResultInfo._from(info)
 
+    def solve_learned(self, b, x, net: "LearnedPreconditioner") -> ResultInfo:
+        """FGMRES with the learned network as the right preconditioner in place of the inner GMRES
+        (device tensors; x holds the initial guess)."""
+        info = _Info()
+        _check(lib().xtr_fgmres_solve_learned_device(self.op._h, C.byref(self.params), net._h, _ptr(b), _ptr(x),
+                                                     C.byref(info)))
+        return ResultInfo._from(info)
+
+    def solve_harvest(self, b, x, capacity: int):
+        """solve() on device tensors that also returns the first `capacity` pairs (v_j, z_j) of the
+        flexible preconditioner: the Krylov vector and the inner GMRES's result for it."""
+        torch = _torch()
+        vs = [torch.empty_like(b) for _ in range(capacity)]
+        zs = [torch.empty_like(b) for _ in range(capacity)]
+        vp = (C.c_void_p * max(capacity, 1))(*[v.data_ptr() for v in vs])
+        zp = (C.c_void_p * max(capacity, 1))(*[z.data_ptr() for z in zs])
+        info, cnt = _Info(), C.c_int()
+        _check(lib().xtr_fgmres_solve_harvest_device(self.op._h, C.byref(self.params), _ptr(b), _ptr(x), C.byref(info),
+                                                     capacity, vp, zp, C.byref(cnt)))
+        return ResultInfo._from(info), vs[:cnt.value], zs[:cnt.value]
+
     def solve_batch(self, bs, xs) -> list:
         """Side-by-side solves sharing every operator apply (device tensors)."""
         n = len(bs)
@@ -438,3 +473,68 @@ class AnomalyCurrentFgmresSolver:
         rp = _RefineParams(tol, float(inner_floor), int(max_refinements), self.params)
         _check(lib().xtr_refined_solve_batch_device(self.op._h, C.byref(rp), n, bp, xp, infos))
         return [RefineInfo._from(infos[i]) for i in range(n)]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class LearnedPreconditioner:
+    """A learned right preconditioner for the flexible FGMRES (no counterpart in the reference):
+    z = s (v/s + N(v/s)), s = rms(v), N a two-layer 3 x 3 x 3 convolution network over the grid's
+    cells with channels (Re, Im) x (x, y, z), BF16 MFMA convolutions, FP32 Adam. Weights:
+    w1[t, ci, co] (27, 8, 16), w2[t, ci, co] (27, 16, 8); tap t = offset (t//9 - 1, t//3 % 3 - 1,
+    t % 3 - 1) in (i, j, k); channel 2c + p = part p (Re, Im) of component c; channels 6, 7 unused."""
+
+    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().xtr_net_create(nx, ny, nz, C.c_uint64(seed), C.byref(h)))
+        self._h = h
+        self.shape = (nx, ny, nz)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().xtr_net_destroy(self._h)
+            self._h = None
+
+    def weights(self):
+        w1 = np.zeros((27, 8, 16), dtype=np.float32)
+        w2 = np.zeros((27, 16, 8), dtype=np.float32)
+        f = C.POINTER(C.c_float)
+        _check(lib().xtr_net_get_weights(self._h, w1.ctypes.data_as(f), w2.ctypes.data_as(f)))
+        return w1, w2
+
+    def set_weights(self, w1, w2):
+        w1 = np.ascontiguousarray(w1, dtype=np.float32).reshape(27, 8, 16)
+        w2 = np.ascontiguousarray(w2, dtype=np.float32).reshape(27, 16, 8)
+        f = C.POINTER(C.c_float)
+        _check(lib().xtr_net_set_weights(self._h, w1.ctypes.data_as(f), w2.ctypes.data_as(f)))
+
+    def apply(self, v, out=None):
+        """z = preconditioner(v) for a complex128 device vector of 3 nx ny nz values."""
+        if out is None:
+            out = v.new_empty(v.shape)
+        _check(lib().xtr_net_apply_device(self._h, _ptr(v), _ptr(out)))
+        return out
+
+    @staticmethod
+    def _ptrs(ts):
+        return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def gradients(self, vs, zs):
+        """(dL/dw1, dL/dw2, L) for the mean relative error of the correction over the pairs."""
+        g1 = np.zeros((27, 8, 16), dtype=np.float32)
+        g2 = np.zeros((27, 16, 8), dtype=np.float32)
+        loss = C.c_double()
+        f = C.POINTER(C.c_float)
+        _check(lib().xtr_net_gradients_device(self._h, len(vs), self._ptrs(vs), self._ptrs(zs), g1.ctypes.data_as(f),
+                                              g2.ctypes.data_as(f), C.byref(loss)))
+        return g1, g2, loss.value
+
+    def train_step(self, vs, zs, lr: float = 1e-3) -> float:
+        """One Adam step on the pairs; returns the loss before the step."""
+        loss = C.c_double()
+        _check(lib().xtr_net_train_device(self._h, len(vs), self._ptrs(vs), self._ptrs(zs), C.c_float(lr),
+                                          C.byref(loss)))
+        return loss.value

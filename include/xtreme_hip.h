@@ -28,6 +28,7 @@ extern "C" {
 typedef struct { double re, im; } xtr_complex;
 
 typedef struct xtr_op xtr_op;
+typedef struct xtr_net xtr_net;   /* learned right preconditioner (see the end of this header) */
 
 /* Arithmetic of the operator apply. */
 enum {
@@ -255,6 +256,45 @@ void xtr_op_destroy(xtr_op *op);
 
 /* Message of the last failed call on this thread; returns its length. */
 int xtr_last_error(char *buf, int len);
+
+/* ---- learned right preconditioner (no counterpart in the reference) --------------------------------
+ * A two-layer 3-D convolution network on the residual field, z = s (v/s + N(v/s)) with s the RMS of
+ * v, N = conv(16 -> 6) o relu o conv(6 -> 16) over 3 x 3 x 3 cell neighbourhoods of the anomaly grid,
+ * channels (Re, Im) x (x, y, z); BF16 MFMA convolutions with FP32 accumulation, FP32 master weights
+ * and Adam. It stands in for the inner GMRES of FlexibleGmresWithGmresPreconditioner
+ * (Extreme.Fgmres/FlexibleGmresWithGmresPreconditioner.cs:29-63) through the flexible right
+ * preconditioner hook (FlexibleGmresSolver.cs:24-29). For callers that train and use such a
+ * network; the reference's inner GMRES stays the default. */
+
+/* A network for an nx x ny x nz grid: layer 1 seeded (He-scaled normal values), layer 2 zero, so
+ * that it starts as the identity preconditioner. */
+int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **net);
+void xtr_net_destroy(xtr_net *net);
+/* FP32 master weights: w1[(t*8 + ci)*16 + co] (27 taps t, input channels ci < 8 of which 6 are
+ * used, 16 hidden channels co), w2[(t*16 + ci)*8 + co] (16 hidden, 8 outputs of which 6 are used).
+ * Tap t is the offset (t/9 - 1, (t/3)%3 - 1, t%3 - 1) in (i, j, k); channel 2c + p is part p (Re, Im)
+ * of component c (x, y, z). */
+int xtr_net_set_weights(xtr_net *net, const float *w1, const float *w2);
+int xtr_net_get_weights(xtr_net *net, float *w1, float *w2);
+/* z = the preconditioner applied to v (device vectors of n = 3 nx ny nz complex values). */
+int xtr_net_apply_device(xtr_net *net, const void *v, void *z);
+/* Gradient of the mean over the samples of |N(v/s) - (z - v)/s|^2 / |(z - v)/s|^2 (relative
+ * squared error of the correction) with respect to w1 and w2, into host arrays, and the loss. */
+int xtr_net_gradients_device(xtr_net *net, int nsamples, const void *const *v, const void *const *z, float *g1,
+                             float *g2, double *loss);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8, rate lr) on that gradient; returns the loss before it. */
+int xtr_net_train_device(xtr_net *net, int nsamples, const void *const *v, const void *const *z, float lr,
+                         double *loss);
+/* Flexible FGMRES(OuterBufferLength) with the network as its right preconditioner in place of the
+ * inner GMRES; convergence as xtr_fgmres_solve_device (Arnoldi estimate, then the FP64 true residual). */
+int xtr_fgmres_solve_learned_device(xtr_op *op, const xtr_gmres_params *params, xtr_net *net, const void *b, void *x,
+                                    xtr_result_info *info);
+/* xtr_fgmres_solve_device that also copies the first `capacity` pairs (v_j, z_j) of the flexible
+ * preconditioner (v_j the Krylov vector, z_j the inner GMRES's result) into v_out[k], z_out[k]
+ * (device vectors of n values); *count = pairs written. The training data of the network. */
+int xtr_fgmres_solve_harvest_device(xtr_op *op, const xtr_gmres_params *params, const void *b, void *x,
+                                    xtr_result_info *info, int capacity, void *const *v_out, void *const *z_out,
+                                    int *count);
 
 #ifdef __cplusplus
 }

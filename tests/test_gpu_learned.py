@@ -1,0 +1,190 @@
+"""The learned right preconditioner (no counterpart in the reference): its GPU forward pass and
+gradients against a CPU FP64 restatement of the same network, training on the Krylov pairs the
+solver's own preconditioner calls produce, and what the trained network gains in a solve.
+
+The restatement rounds to BF16 exactly where the GPU does (inputs, weights, hidden activations, the
+backward pass's gradient copy), so the comparison is to FP32-vs-FP64 summation differences."""
+import time
+
+import numpy as np
+import pytest
+
+from extreme_amd import cartesian_model as cm
+from extreme_amd import xtreme as X
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round to BF16 (nearest even) through float32, back to float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+def _shift(x, t, sign=1):
+    """x[i + dx, j + dy, k + dz] with zero padding, (dx, dy, dz) = sign x offset of tap t."""
+    nx, ny, nz = x.shape[:3]
+    d = [sign * (t // 9 - 1), sign * (t // 3 % 3 - 1), sign * (t % 3 - 1)]
+    xp = np.pad(x, ((1, 1), (1, 1), (1, 1), (0, 0)))
+    return xp[1 + d[0]:1 + d[0] + nx, 1 + d[1]:1 + d[1] + ny, 1 + d[2]:1 + d[2] + nz]
+
+
+def _conv(x, w, sign=1):
+    return sum(_shift(x, t, sign) @ w[t] for t in range(27))
+
+
+def _channels(v, shape, s):
+    nx, ny, nz = shape
+    v4 = v.reshape(nx, ny, 3, nz)
+    x = np.zeros((nx, ny, nz, 8))
+    for c in range(3):
+        x[..., 2 * c] = v4[:, :, c, :].real
+        x[..., 2 * c + 1] = v4[:, :, c, :].imag
+    return x / s
+
+
+def _reference(shape, w1, w2, vs, zs):
+    """Forward outputs z, mean relative loss and its gradients, as the GPU computes them."""
+    n = 3 * np.prod(shape)
+    W1, W2 = bf16(w1), bf16(w2)
+    g1, g2, loss, outs = np.zeros_like(w1, dtype=np.float64), np.zeros_like(w2, dtype=np.float64), 0.0, []
+    for v, z in zip(vs, zs):
+        s = np.sqrt(np.vdot(v, v).real / n)
+        x0 = bf16(np.float32(_channels(v, shape, s)))
+        pre = _conv(x0, W1)
+        h = bf16(np.maximum(pre, 0))
+        y = np.float32(_conv(h, W2))
+        nx, ny, nz = shape
+        out = v.reshape(nx, ny, 3, nz).copy()
+        for c in range(3):
+            out[:, :, c, :] += s * (y[..., 2 * c] + 1j * y[..., 2 * c + 1])
+        outs.append(out.reshape(-1))
+        t = np.float32(_channels(z - v, shape, s))
+        d = np.float32(y - t)
+        d[..., 6:] = 0
+        st = float((t.astype(np.float64) ** 2).sum())
+        loss += float((d.astype(np.float64) ** 2).sum()) / st
+        dy = 2.0 * d.astype(np.float64)
+        scale = 1.0 / (st * len(vs))
+        g2 += scale * np.stack([_shift(h, t_).reshape(-1, 16).T @ dy.reshape(-1, 8) for t_ in range(27)])
+        dh = sum(_shift(bf16(np.float32(dy)), t_, -1) @ W2[t_].T for t_ in range(27))
+        dp = np.where(h > 0, dh, 0.0)
+        g1 += scale * np.stack([_shift(x0, t_).reshape(-1, 8).T @ dp.reshape(-1, 16) for t_ in range(27)])
+    return outs, loss / len(vs), g1, g2
+
+
+def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda):
+    torch = torch_cuda
+    shape = (5, 4, 6)                                   # 120 cells: a partial 16-cell tile
+    n = 3 * np.prod(shape)
+    rng = np.random.default_rng(7)
+    w1 = (rng.standard_normal((27, 8, 16)) * 0.2).astype(np.float32)
+    w1[:, 6:, :] = 0
+    w2 = (rng.standard_normal((27, 16, 8)) * 0.2).astype(np.float32)
+    vs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(2)]
+    zs = [v + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for v in vs]
+    net = X.LearnedPreconditioner(*shape, seed=3)
+    net.set_weights(w1, w2)
+    gw1, gw2 = net.weights()
+    assert np.array_equal(gw1, w1) and np.array_equal(gw2, w2)
+    dv = [torch.from_numpy(v).to("cuda") for v in vs]
+    dz = [torch.from_numpy(z).to("cuda") for z in zs]
+    outs, loss, g1, g2 = _reference(shape, w1.astype(np.float64), w2.astype(np.float64), vs, zs)
+    for v, o, ref in zip(dv, [net.apply(v) for v in dv], outs):
+        corr = np.linalg.norm(ref - v.cpu().numpy())
+        e = np.linalg.norm(o.cpu().numpy() - ref) / corr
+        print(f"forward: correction error {e:.2e}", flush=True)
+        assert e < 1e-5
+    G1, G2, L = net.gradients(dv, dz)
+    e1 = np.linalg.norm(G1 - g1) / np.linalg.norm(g1)
+    e2 = np.linalg.norm(G2 - g2) / np.linalg.norm(g2)
+    print(f"loss {L:.6f} vs {loss:.6f}; gradient errors w1 {e1:.2e} w2 {e2:.2e}", flush=True)
+    assert abs(L - loss) <= 1e-5 * loss
+    assert e1 < 1e-4 and e2 < 1e-4
+
+
+def _problem(model, pol):
+    import oracle.oracle as O
+    w = cm.omega_of(model.frequencies[0])
+    G = O.green_atoa(model, w)
+    z, z0 = model.zeta(w).reshape(-1), model.zeta0(w)
+    e0 = O.normal_field(model, w)
+    js = O.jscattered(model, pol, z, z0, e0)
+    oop = O.Operator(model, G)
+    oop.prepare("Chi0", z, z0)
+    chi0 = oop.apply(js)
+    return w, G, chi0
+
+
+def _c1_variant(seed):
+    """The C1 grid with a seeded random conductivity block (0.3..30 ohm-m) in place of the centre one."""
+    m = cm.config_c1()
+    rng = np.random.default_rng(seed)
+    i0, j0 = rng.integers(4, 20, size=2)
+    k0 = int(rng.integers(0, 4))
+    m.sigma[:] = 1e-2
+    m.sigma[i0:i0 + 8, j0:j0 + 8, k0:k0 + 4] = 10 ** rng.uniform(-1.5, 0.5)
+    return m
+
+
+def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
+    """Pairs (v_j, z_j) from FGMRES(30)/GMRES(10) solves of eight seeded models (both polarisations)
+    train the network; then a held-out model is solved with the reference's inner GMRES, with the
+    trained network, and with
+    no preconditioning (the untrained network, which is the identity). All reach the FP64 true
+    residual; the printout is the comparison."""
+    torch = torch_cuda
+    from extreme_amd import forward as F
+    s = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30, max_repeats_number=100)
+    pairs_v, pairs_z = [], []
+    for seed in range(8):
+        m = _c1_variant(seed)
+        p = F.FrequencySolve(m, m.frequencies[0], s)          # Green tensor and chi0 on the GPU
+        for pol in (0, 1):
+            b = p.chi0[pol].contiguous()
+            info, vs, zs = X.AnomalyCurrentFgmresSolver(p.op, s).solve_harvest(b, b.clone(), 16)
+            pairs_v += vs
+            pairs_z += zs
+        del p
+    print(f"{len(pairs_v)} training pairs", flush=True)
+    m = _c1_variant(99)
+    net = X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5)
+    first = None
+    rng = np.random.default_rng(1)
+    for step in range(300):
+        idx = rng.choice(len(pairs_v), size=8, replace=False)
+        loss = net.train_step([pairs_v[i] for i in idx], [pairs_z[i] for i in idx], lr=3e-3)
+        first = loss if first is None else first
+    _, _, final = net.gradients(pairs_v, pairs_z)
+    print(f"training loss {first:.4f} -> {final:.4f}", flush=True)
+    assert final < 0.9 * first
+    # the held-out model
+    w, G, chi0 = _problem(m, 0)
+    op = X.ConvolutionOperator(m, w, s)
+    op.set_green_tensor(G)
+    op.prepare_operator("A", m.zeta(w))
+    solver = X.AnomalyCurrentFgmresSolver(op, s)
+    b = torch.from_numpy(chi0).to("cuda")
+    results = {}
+    for name, run in (("inner GMRES(10)", lambda x: solver.solve(b, x)),
+                      ("learned", lambda x: solver.solve_learned(b, x, net)),
+                      ("identity", lambda x: solver.solve_learned(b, x, X.LearnedPreconditioner(m.nx, m.ny, m.nz)))):
+        x = b.clone()
+        run(x.clone())                                  # warm-up (FFT tile measurement, allocations)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = run(x)
+        torch.cuda.synchronize()
+        results[name] = (info, time.perf_counter() - t0, x.cpu().numpy())
+        print(f"{name}: converged {info.convergence_achieved}, outer iterations {info.number_of_iterations}, "
+              f"applies {info.matvecs}, true residual {info.backward_error:.2e}, {results[name][1] * 1e3:.1f} ms",
+              flush=True)
+        assert info.convergence_achieved and info.backward_error < 1e-8
+    ref = results["inner GMRES(10)"][2]
+    for name in ("learned", "identity"):
+        assert rel(results[name][2], ref) < 1e-6
+    gain = results["identity"][0].matvecs / results["learned"][0].matvecs
+    print(f"trained network vs none: {gain:.2f}x fewer operator applies; vs inner GMRES(10): "
+          f"{results['inner GMRES(10)'][1] / results['learned'][1]:.2f}x the time", flush=True)
