@@ -32,7 +32,7 @@ constexpr int radix_of(int len)
 }
 
 template <typename T, int R, int DIR>
-__device__ inline void dft_small(cplx<T> *a)
+__host__ __device__ inline void dft_small(cplx<T> *a)
 {
     if constexpr (R == 2) {
         cplx<T> t = a[0];
@@ -76,6 +76,48 @@ __device__ inline void dft_small(cplx<T> *a)
         a[0] = cadd<T>(a[0], s);
         a[1] = cadd<T>(m, id);
         a[2] = csub<T>(m, id);
+    }
+}
+
+// exp(DIR 2 pi i j / 16) applied to x, j in 0..15 known at compile time after unrolling. The
+// cosines of multiples of pi/8 are 1, cos(pi/8), sqrt(1/2), sin(pi/8), 0 and their mirrors.
+template <typename T, int DIR>
+__host__ __device__ inline cplx<T> rot16(cplx<T> x, int j)
+{
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    const double cs[16] = {1, c1, h, s1, 0, -s1, -h, -c1, -1, -c1, -h, -s1, 0, s1, h, c1};
+    j &= 15;
+    if (j == 0) return x;
+    if (j == 4) return DIR < 0 ? mk<T>(x.y, -x.x) : mk<T>(-x.y, x.x);       // DIR i
+    if (j == 8) return mk<T>(-x.x, -x.y);
+    if (j == 12) return DIR < 0 ? mk<T>(-x.y, x.x) : mk<T>(x.y, -x.x);      // -DIR i
+    const T c = (T)cs[j], sn = (T)(DIR < 0 ? -cs[(j + 12) & 15] : cs[(j + 12) & 15]);   // sin = cos(. - pi/2)
+    return mk<T>(x.x * c - x.y * sn, x.x * sn + x.y * c);
+}
+
+// In-register DFT of length R = 8 or 16 (sign DIR): n = (R/4) n1 + n2, k = k1 + 4 k2, a length-4
+// DFT over n1 for each n2, the twiddles exp(DIR 2 pi i n2 k1 / R), then a length-R/4 DFT over n2.
+template <typename T, int R, int DIR>
+__host__ __device__ inline void dft_pow2(cplx<T> *a)
+{
+    static_assert(R == 8 || R == 16, "dft_pow2 covers 8 and 16");
+    constexpr int R2 = R / 4;
+    cplx<T> b[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        cplx<T> t[4] = {a[n2], a[R2 + n2], a[2 * R2 + n2], a[3 * R2 + n2]};
+        dft_small<T, 4, DIR>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) b[n2 * 4 + k1] = rot16<T, DIR>(t[k1], n2 * k1 * (16 / R));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cplx<T> t[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) t[n2] = b[n2 * 4 + k1];
+        dft_small<T, R2, DIR>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) a[k1 + 4 * k2] = t[k2];
     }
 }
 
@@ -242,6 +284,90 @@ __global__ __launch_bounds__(NT) void pencil_fft_kernel(IO io, const cplx<T> *__
     }
 }
 
+// ---- register-resident pencils for lengths 64, 128, 256, 512 -------------------------------
+// The same Stockham stages with radix 16 or 8, so a length takes two or three stages: the first
+// reads its butterfly inputs straight from global memory (through the load policy), the last
+// writes its outputs straight to global memory, and only the exchanges between stages go through
+// the LDS tile (one exchange for 64, 128 and 256; two for 512 = 8 x 8 x 8).
+constexpr int reg_radix(int len)
+{
+    return (len == 512 || len == 64 || len % 16 != 0) ? 8 : 16;
+}
+constexpr bool reg_length(int p) { return p == 64 || p == 128 || p == 256 || p == 512; }
+template <int P, int TC>
+constexpr int reg_threads()
+{
+    const int n = TC * (P / reg_radix(P));
+    return n < 64 ? 64 : n > 512 ? 512 : n;
+}
+
+template <typename T, int P, int TC, int DIR, int LEN, int ST, int NT, bool FIRST, class IO>
+__device__ inline void reg_stage(const IO &io, int rhs, int64_t c0, cplx<T> *s, const cplx<T> *tw, int tid)
+{
+    constexpr int R = reg_radix(LEN);
+    constexpr int M = LEN / R;
+    constexpr int NB = (P / R) * TC;
+    constexpr int NPT = (NB + NT - 1) / NT;
+    constexpr bool LAST = M == 1;
+    static_assert(LEN % R == 0 && !(FIRST && LAST), "a register pencil has at least two stages");
+    cplx<T> y[NPT][R];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int b = tid + k * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int cl = b % TC, bb = b / TC;
+            const int p = bb / ST, q = bb % ST;
+            cplx<T> a[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const int pt = q + ST * (p + t * M);
+                if constexpr (FIRST) a[t] = io.load(rhs, c0 + cl, pt);
+                else a[t] = s[pt * TC + cl];
+            }
+            dft_pow2<T, R, DIR>(a);
+            if constexpr (LAST) {      // p == 0: no twiddles; point q + ST u of column cl
+#pragma unroll
+                for (int u = 0; u < R; ++u) io.store(rhs, c0 + cl, q + ST * u, a[u]);
+            } else {
+#pragma unroll
+                for (int u = 1; u < R; ++u) a[u] = cmul<T>(a[u], tw[p * u * (P / LEN)]);
+#pragma unroll
+                for (int u = 0; u < R; ++u) y[k][u] = a[u];
+            }
+        }
+    }
+    if constexpr (!LAST) {
+        if constexpr (!FIRST) __syncthreads();     // every read of this stage's inputs is done
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int b = tid + k * NT;
+            if (NB % NT == 0 || b < NB) {
+                const int cl = b % TC, bb = b / TC;
+                const int p = bb / ST, q = bb % ST;
+#pragma unroll
+                for (int u = 0; u < R; ++u) s[(q + ST * (R * p + u)) * TC + cl] = y[k][u];
+            }
+        }
+        __syncthreads();
+        reg_stage<T, P, TC, DIR, M, ST * R, NT, false>(io, rhs, c0, s, tw, tid);
+    }
+}
+
+template <typename T, int P, int TC, int DIR, class IO, int NT>
+__global__ __launch_bounds__(NT) void pencil_fft_reg_kernel(IO io, const cplx<T> *__restrict__ twg)
+{
+    static_assert(reg_length(P), "register pencils cover 64, 128, 256 and 512");
+    __shared__ cplx<T> s[P * TC];
+    __shared__ cplx<T> tw[P];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < P; k += NT) {
+        cplx<T> w = twg[k];
+        tw[k] = DIR < 0 ? w : cconj<T>(w);
+    }
+    __syncthreads();
+    reg_stage<T, P, TC, DIR, P, 1, NT, true>(io, blockIdx.y, (int64_t)blockIdx.x * TC, s, tw, tid);
+}
+
 // Column tiles per pencil length. The default tile has 16 columns while P x 16 stays within 32 KB,
 // else 8. For lengths from 64 there are two more: the narrow tile (half the columns, more workgroups
 // per CU) and the wide one (twice the columns, longer contiguous runs per row, within 128 KB); from
@@ -267,6 +393,15 @@ constexpr int tile_threads_of(int tile)
 {
     return tile == FFT_TILE_NARROW_T512 && P >= 128 ? 512 : kFftThreads;
 }
+// register pencils: the default column count, or the alternative (twice that within 64 KB of LDS,
+// else half)
+template <typename T, int P>
+constexpr int reg_cols_of(int tile)
+{
+    constexpr int d = tile_cols<T, P>();
+    if (tile != FFT_TILE_REG_ALT) return d;
+    return P * 2 * d * (int)sizeof(cplx<T>) <= 65536 ? 2 * d : d / 2;
+}
 
 template <typename T, int DIR, class IO>
 void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw, hipStream_t s, int tile)
@@ -283,6 +418,19 @@ void launch_pencil(int P, const IO &io, int64_t ncols, int nrhs, const void *tw,
                                (const cplx<T> *)tw);
         };
         using T256 = std::integral_constant<int, kFftThreads>;
+        if constexpr (reg_length(PP)) {
+            if (tile == FFT_TILE_REG || tile == FFT_TILE_REG_ALT) {
+                auto reg = [&](auto tc) {
+                    constexpr int TC = decltype(tc)::value, NT = reg_threads<PP, TC>();
+                    dim3 grid((unsigned)ceil_div(ncols, TC), nrhs);
+                    hipLaunchKernelGGL((pencil_fft_reg_kernel<T, PP, TC, DIR, IO, NT>), grid, dim3(NT), 0, s, io,
+                                       (const cplx<T> *)tw);
+                };
+                if (tile == FFT_TILE_REG) reg(std::integral_constant<int, reg_cols_of<T, PP>(FFT_TILE_REG)>{});
+                else reg(std::integral_constant<int, reg_cols_of<T, PP>(FFT_TILE_REG_ALT)>{});
+                return;
+            }
+        }
         if (tile == FFT_TILE_WIDE && TW != TD) run(std::integral_constant<int, TW>{}, T256{});
         else if (tile == FFT_TILE_NARROW && TN != TD) run(std::integral_constant<int, TN>{}, T256{});
         else if (tile == FFT_TILE_NARROW_T512 && TN != TD && NW != kFftThreads)
@@ -338,6 +486,10 @@ int fft_tile_options(int p, int precision, int *tiles)
     tiles[n++] = FFT_TILE_NARROW;                    // d / 2 >= 4 always
     if (p * 2 * d * e <= 131072) tiles[n++] = FFT_TILE_WIDE;
     if (p >= 128) tiles[n++] = FFT_TILE_NARROW_T512;
+    if (p == 64 || p == 128 || p == 256 || p == 512) {
+        tiles[n++] = FFT_TILE_REG;
+        tiles[n++] = FFT_TILE_REG_ALT;
+    }
     return n;
 }
 

@@ -215,10 +215,11 @@ struct xtr_op {
     int arith() const { return precision == XTR_MIXED ? active : precision; }
 
     // Column tiles of the four FFT passes for this apply's precision and grid: the
-    // XTR_FFT_TILES override (four letters d/n/w/m: default, narrow, wide, narrow with 512 threads),
-    // else the choice measured
-    // once per process and shape. Measuring runs each pass with each of its tiles on this apply's
-    // own buffers (the passes rewrite the spectrum and y, which the apply then recomputes from x).
+    // XTR_FFT_TILES override (four letters d/n/w/m/r/s: default, narrow, wide, narrow with 512
+    // threads, register-resident stages, register-resident stages with the other column count),
+    // else the choice measured once per process and shape. Measuring runs whole applies on this
+    // apply's own buffers (they rewrite the spectrum and y, which the apply then recomputes from
+    // x; x and y never overlap, which apply_raw checks).
     std::array<int, 4> fft_tiles(xtr::ApplyArgs &a, hipStream_t s)
     {
         const int P[4] = {2 * g.ny, 2 * g.nx, 2 * g.nx, 2 * g.ny};
@@ -229,6 +230,8 @@ struct xtr_op {
                 t[k] = c == 'w'   ? xtr::FFT_TILE_WIDE
                        : c == 'n' ? xtr::FFT_TILE_NARROW
                        : c == 'm' ? xtr::FFT_TILE_NARROW_T512
+                       : c == 'r' ? xtr::FFT_TILE_REG
+                       : c == 's' ? xtr::FFT_TILE_REG_ALT
                                   : xtr::FFT_TILE_DEFAULT;
             }
             return t;
@@ -240,29 +243,40 @@ struct xtr_op {
             if (it != tile_cache().end()) return it->second;
         }
         using Pass = void (*)(const xtr::ApplyArgs &, hipStream_t);
-        const Pass pass[4] = {xtr::launch_fwd_y, xtr::launch_fwd_x, xtr::launch_inv_x, xtr::launch_inv_y};
+        // the apply's five launches in order; the FFT passes are stages 0, 1, 3 and 4
+        const Pass stage[5] = {xtr::launch_fwd_y, xtr::launch_fwd_x, xtr::launch_contract, xtr::launch_inv_x,
+                               xtr::launch_inv_y};
+        const int stage_of_pass[4] = {0, 1, 3, 4};
         hipEvent_t e0, e1;
         XTR_CHECK(hipEventCreate(&e0));
         XTR_CHECK(hipEventCreate(&e1));
         std::array<int, 4> best{};
         for (int k = 0; k < 4; ++k) a.fft_tile[k] = xtr::FFT_TILE_DEFAULT;
         try {
+            // Each candidate is timed inside whole applies (one warm-up, three timed), so the
+            // contraction's stream of the Green tensor between the passes leaves the caches as a
+            // solve does; repeating one pass alone would time it on cache-resident data.
             for (int k = 0; k < 4; ++k) {
-                int opts[4];
+                int opts[8];
                 const int nopt = xtr::fft_tile_options(P[k], a.precision, opts);
                 float best_ms = 0;
                 for (int o = 0; o < nopt; ++o) {
                     const int t = opts[o];
                     a.fft_tile[k] = t;
-                    pass[k](a, s);                           // warm-up
-                    XTR_CHECK(hipEventRecord(e0, s));
-                    for (int r = 0; r < 3; ++r) pass[k](a, s);
-                    XTR_CHECK(hipEventRecord(e1, s));
-                    XTR_CHECK(hipEventSynchronize(e1));
-                    float ms = 0;
-                    XTR_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                    if (o == 0 || ms < best_ms) {
-                        best_ms = ms;
+                    float total = 0;
+                    for (int r = 0; r < 4; ++r) {
+                        for (int st = 0; st < 5; ++st) {
+                            if (st == stage_of_pass[k]) XTR_CHECK(hipEventRecord(e0, s));
+                            stage[st](a, s);
+                            if (st == stage_of_pass[k]) XTR_CHECK(hipEventRecord(e1, s));
+                        }
+                        XTR_CHECK(hipEventSynchronize(e1));
+                        float ms = 0;
+                        XTR_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                        if (r > 0) total += ms;
+                    }
+                    if (o == 0 || total < best_ms) {
+                        best_ms = total;
                         best[k] = t;
                     }
                 }
@@ -286,6 +300,14 @@ struct xtr_op {
         require(green_ready, "the Green tensor is not set");
         require(type == XTR_OP_A || type == XTR_OP_CHI0, "the operator is not prepared");
         require(nrhs >= 1 && nrhs <= max_rhs, "nrhs out of range");
+        // the last pass reads x while it writes y, element by element, and the tile measurement
+        // repeats that pass: y must not overlap any x
+        const size_t vb = (size_t)g.n() * sizeof(double2);
+        for (int r = 0; r < nrhs; ++r)
+            for (int q = 0; q < nrhs; ++q) {
+                const char *xb = (const char *)v.x[q], *yb = (const char *)v.y[r];
+                require(yb + vb <= xb || xb + vb <= yb, "the result vector overlaps an input vector");
+            }
         std::lock_guard<std::mutex> lk(apply_mu);
         const int p = arith();
         xtr::ApplyArgs a{};

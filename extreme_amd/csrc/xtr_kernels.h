@@ -107,8 +107,15 @@ struct ApplyArgs {
     RhsPtrs v;
 };
 
-enum { FFT_TILE_DEFAULT = 0, FFT_TILE_NARROW = 1, FFT_TILE_WIDE = 2, FFT_TILE_NARROW_T512 = 3 };
-// the column tiles a pencil length offers in this precision (FFT_TILE_*), into tiles[4]; returns how many
+enum {
+    FFT_TILE_DEFAULT = 0,
+    FFT_TILE_NARROW = 1,
+    FFT_TILE_WIDE = 2,
+    FFT_TILE_NARROW_T512 = 3,
+    FFT_TILE_REG = 4,        // register-resident stages (lengths 64..512), the default column count
+    FFT_TILE_REG_ALT = 5,    // the same with twice the columns (half where that exceeds 64 KB of LDS)
+};
+// the column tiles a pencil length offers in this precision (FFT_TILE_*), into tiles[8]; returns how many
 int fft_tile_options(int p, int precision, int *tiles);
 
 // Stage launchers (used separately for per-stage timing).

@@ -360,7 +360,7 @@ class ConvolutionOperator:
         _check(lib().xtr_op_stage_times(self._h, out, C.byref(n)))
         return dict(zip(self.STAGES, list(out))), n.value
 
-    FFT_TILES = ("default", "narrow", "wide", "narrow512")
+    FFT_TILES = ("default", "narrow", "wide", "narrow512", "reg", "reg_alt")
 
     def fft_tiles(self):
         """Column tile of each FFT pass (fwd_y, fwd_x, inv_x, inv_y) in the latest apply: chosen

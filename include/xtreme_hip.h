@@ -123,7 +123,7 @@ int xtr_op_prepare(xtr_op *op, int type, const xtr_complex *zeta);
 int xtr_op_prepare_models(xtr_op *op, int type, int nmodels, const xtr_complex *const *zeta);
 
 /* One apply of nrhs device vectors x[r] -> y[r], right-hand side r on model model[r]: the Green
- * tensor is streamed once for all of them. */
+ * tensor is streamed once for all of them. No y[r] may overlap any x[q]. */
 int xtr_op_apply_models_device(xtr_op *op, int nrhs, const int *model, const void *const *x, void *const *y);
 
 /* Replaces: ConvolutionOperator.Apply (ConvolutionOperator.cs:45-62,162-327).
@@ -131,7 +131,7 @@ int xtr_op_apply_models_device(xtr_op *op, int nrhs, const int *model, const voi
 int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs);
 
 /* Same on device-resident vectors (HIP device pointers), ordered on the handle's stream
- * and complete on return. */
+ * and complete on return. y must not overlap x (an error otherwise). */
 int xtr_op_apply_device(xtr_op *op, const void *x, void *y, int nrhs);
 
 /* Replaces: AnomalyCurrentFgmresSolver.Solve -> FlexibleGmresWithGmresPreconditioner.Solve
@@ -206,10 +206,12 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
 /* No counterpart in the reference (its FFTs are FFTW plans): the column tile each of the four FFT
  * passes (y forward, x forward, x inverse, y inverse) used in the latest apply, 0 = default,
  * 1 = narrow (half the columns), 2 = wide (twice the columns), 3 = narrow with 512 threads per
- * workgroup, -1 before the first apply. The tile is measured once per process for each precision
- * and grid, on the first apply of that shape (a few milliseconds at 256 x 256 x 64); the environment
- * variable XTR_FFT_TILES (four letters d, n, w, m) fixes it instead. The tile changes only how
- * columns are grouped into workgroups, not the arithmetic. */
+ * workgroup, 4 = register-resident radix-16/8 stages (pencil lengths 64, 128, 256, 512), 5 = the
+ * same with the other column count, -1 before the first apply. The tile is measured once per
+ * process for each precision and grid, on the first apply of that shape (a few milliseconds at
+ * 256 x 256 x 64); the environment variable XTR_FFT_TILES (four letters d, n, w, m, r, s) fixes it
+ * instead. Tiles 0-3 change only how columns are grouped into workgroups; tiles 4 and 5 also
+ * change the radix plan, so results move by rounding only. */
 int xtr_op_fft_tiles(xtr_op *op, int *tiles4);
 
 /* ---- observation level (AtoO) ----------------------------------------------------------------- */
