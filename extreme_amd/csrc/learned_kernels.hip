@@ -55,8 +55,9 @@ __device__ inline int64_t neighbour(const NetGeom &g, int i, int j, int k, int t
 
 // out[cell][co] (16 FP32 columns) = sum over taps t and input channels ci of
 // W[t CIN + ci][co] in[neighbour(cell, t)][ci]; in: BF16 [cells][CIN], W: BF16 [KPAD][16].
-// One wave per 16 cells; four waves per workgroup. RELU also writes relu(out) as BF16 [cells][16].
-template <int CIN, bool FLIP, bool RELU>
+// One wave per 16 cells; four waves per workgroup. ACT 1 also writes relu(out) as BF16 [cells][16],
+// ACT 2 writes out itself as BF16 (the linear network), ACT 0 nothing.
+template <int CIN, bool FLIP, int ACT>
 __global__ __launch_bounds__(256) void conv_kernel(NetGeom g, const unsigned short *__restrict__ in,
                                                    const unsigned short *__restrict__ W, float *__restrict__ out,
                                                    unsigned short *__restrict__ act)
@@ -89,7 +90,8 @@ __global__ __launch_bounds__(256) void conv_kernel(NetGeom g, const unsigned sho
         const int64_t oc = c0 + 4 * kg + r;
         if (oc < g.cells) {
             out[oc * 16 + row] = acc[r];
-            if (RELU) act[oc * 16 + row] = f2bf(acc[r] > 0 ? acc[r] : 0.0f);
+            if (ACT == 1) act[oc * 16 + row] = f2bf(acc[r] > 0 ? acc[r] : 0.0f);
+            if (ACT == 2) act[oc * 16 + row] = f2bf(acc[r]);
         }
     }
 }
@@ -181,13 +183,14 @@ __global__ __launch_bounds__(256) void loss_kernel(NetGeom g, const float *__res
     }
 }
 
-// dH (FP32 [cells][16]) masked by relu'(pre) (h > 0), as FP32 and BF16
+// dH (FP32 [cells][16]) masked by relu'(pre) (h > 0), or passed through for the linear network,
+// as FP32 and BF16
 __global__ void relu_back_kernel(int64_t n, const float *__restrict__ dh, const unsigned short *__restrict__ h,
-                                 float *__restrict__ dp, unsigned short *__restrict__ dpb)
+                                 float *__restrict__ dp, unsigned short *__restrict__ dpb, int linear)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const float v = bf2f(h[e]) > 0 ? dh[e] : 0.0f;
+    const float v = (linear || bf2f(h[e]) > 0) ? dh[e] : 0.0f;
     dp[e] = v;
     dpb[e] = f2bf(v);
 }
@@ -292,9 +295,12 @@ static unsigned grid_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 void net_forward(const NetGeom &g, const NetBuffers &b, hipStream_t s)
 {
     const unsigned tiles = grid_of(grid_of(g.cells, 16), 4);
-    hipLaunchKernelGGL((conv_kernel<8, false, true>), dim3(tiles), dim3(256), 0, s, g, b.x0, b.W1, b.pre, b.h);
+    if (g.linear)
+        hipLaunchKernelGGL((conv_kernel<8, false, 2>), dim3(tiles), dim3(256), 0, s, g, b.x0, b.W1, b.pre, b.h);
+    else
+        hipLaunchKernelGGL((conv_kernel<8, false, 1>), dim3(tiles), dim3(256), 0, s, g, b.x0, b.W1, b.pre, b.h);
     XTR_LAUNCH_CHECK("conv_kernel (layer 1)");
-    hipLaunchKernelGGL((conv_kernel<16, false, false>), dim3(tiles), dim3(256), 0, s, g, b.h, b.W2, b.y,
+    hipLaunchKernelGGL((conv_kernel<16, false, 0>), dim3(tiles), dim3(256), 0, s, g, b.h, b.W2, b.y,
                        (unsigned short *)nullptr);
     XTR_LAUNCH_CHECK("conv_kernel (layer 2)");
 }
@@ -335,11 +341,11 @@ void net_backward(const NetGeom &g, const NetBuffers &b, float *g1, float *g2, f
     XTR_LAUNCH_CHECK("wgrad_reduce_kernel (layer 2)");
     // into the hidden layer: dH[cell][ci] = sum W2[t 16 + ci][co] dY[cell - off t][co], then relu'
     const unsigned tiles = grid_of(grid_of(g.cells, 16), 4);
-    hipLaunchKernelGGL((conv_kernel<8, true, false>), dim3(tiles), dim3(256), 0, s, g, b.dyb, b.WT2, b.dh,
+    hipLaunchKernelGGL((conv_kernel<8, true, 0>), dim3(tiles), dim3(256), 0, s, g, b.dyb, b.WT2, b.dh,
                        (unsigned short *)nullptr);
     XTR_LAUNCH_CHECK("conv_kernel (backward)");
     hipLaunchKernelGGL(relu_back_kernel, dim3(grid_of(g.cells * 16, 256)), dim3(256), 0, s, g.cells * 16, b.dh, b.h,
-                       b.dp, b.dpb);
+                       b.dp, b.dpb, g.linear);
     XTR_LAUNCH_CHECK("relu_back_kernel");
     // layer 1: dW1[t 8 + ci][co] = sum x0[nb][ci] dP[cell][co]
     hipLaunchKernelGGL((wgrad_kernel<8, 16, 16>), dim3(nchunk), dim3(256), 0, s, g, chunk, b.x0, b.dp, b.part);

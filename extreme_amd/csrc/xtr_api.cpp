@@ -1502,7 +1502,7 @@ int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **out)
         require(out && nx >= 1 && ny >= 1 && nz >= 1, "invalid network shape");
         *out = nullptr;
         auto net = std::make_unique<xtr_net>();
-        net->g = xtr::NetGeom{nx, ny, nz, (int64_t)nx * ny * nz};
+        net->g = xtr::NetGeom{nx, ny, nz, (int64_t)nx * ny * nz, 0};
         const int64_t c = net->g.cells;
         XTR_CHECK(hipStreamCreateWithFlags(&net->stream, hipStreamDefault));
         net->x0.alloc(c * 8);
@@ -1556,6 +1556,16 @@ int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **out)
 }
 
 void xtr_net_destroy(xtr_net *net) { delete net; }
+
+int xtr_net_set_linear(xtr_net *net, int linear)
+{
+    return guarded([&] {
+        require(net, "null argument");
+        require(linear == 0 || linear == 1, "linear must be 0 or 1");
+        std::lock_guard<std::mutex> lk(net->mu);
+        net->g.linear = linear;
+    });
+}
 
 int xtr_net_set_weights(xtr_net *net, const float *w1, const float *w2)
 {

@@ -8,6 +8,7 @@ namespace xtr {
 struct NetGeom {
     int nx, ny, nz;
     int64_t cells;   // nx ny nz
+    int linear;      // 1: the hidden layer has no relu (the network is a fixed linear map)
 };
 
 // Per-network device buffers; sizes in elements.

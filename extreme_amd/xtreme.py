@@ -111,6 +111,7 @@ def lib():
         f32p = C.POINTER(C.c_float)
         vpp = C.POINTER(C.c_void_p)
         L.xtr_net_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp]
+        L.xtr_net_set_linear.argtypes = [C.c_void_p, C.c_int]
         L.xtr_net_destroy.argtypes = [C.c_void_p]
         L.xtr_net_destroy.restype = None
         L.xtr_net_set_weights.argtypes = [C.c_void_p, f32p, f32p]
@@ -485,13 +486,17 @@ class LearnedPreconditioner:
     z = s (v/s + N(v/s)), s = rms(v), N a two-layer 3 x 3 x 3 convolution network over the grid's
     cells with channels (Re, Im) x (x, y, z), BF16 MFMA convolutions, FP32 Adam. Weights:
     w1[t, ci, co] (27, 8, 16), w2[t, ci, co] (27, 16, 8); tap t = offset (t//9 - 1, t//3 % 3 - 1,
-    t % 3 - 1) in (i, j, k); channel 2c + p = part p (Re, Im) of component c; channels 6, 7 unused."""
+    t % 3 - 1) in (i, j, k); channel 2c + p = part p (Re, Im) of component c; channels 6, 7 unused.
+    linear=True drops the hidden layer's relu: the preconditioner is then a fixed linear map."""
 
-    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1):
+    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1, linear: bool = False):
         h = C.c_void_p()
         _check(lib().xtr_net_create(nx, ny, nz, C.c_uint64(seed), C.byref(h)))
         self._h = h
         self.shape = (nx, ny, nz)
+        self.linear = bool(linear)
+        if linear:
+            _check(lib().xtr_net_set_linear(self._h, 1))
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -272,6 +272,10 @@ int xtr_last_error(char *buf, int len);
  * that it starts as the identity preconditioner. */
 int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **net);
 void xtr_net_destroy(xtr_net *net);
+/* linear = 1 drops the relu of the hidden layer (forward and gradient): the preconditioner is then
+ * a fixed linear map z = v + N v, as the flexible FGMRES's Krylov relation prefers; 0 (the
+ * default) keeps it. Weights are kept; set it before training. */
+int xtr_net_set_linear(xtr_net *net, int linear);
 /* FP32 master weights: w1[(t*8 + ci)*16 + co] (27 taps t, input channels ci < 8 of which 6 are
  * used, 16 hidden channels co), w2[(t*16 + ci)*8 + co] (16 hidden, 8 outputs of which 6 are used).
  * Tap t is the offset (t/9 - 1, (t/3)%3 - 1, t%3 - 1) in (i, j, k); channel 2c + p is part p (Re, Im)

@@ -45,8 +45,9 @@ def _channels(v, shape, s):
     return x / s
 
 
-def _reference(shape, w1, w2, vs, zs):
-    """Forward outputs z, mean relative loss and its gradients, as the GPU computes them."""
+def _reference(shape, w1, w2, vs, zs, linear=False):
+    """Forward outputs z, mean relative loss and its gradients, as the GPU computes them (the
+    hidden layer with relu, or without it for the linear network)."""
     n = 3 * np.prod(shape)
     W1, W2 = bf16(w1), bf16(w2)
     g1, g2, loss, outs = np.zeros_like(w1, dtype=np.float64), np.zeros_like(w2, dtype=np.float64), 0.0, []
@@ -54,7 +55,7 @@ def _reference(shape, w1, w2, vs, zs):
         s = np.sqrt(np.vdot(v, v).real / n)
         x0 = bf16(np.float32(_channels(v, shape, s)))
         pre = _conv(x0, W1)
-        h = bf16(np.maximum(pre, 0))
+        h = bf16(np.float32(pre) if linear else np.maximum(pre, 0))
         y = np.float32(_conv(h, W2))
         nx, ny, nz = shape
         out = v.reshape(nx, ny, 3, nz).copy()
@@ -70,12 +71,13 @@ def _reference(shape, w1, w2, vs, zs):
         scale = 1.0 / (st * len(vs))
         g2 += scale * np.stack([_shift(h, t_).reshape(-1, 16).T @ dy.reshape(-1, 8) for t_ in range(27)])
         dh = sum(_shift(bf16(np.float32(dy)), t_, -1) @ W2[t_].T for t_ in range(27))
-        dp = np.where(h > 0, dh, 0.0)
+        dp = dh if linear else np.where(h > 0, dh, 0.0)
         g1 += scale * np.stack([_shift(x0, t_).reshape(-1, 8).T @ dp.reshape(-1, 16) for t_ in range(27)])
     return outs, loss / len(vs), g1, g2
 
 
-def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda):
+@pytest.mark.parametrize("linear", [False, True], ids=["relu", "linear"])
+def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda, linear):
     torch = torch_cuda
     shape = (5, 4, 6)                                   # 120 cells: a partial 16-cell tile
     n = 3 * np.prod(shape)
@@ -85,13 +87,13 @@ def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda):
     w2 = (rng.standard_normal((27, 16, 8)) * 0.2).astype(np.float32)
     vs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(2)]
     zs = [v + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for v in vs]
-    net = X.LearnedPreconditioner(*shape, seed=3)
+    net = X.LearnedPreconditioner(*shape, seed=3, linear=linear)
     net.set_weights(w1, w2)
     gw1, gw2 = net.weights()
     assert np.array_equal(gw1, w1) and np.array_equal(gw2, w2)
     dv = [torch.from_numpy(v).to("cuda") for v in vs]
     dz = [torch.from_numpy(z).to("cuda") for z in zs]
-    outs, loss, g1, g2 = _reference(shape, w1.astype(np.float64), w2.astype(np.float64), vs, zs)
+    outs, loss, g1, g2 = _reference(shape, w1.astype(np.float64), w2.astype(np.float64), vs, zs, linear)
     for v, o, ref in zip(dv, [net.apply(v) for v in dv], outs):
         corr = np.linalg.norm(ref - v.cpu().numpy())
         e = np.linalg.norm(o.cpu().numpy() - ref) / corr
@@ -131,10 +133,10 @@ def _c1_variant(seed):
 
 def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
     """Pairs (v_j, z_j) from FGMRES(30)/GMRES(10) solves of eight seeded models (both polarisations)
-    train the network; then a held-out model is solved with the reference's inner GMRES, with the
-    trained network, and with
-    no preconditioning (the untrained network, which is the identity). All reach the FP64 true
-    residual; the printout is the comparison."""
+    train two networks, one with the relu hidden layer and one linear; then a held-out model is
+    solved with the reference's inner GMRES, with each trained network, and with no preconditioning
+    (the untrained network, which is the identity). All reach the FP64 true residual; the printout
+    is the comparison."""
     torch = torch_cuda
     from extreme_amd import forward as F
     s = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30, max_repeats_number=100)
@@ -150,16 +152,18 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
         del p
     print(f"{len(pairs_v)} training pairs", flush=True)
     m = _c1_variant(99)
-    net = X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5)
-    first = None
-    rng = np.random.default_rng(1)
-    for step in range(300):
-        idx = rng.choice(len(pairs_v), size=8, replace=False)
-        loss = net.train_step([pairs_v[i] for i in idx], [pairs_z[i] for i in idx], lr=3e-3)
-        first = loss if first is None else first
-    _, _, final = net.gradients(pairs_v, pairs_z)
-    print(f"training loss {first:.4f} -> {final:.4f}", flush=True)
-    assert final < 0.9 * first
+    nets = {"learned": X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5),
+            "learned linear": X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5, linear=True)}
+    for name, net in nets.items():
+        first = None
+        rng = np.random.default_rng(1)
+        for step in range(300):
+            idx = rng.choice(len(pairs_v), size=8, replace=False)
+            loss = net.train_step([pairs_v[i] for i in idx], [pairs_z[i] for i in idx], lr=3e-3)
+            first = loss if first is None else first
+        _, _, final = net.gradients(pairs_v, pairs_z)
+        print(f"{name}: training loss {first:.4f} -> {final:.4f}", flush=True)
+        assert final < 0.9 * first
     # the held-out model
     w, G, chi0 = _problem(m, 0)
     op = X.ConvolutionOperator(m, w, s)
@@ -168,9 +172,10 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
     solver = X.AnomalyCurrentFgmresSolver(op, s)
     b = torch.from_numpy(chi0).to("cuda")
     results = {}
-    for name, run in (("inner GMRES(10)", lambda x: solver.solve(b, x)),
-                      ("learned", lambda x: solver.solve_learned(b, x, net)),
-                      ("identity", lambda x: solver.solve_learned(b, x, X.LearnedPreconditioner(m.nx, m.ny, m.nz)))):
+    runs = [("inner GMRES(10)", lambda x: solver.solve(b, x))]
+    runs += [(name, lambda x, net=net: solver.solve_learned(b, x, net)) for name, net in nets.items()]
+    runs += [("identity", lambda x: solver.solve_learned(b, x, X.LearnedPreconditioner(m.nx, m.ny, m.nz)))]
+    for name, run in runs:
         x = b.clone()
         run(x.clone())                                  # warm-up (FFT tile measurement, allocations)
         torch.cuda.synchronize()
@@ -183,8 +188,10 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
               flush=True)
         assert info.convergence_achieved and info.backward_error < 1e-8
     ref = results["inner GMRES(10)"][2]
-    for name in ("learned", "identity"):
+    for name in ("learned", "learned linear", "identity"):
         assert rel(results[name][2], ref) < 1e-6
-    gain = results["identity"][0].matvecs / results["learned"][0].matvecs
-    print(f"trained network vs none: {gain:.2f}x fewer operator applies; vs inner GMRES(10): "
-          f"{results['inner GMRES(10)'][1] / results['learned'][1]:.2f}x the time", flush=True)
+    for name in ("learned", "learned linear"):
+        gain = results["identity"][0].matvecs / results[name][0].matvecs
+        print(f"{name} vs none: {gain:.2f}x fewer operator applies, {results['identity'][1] / results[name][1]:.2f}x "
+              f"the speed; vs inner GMRES(10): {results['inner GMRES(10)'][1] / results[name][1]:.2f}x the speed",
+              flush=True)
