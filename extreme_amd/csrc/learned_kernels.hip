@@ -2,7 +2,8 @@
 // two-layer 3-D convolution network on the residual field, z = s (v/s + N(v/s)), s = rms(v),
 //   N = conv3(H -> 8) o relu o conv3(8 -> H),   3 x 3 x 3 taps, zero padding, no biases,
 // over the cells (i, j, k) of the anomaly grid with channels (Re, Im) x (x, y, z) of the vector
-// (two zero channels pad 6 to 8). Activations and weights are BF16 for the MFMAs, accumulation FP32.
+// plus two feature channels of the cell (log10 sigma, log10 f; zero when the caller sets none).
+// Activations and weights are BF16 for the MFMAs, accumulation FP32.
 //
 // The convolutions run as implicit GEMMs on v_mfma_f32_16x16x16_bf16: one wave computes 16 cells x
 // 16 output channels, K = 27 taps x C_in input channels in chunks of 16 (four consecutive channels
@@ -98,7 +99,8 @@ __global__ __launch_bounds__(256) void conv_kernel(NetGeom g, const unsigned sho
 
 // Input channels of v / s and the training target (z - v) / s, s = sqrt(nn / n); 0 when nn = 0.
 __global__ void pack_kernel(NetGeom g, const double2 *__restrict__ v, const double2 *__restrict__ z,
-                            const double *nn, unsigned short *__restrict__ x0, float *__restrict__ tgt)
+                            const double *nn, const float *__restrict__ feat, unsigned short *__restrict__ x0,
+                            float *__restrict__ tgt)
 {
     const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= g.cells) return;
@@ -117,8 +119,9 @@ __global__ void pack_kernel(NetGeom g, const double2 *__restrict__ v, const doub
             tgt[cell * 8 + 2 * c + 1] = (float)((b.y - a.y) * rs);
         }
     }
-    x0[cell * 8 + 6] = 0;
-    x0[cell * 8 + 7] = 0;
+    // channels 6 and 7: the cell's features (log10 sigma, log10 f), not scaled by s; 0 without them
+    x0[cell * 8 + 6] = feat ? f2bf(feat[2 * cell]) : 0;
+    x0[cell * 8 + 7] = feat ? f2bf(feat[2 * cell + 1]) : 0;
     if (tgt) {
         tgt[cell * 8 + 6] = 0;
         tgt[cell * 8 + 7] = 0;
@@ -307,7 +310,8 @@ void net_forward(const NetGeom &g, const NetBuffers &b, hipStream_t s)
 
 void net_pack(const NetGeom &g, const double2 *v, const double2 *z, const double *nn, const NetBuffers &b, hipStream_t s)
 {
-    hipLaunchKernelGGL(pack_kernel, dim3(grid_of(g.cells, 256)), dim3(256), 0, s, g, v, z, nn, b.x0, z ? b.tgt : nullptr);
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_of(g.cells, 256)), dim3(256), 0, s, g, v, z, nn, b.feat, b.x0,
+                       z ? b.tgt : nullptr);
     XTR_LAUNCH_CHECK("pack_kernel");
 }
 

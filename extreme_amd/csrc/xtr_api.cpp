@@ -432,6 +432,8 @@ struct xtr_net {
     DevBuf<float> pre, y, tgt, dy, dh, dp, part;
     DevBuf<float> w1, w2, m1, m2, s1, s2, g1, g2;   // FP32 master weights, Adam moments, gradients
     DevBuf<double> partial_r, nn, lpart;
+    DevBuf<float> feat;                               // [cells][2] feature channels, when set
+    bool has_feat = false;
     int step = 0;
     std::mutex mu;
 
@@ -441,7 +443,8 @@ struct xtr_net {
     }
     xtr::NetBuffers bufs()
     {
-        return xtr::NetBuffers{x0.p, h.p, pre.p, y.p, tgt.p, dy.p, dyb.p, dh.p, dp.p, dpb.p, part.p, W1.p, W2.p, WT2.p};
+        return xtr::NetBuffers{x0.p,  h.p,    pre.p, y.p,  tgt.p,  dy.p,  dyb.p,
+                               dh.p,  dp.p,   dpb.p, part.p, W1.p, W2.p, WT2.p, has_feat ? feat.p : nullptr};
     }
 };
 
@@ -1556,6 +1559,21 @@ int xtr_net_create(int nx, int ny, int nz, uint64_t seed, xtr_net **out)
 }
 
 void xtr_net_destroy(xtr_net *net) { delete net; }
+
+int xtr_net_set_features(xtr_net *net, const float *feat)
+{
+    return guarded([&] {
+        require(net, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        if (!feat) {
+            net->has_feat = false;
+            return;
+        }
+        if (!net->feat.p) net->feat.alloc((size_t)net->g.cells * 2);
+        XTR_CHECK(hipMemcpy(net->feat.p, feat, (size_t)net->g.cells * 2 * sizeof(float), hipMemcpyHostToDevice));
+        net->has_feat = true;
+    });
+}
 
 int xtr_net_set_linear(xtr_net *net, int linear)
 {

@@ -27,6 +27,7 @@ struct NetBuffers {
     unsigned short *W1;    // [224][16]   BF16 layer-1 weights (K = 27*8 rows, zero rows to 224: 14 MFMA chunks)
     unsigned short *W2;    // [27*16][16] BF16 layer-2 weights, columns 8..15 zero
     unsigned short *WT2;   // [224][16]   BF16 layer-2 weights transposed per tap (backward), zero rows to 224
+    const float *feat;     // [cells][2]  input channels 6, 7 (log10 sigma, log10 f), or null (zeros)
 };
 
 constexpr int kNetW1 = 27 * 8 * 16;    // layer-1 weights [t*8 + ci][co], ci < 8, co < 16

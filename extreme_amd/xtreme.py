@@ -112,6 +112,7 @@ def lib():
         vpp = C.POINTER(C.c_void_p)
         L.xtr_net_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp]
         L.xtr_net_set_linear.argtypes = [C.c_void_p, C.c_int]
+        L.xtr_net_set_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.xtr_net_destroy.argtypes = [C.c_void_p]
         L.xtr_net_destroy.restype = None
         L.xtr_net_set_weights.argtypes = [C.c_void_p, f32p, f32p]
@@ -486,7 +487,8 @@ class LearnedPreconditioner:
     z = s (v/s + N(v/s)), s = rms(v), N a two-layer 3 x 3 x 3 convolution network over the grid's
     cells with channels (Re, Im) x (x, y, z), BF16 MFMA convolutions, FP32 Adam. Weights:
     w1[t, ci, co] (27, 8, 16), w2[t, ci, co] (27, 16, 8); tap t = offset (t//9 - 1, t//3 % 3 - 1,
-    t % 3 - 1) in (i, j, k); channel 2c + p = part p (Re, Im) of component c; channels 6, 7 unused.
+    t % 3 - 1) in (i, j, k); channel 2c + p = part p (Re, Im) of component c; channels 6, 7 the cell's
+    features (log10 sigma, log10 f) once set_features is called, else zero.
     linear=True drops the hidden layer's relu: the preconditioner is then a fixed linear map."""
 
     def __init__(self, nx: int, ny: int, nz: int, seed: int = 1, linear: bool = False):
@@ -502,6 +504,19 @@ class LearnedPreconditioner:
         if getattr(self, "_h", None):
             lib().xtr_net_destroy(self._h)
             self._h = None
+
+    def set_features(self, sigma=None, frequency=None):
+        """Input channels 6 and 7 of every cell: log10 of the cell's conductivity sigma (S/m, shape
+        (nx, ny, nz)) and log10 of the frequency (Hz); None clears them to zeros."""
+        if sigma is None:
+            _check(lib().xtr_net_set_features(self._h, None))
+            return
+        sig = np.asarray(sigma, dtype=np.float64).reshape(self.shape)
+        feat = np.empty(sig.shape + (2,), dtype=np.float32)
+        feat[..., 0] = np.log10(sig)
+        feat[..., 1] = np.log10(frequency)
+        feat = np.ascontiguousarray(feat)
+        _check(lib().xtr_net_set_features(self._h, feat.ctypes.data_as(C.POINTER(C.c_float))))
 
     def weights(self):
         w1 = np.zeros((27, 8, 16), dtype=np.float32)

@@ -261,8 +261,10 @@ int xtr_last_error(char *buf, int len);
 
 /* ---- learned right preconditioner (no counterpart in the reference) --------------------------------
  * A two-layer 3-D convolution network on the residual field, z = s (v/s + N(v/s)) with s the RMS of
- * v, N = conv(16 -> 6) o relu o conv(6 -> 16) over 3 x 3 x 3 cell neighbourhoods of the anomaly grid,
- * channels (Re, Im) x (x, y, z); BF16 MFMA convolutions with FP32 accumulation, FP32 master weights
+ * v, N = conv(16 -> 6) o relu o conv(8 -> 16) over 3 x 3 x 3 cell neighbourhoods of the anomaly grid,
+ * input channels (Re, Im) x (x, y, z) and the cell's features (log10 sigma, log10 f; see
+ * xtr_net_set_features), output channels (Re, Im) x (x, y, z); BF16 MFMA convolutions with FP32
+ * accumulation, FP32 master weights
  * and Adam. It stands in for the inner GMRES of FlexibleGmresWithGmresPreconditioner
  * (Extreme.Fgmres/FlexibleGmresWithGmresPreconditioner.cs:29-63) through the flexible right
  * preconditioner hook (FlexibleGmresSolver.cs:24-29). For callers that train and use such a
@@ -276,8 +278,13 @@ void xtr_net_destroy(xtr_net *net);
  * a fixed linear map z = v + N v, as the flexible FGMRES's Krylov relation prefers; 0 (the
  * default) keeps it. Weights are kept; set it before training. */
 int xtr_net_set_linear(xtr_net *net, int linear);
-/* FP32 master weights: w1[(t*8 + ci)*16 + co] (27 taps t, input channels ci < 8 of which 6 are
- * used, 16 hidden channels co), w2[(t*16 + ci)*8 + co] (16 hidden, 8 outputs of which 6 are used).
+/* Input channels 6 and 7 of every cell: feat[2*cell] and feat[2*cell + 1] (host, [nx][ny][nz][2];
+ * log10 of the cell's conductivity and log10 of the frequency), used by every later apply and
+ * training step until set again; null returns to zeros (the default). Set the features of the
+ * model a sample comes from before training on it, and of the model being solved before a solve. */
+int xtr_net_set_features(xtr_net *net, const float *feat);
+/* FP32 master weights: w1[(t*8 + ci)*16 + co] (27 taps t, input channels ci < 8: six of the vector,
+ * two features, 16 hidden channels co), w2[(t*16 + ci)*8 + co] (16 hidden, 8 outputs of which 6 are used).
  * Tap t is the offset (t/9 - 1, (t/3)%3 - 1, t%3 - 1) in (i, j, k); channel 2c + p is part p (Re, Im)
  * of component c (x, y, z). */
 int xtr_net_set_weights(xtr_net *net, const float *w1, const float *w2);

@@ -35,17 +35,21 @@ def _conv(x, w, sign=1):
     return sum(_shift(x, t, sign) @ w[t] for t in range(27))
 
 
-def _channels(v, shape, s):
+def _channels(v, shape, s, feat=None):
+    """Input channels: the vector's (Re, Im) x (x, y, z) over s, then the cell features unscaled."""
     nx, ny, nz = shape
     v4 = v.reshape(nx, ny, 3, nz)
     x = np.zeros((nx, ny, nz, 8))
     for c in range(3):
         x[..., 2 * c] = v4[:, :, c, :].real
         x[..., 2 * c + 1] = v4[:, :, c, :].imag
-    return x / s
+    x /= s
+    if feat is not None:
+        x[..., 6:] = feat
+    return x
 
 
-def _reference(shape, w1, w2, vs, zs, linear=False):
+def _reference(shape, w1, w2, vs, zs, linear=False, feat=None):
     """Forward outputs z, mean relative loss and its gradients, as the GPU computes them (the
     hidden layer with relu, or without it for the linear network)."""
     n = 3 * np.prod(shape)
@@ -53,7 +57,7 @@ def _reference(shape, w1, w2, vs, zs, linear=False):
     g1, g2, loss, outs = np.zeros_like(w1, dtype=np.float64), np.zeros_like(w2, dtype=np.float64), 0.0, []
     for v, z in zip(vs, zs):
         s = np.sqrt(np.vdot(v, v).real / n)
-        x0 = bf16(np.float32(_channels(v, shape, s)))
+        x0 = bf16(np.float32(_channels(v, shape, s, feat)))
         pre = _conv(x0, W1)
         h = bf16(np.float32(pre) if linear else np.maximum(pre, 0))
         y = np.float32(_conv(h, W2))
@@ -76,24 +80,35 @@ def _reference(shape, w1, w2, vs, zs, linear=False):
     return outs, loss / len(vs), g1, g2
 
 
-@pytest.mark.parametrize("linear", [False, True], ids=["relu", "linear"])
-def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda, linear):
+@pytest.mark.parametrize("linear, features", [(False, False), (False, True), (True, True)],
+                         ids=["relu", "relu_features", "linear_features"])
+def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda, linear, features):
     torch = torch_cuda
     shape = (5, 4, 6)                                   # 120 cells: a partial 16-cell tile
     n = 3 * np.prod(shape)
     rng = np.random.default_rng(7)
     w1 = (rng.standard_normal((27, 8, 16)) * 0.2).astype(np.float32)
-    w1[:, 6:, :] = 0
+    feat = None
+    if features:                                        # channels 6, 7: log10 sigma, log10 f
+        feat = np.stack([rng.uniform(-3, 0, shape), np.full(shape, -1.0)], axis=-1).astype(np.float32)
+    else:
+        w1[:, 6:, :] = 0
     w2 = (rng.standard_normal((27, 16, 8)) * 0.2).astype(np.float32)
     vs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(2)]
     zs = [v + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for v in vs]
     net = X.LearnedPreconditioner(*shape, seed=3, linear=linear)
     net.set_weights(w1, w2)
+    if features:
+        sigma = 10.0 ** feat[..., 0].astype(np.float64)
+        net.set_features(sigma, 0.1)
+        # the channels exactly as set_features forms them (float32 of log10 in float64)
+        feat = np.stack([np.log10(sigma), np.full(shape, np.log10(0.1))], axis=-1).astype(np.float32)
     gw1, gw2 = net.weights()
     assert np.array_equal(gw1, w1) and np.array_equal(gw2, w2)
     dv = [torch.from_numpy(v).to("cuda") for v in vs]
     dz = [torch.from_numpy(z).to("cuda") for z in zs]
-    outs, loss, g1, g2 = _reference(shape, w1.astype(np.float64), w2.astype(np.float64), vs, zs, linear)
+    outs, loss, g1, g2 = _reference(shape, w1.astype(np.float64), w2.astype(np.float64), vs, zs, linear,
+                                    None if feat is None else feat.astype(np.float64))
     for v, o, ref in zip(dv, [net.apply(v) for v in dv], outs):
         corr = np.linalg.norm(ref - v.cpu().numpy())
         e = np.linalg.norm(o.cpu().numpy() - ref) / corr
@@ -133,37 +148,58 @@ def _c1_variant(seed):
 
 def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
     """Pairs (v_j, z_j) from FGMRES(30)/GMRES(10) solves of eight seeded models (both polarisations)
-    train two networks, one with the relu hidden layer and one linear; then a held-out model is
-    solved with the reference's inner GMRES, with each trained network, and with no preconditioning
-    (the untrained network, which is the identity). All reach the FP64 true residual; the printout
-    is the comparison."""
+    train four networks: relu or linear hidden layer, with or without the cell features (log10
+    sigma, log10 f) in input channels 6 and 7. Each training batch comes from one model, whose
+    features are set first. Then a held-out model is solved with the reference's inner GMRES, with
+    each trained network, and with no preconditioning (the untrained network, which is the
+    identity). All reach the FP64 true residual; the printout is the comparison."""
     torch = torch_cuda
     from extreme_amd import forward as F
     s = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30, max_repeats_number=100)
-    pairs_v, pairs_z = [], []
+    models, pairs = [], []                                 # pairs[k]: (vs, zs) of model k
     for seed in range(8):
         m = _c1_variant(seed)
         p = F.FrequencySolve(m, m.frequencies[0], s)          # Green tensor and chi0 on the GPU
+        vs_m, zs_m = [], []
         for pol in (0, 1):
             b = p.chi0[pol].contiguous()
             info, vs, zs = X.AnomalyCurrentFgmresSolver(p.op, s).solve_harvest(b, b.clone(), 16)
-            pairs_v += vs
-            pairs_z += zs
+            vs_m += vs
+            zs_m += zs
+        models.append(m)
+        pairs.append((vs_m, zs_m))
         del p
-    print(f"{len(pairs_v)} training pairs", flush=True)
+    print(f"{sum(len(v) for v, _ in pairs)} training pairs", flush=True)
     m = _c1_variant(99)
-    nets = {"learned": X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5),
-            "learned linear": X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5, linear=True)}
-    for name, net in nets.items():
-        first = None
+    variants = {"learned": (False, False), "learned linear": (True, False),
+                "learned, features": (False, True), "learned linear, features": (True, True)}
+    nets = {}
+    for name, (linear, features) in variants.items():
+        net = X.LearnedPreconditioner(m.nx, m.ny, m.nz, seed=5, linear=linear)
+
+        def loss_all():
+            tot = 0.0
+            for k, (vs_m, zs_m) in enumerate(pairs):
+                if features:
+                    net.set_features(models[k].sigma, models[k].frequencies[0])
+                tot += net.gradients(vs_m, zs_m)[2] * len(vs_m)
+            return tot / sum(len(v) for v, _ in pairs)
+
+        first = loss_all()
         rng = np.random.default_rng(1)
         for step in range(300):
-            idx = rng.choice(len(pairs_v), size=8, replace=False)
-            loss = net.train_step([pairs_v[i] for i in idx], [pairs_z[i] for i in idx], lr=3e-3)
-            first = loss if first is None else first
-        _, _, final = net.gradients(pairs_v, pairs_z)
+            k = int(rng.integers(len(pairs)))
+            vs_m, zs_m = pairs[k]
+            if features:
+                net.set_features(models[k].sigma, models[k].frequencies[0])
+            idx = rng.choice(len(vs_m), size=min(8, len(vs_m)), replace=False)
+            net.train_step([vs_m[i] for i in idx], [zs_m[i] for i in idx], lr=3e-3)
+        final = loss_all()
         print(f"{name}: training loss {first:.4f} -> {final:.4f}", flush=True)
         assert final < 0.9 * first
+        if features:
+            net.set_features(m.sigma, m.frequencies[0])       # the held-out model's
+        nets[name] = net
     # the held-out model
     w, G, chi0 = _problem(m, 0)
     op = X.ConvolutionOperator(m, w, s)
@@ -188,9 +224,9 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
               flush=True)
         assert info.convergence_achieved and info.backward_error < 1e-8
     ref = results["inner GMRES(10)"][2]
-    for name in ("learned", "learned linear", "identity"):
+    for name in list(nets) + ["identity"]:
         assert rel(results[name][2], ref) < 1e-6
-    for name in ("learned", "learned linear"):
+    for name in nets:
         gain = results["identity"][0].matvecs / results[name][0].matvecs
         print(f"{name} vs none: {gain:.2f}x fewer operator applies, {results['identity'][1] / results[name][1]:.2f}x "
               f"the speed; vs inner GMRES(10): {results['inner GMRES(10)'][1] / results[name][1]:.2f}x the speed",
