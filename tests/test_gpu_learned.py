@@ -146,14 +146,20 @@ def _c1_variant(seed):
     return m
 
 
-def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
+def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda, monkeypatch):
     """Pairs (v_j, z_j) from FGMRES(30)/GMRES(10) solves of eight seeded models (both polarisations)
     train four networks: relu or linear hidden layer, with or without the cell features (log10
     sigma, log10 f) in input channels 6 and 7. Each training batch comes from one model, whose
     features are set first. Then a held-out model is solved with the reference's inner GMRES, with
     each trained network, and with no preconditioning (the untrained network, which is the
-    identity). All reach the FP64 true residual; the printout is the comparison."""
+    identity). All reach the FP64 true residual; the printout is the comparison.
+
+    The FFT tiles are fixed: the measured choice can differ between devices, and with it the
+    rounding of the harvested pairs, which training amplifies. The feature variants train at a
+    third of the rate: their constant log10 f channel acts as an unscaled bias, and at 3e-3 the
+    linear one diverged on some devices."""
     torch = torch_cuda
+    monkeypatch.setenv("XTR_FFT_TILES", "dddd")
     from extreme_amd import forward as F
     s = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30, max_repeats_number=100)
     models, pairs = [], []                                 # pairs[k]: (vs, zs) of model k
@@ -193,7 +199,7 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda):
             if features:
                 net.set_features(models[k].sigma, models[k].frequencies[0])
             idx = rng.choice(len(vs_m), size=min(8, len(vs_m)), replace=False)
-            net.train_step([vs_m[i] for i in idx], [zs_m[i] for i in idx], lr=3e-3)
+            net.train_step([vs_m[i] for i in idx], [zs_m[i] for i in idx], lr=1e-3 if features else 3e-3)
         final = loss_all()
         print(f"{name}: training loss {first:.4f} -> {final:.4f}", flush=True)
         assert final < 0.9 * first
