@@ -161,6 +161,8 @@ class FrequencySolve:
         t0 = time.perf_counter()
         self.op.calculate_green_tensor()
         self.green_seconds = time.perf_counter() - t0
+        self.atoo_green_seconds = 0.0
+        self.atoo_field_seconds = [0.0, 0.0]
         e0 = normal_field(model, self.omega)
         self.js = [scattered_current(model, self.omega, pol, e0) for pol in (0, 1)]
         self.op.prepare_operator("Chi0")
@@ -183,15 +185,19 @@ class FrequencySolve:
         (x_shift, y_shift) metres)."""
         torch = self.torch
         op = self.op
+        t0 = time.perf_counter()
         op.calculate_green_atoo(level_z, electric=True, x_shift=x_shift, y_shift=y_shift)
         op.calculate_green_atoo(level_z, electric=False, x_shift=x_shift, y_shift=y_shift)
+        self.atoo_green_seconds += time.perf_counter() - t0          # AtoOGreenCalc, summed over levels
         normals = level_normal_fields(self.model.section, self.omega, level_z)
         E, H, Ea, Ha = [], [], [], []
         for pol in (0, 1):
+            t0 = time.perf_counter()
             js = torch.from_numpy(self.js[pol]).to("cuda")
             jq = op.jq(self.chi[pol], js)
             ea = op.level_field(jq, electric=True).cpu().numpy()
             ha = op.level_field(jq, electric=False).cpu().numpy()
+            self.atoo_field_seconds[pol] += time.perf_counter() - t0  # AtoOFields of this polarisation
             Ea.append(ea)
             Ha.append(ha)
             E.append(normals[pol][0][None, None, :] + ea)

@@ -14,6 +14,7 @@ from __future__ import annotations
 import math
 import os
 import re
+import time
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass, field
 
@@ -411,11 +412,42 @@ def read_result_table(path: str):
 
 # ---------------------------------------------------------------------------- driver
 
+COMMON_COLUMNS = (("Date", 15), ("Time", 12), ("nx", 6), ("ny", 6), ("nz", 6), ("nhank", 6), ("mpi_size", 9),
+                  ("nthreads", 9), ("AtoA_Green", 15), ("polX", 15), ("polY", 15), ("AtoO_Green", 15),
+                  ("AtoO_calc_X", 15), ("AtoO_calc_Y", 15))
+
+
+def common_dat_line(model: cm.CartesianModel, nhank: int, atoa_s: float, pol_s, atoo_s: float, atoo_calc_s,
+                    when=None, mpi_size: int = 1, nthreads: int = 1) -> str:
+    """One row of common.dat, as ProfilerResultsTextExporter.SaveProfilingResultsToCommon writes it
+    (EndUser/Profiling/ProfilerResultsTextExporter.cs:266-305): short date and time of the invariant
+    culture (MM/dd/yyyy, HH:mm), the grid, NumberOfHankels, MPI size and threads, then seconds as
+    "0.00000" for the AtoA Green tensor, SolveCie of each polarisation, the AtoO Green tensors and the
+    AtoO fields of each polarisation; every field left-aligned in its column width."""
+    import datetime
+    now = when or datetime.datetime.now()
+    vals = [now.strftime("%m/%d/%Y"), now.strftime("%H:%M"), str(model.nx), str(model.ny), str(model.nz),
+            str(nhank), str(mpi_size), str(nthreads)]
+    vals += [f"{t:.5f}" for t in (atoa_s, pol_s[0], pol_s[1], atoo_s, atoo_calc_s[0], atoo_calc_s[1])]
+    return "".join(v.ljust(w) for v, (_, w) in zip(vals, COMMON_COLUMNS)) + "\n"
+
+
+def append_common_dat(path: str, line: str) -> None:
+    """Appends a row, writing the header first when the file does not exist yet."""
+    if not os.path.exists(path):
+        with open(path, "a", newline="") as f:
+            f.write("".join(name.ljust(w) for name, w in COMMON_COLUMNS) + "\n")
+    with open(path, "a", newline="") as f:
+        f.write(line)
+
+
 def run_project(path: str, precision: str = "fp64", results_path: str | None = None, log=print) -> list:
     """ExtremeMtSolverRunner.Run (EndUser/ExtremeMt/ExtremeMtSolverRunner.cs:47-76) on the GPU:
     for each frequency, solve both polarisations, compute every observation level's fields and
-    write fields_freq* and responses_freq* into the results path. Observation sites (single points
-    off the level grid) are not built here."""
+    write fields_freq* and responses_freq* into the results path, and append the frequency's
+    timings to common.dat in the results path's parent (ExportProfiling, :114-141; one process and
+    one host thread driving the GPU). Observation sites (single points off the level grid) are not
+    built here."""
     from . import forward as F
     project = read_project(path)
     if project.model_file is None:
@@ -428,7 +460,9 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
     written = []
     for freq in project.frequencies:
         fs = F.FrequencySolve(model, freq, project.settings, precision=precision)
+        t0 = time.perf_counter()
         infos = fs.solve() if precision != "mixed" else fs.solve_mixed()
+        solve_s = time.perf_counter() - t0
         log(f"frequency {freq}: {[getattr(i, 'number_of_iterations', getattr(i, 'refinements', None)) for i in infos]}")
         levels = []
         for lev in project.levels:
@@ -442,5 +476,11 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
         export_raw_fields(fpath, model, levels, freq)
         export_mt_responses(rpath, model, levels, freq)
         written += [fpath, rpath]
+        # SolveCie per polarisation: each FP64 solve's own time (the two run concurrently); the
+        # mixed-precision pair is timed together, so both columns carry that time
+        pol_s = [i.seconds for i in infos] if precision != "mixed" else [solve_s, solve_s]
+        append_common_dat(os.path.join(os.path.dirname(os.path.abspath(out_dir)), "common.dat"),
+                          common_dat_line(model, project.settings.number_of_hankels, fs.green_seconds, pol_s,
+                                          fs.atoo_green_seconds, fs.atoo_field_seconds))
         del fs
     return written

@@ -60,6 +60,14 @@ def test_commemi_end_to_end(torch_cuda, tmp_path):
     # the fields table carries the blocks' anomalous fields (the last 24 columns)
     names_f, fields = P.read_result_table(files[0])
     assert fields.shape == (1600, 52) and np.abs(fields[:, 28:]).max() > 0
+    # the frequency's timings, appended to common.dat beside the results folder
+    rows = (tmp_path / "common.dat").read_text().splitlines()
+    assert len(rows) == 2 and rows[0].startswith("Date           Time        nx")
+    cols = rows[1].split()
+    assert cols[2:8] == ["40", "40", "40", "5", "1", "1"]
+    times = [float(c) for c in cols[8:]]
+    print(f"common.dat: {rows[1]}", flush=True)
+    assert len(times) == 6 and all(t > 0 for t in times)
 
 
 def test_commemi_reference_project(torch_cuda, tmp_path):
@@ -68,9 +76,10 @@ def test_commemi_reference_project(torch_cuda, tmp_path):
     agree to the written digits; tablets (x, -20 km) and (x, +20 km) are y-mirror images."""
     logs = []
     files = P.run_project(os.path.join(HERE, "golden", "commemi3d2", "forward.xproj"),
-                          results_path=str(tmp_path), log=logs.append)
+                          results_path=str(tmp_path / "results"), log=logs.append)
     print(logs, flush=True)
     assert len(files) == 6
+    assert len((tmp_path / "common.dat").read_text().splitlines()) == 1 + 3     # header + 3 frequencies
     proj = P.read_project(os.path.join(HERE, "golden", "commemi3d2", "forward.xproj"))
     shifts = [(l.x_shift, l.y_shift) for l in proj.levels]
     for f in files[1::2]:

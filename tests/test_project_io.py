@@ -148,3 +148,22 @@ def test_run_project_refuses_sites(tmp_path):
                  '</Observations></ExtremeProject>')
     with pytest.raises(NotImplementedError):
         P.run_project(str(f))
+
+
+def test_common_dat_format(tmp_path):
+    """common.dat as SaveProfilingResultsToCommon writes it (ProfilerResultsTextExporter.cs:266-305):
+    the header once, then one row per frequency; columns left-aligned in widths 15, 12, 6, 6, 6, 6,
+    9, 9 and 15 for the six times, which are written as "0.00000" (C# TimeSpan.TotalSeconds)."""
+    import datetime
+    m = cm.small_model(4, 6, 2)
+    when = datetime.datetime(2026, 12, 2, 14, 5)
+    line = P.common_dat_line(m, 10, 1.5, (2.25, 3.125), 0.000001, (0.123456789, 7.0), when=when)
+    assert line == ("12/02/2026     14:05       4     6     2     10    1        1        "
+                    "1.50000        2.25000        3.12500        0.00000        0.12346        7.00000        \n")
+    path = str(tmp_path / "common.dat")
+    P.append_common_dat(path, line)
+    P.append_common_dat(path, line)
+    text = open(path).read().split("\n")
+    assert text[0] == ("Date           Time        nx    ny    nz    nhank mpi_size nthreads AtoA_Green     polX"
+                       "           polY           AtoO_Green     AtoO_calc_X    AtoO_calc_Y    ")
+    assert text[1] == text[2] == line[:-1] and text[3] == ""
