@@ -251,6 +251,16 @@ struct IoInvY {
     }
 };
 
+// Column tile of workgroup b among nb. Workgroups go to the 8 XCDs round-robin (b mod 8), and
+// each XCD has its own L2. Giving XCD x a contiguous run of tiles puts neighbouring tiles, which
+// share cache lines of narrow rows (and the R values of the three components of one cell), on one
+// L2; in launch order they would sit on eight different ones. A bijection on [0, nb).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb)
+{
+    const int64_t per = nb / 8, rem = nb % 8, x = b % 8, i = b / 8;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 template <typename T, int P, int TC, int DIR, class IO, int NT>
 __global__ __launch_bounds__(NT) void pencil_fft_kernel(IO io, const cplx<T> *__restrict__ twg)
 {
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(NT) void pencil_fft_kernel(IO io, const cplx<T> *__
         cplx<T> w = twg[k];
         tw[k] = DIR < 0 ? w : cconj<T>(w);
     }
-    const int64_t c0 = (int64_t)blockIdx.x * TC;
+    const int64_t c0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
     const int rhs = blockIdx.y;
     constexpr int NE = (P * TC + NT - 1) / NT;
 #pragma unroll
@@ -365,7 +375,7 @@ __global__ __launch_bounds__(NT) void pencil_fft_reg_kernel(IO io, const cplx<T>
         tw[k] = DIR < 0 ? w : cconj<T>(w);
     }
     __syncthreads();
-    reg_stage<T, P, TC, DIR, P, 1, NT, true>(io, blockIdx.y, (int64_t)blockIdx.x * TC, s, tw, tid);
+    reg_stage<T, P, TC, DIR, P, 1, NT, true>(io, blockIdx.y, xcd_tile(blockIdx.x, gridDim.x) * TC, s, tw, tid);
 }
 
 // Column tiles per pencil length. The default tile has 16 columns while P x 16 stays within 32 KB,
