@@ -35,7 +35,7 @@ METRIC = "FGMRES solve time per freq (2 pol) to 1e-8 residual; op-apply HBM GB/s
 # WRITE_SIZE (one counter per run) and scaled by the same counters on a 1 GiB device copy
 # (FETCH x 2.0000, WRITE x 1.0000); summaries in profiles/<kernel>_<workload>_hbm_traffic.md.
 # Keyed by (workload, nz, right-hand sides per launch, Green layout); no entry reports null.
-MEASURED_TRAFFIC = {("C2", 32, 2, "folded"): 1.5109e9,   # contract_fold32_kernel<double>
+MEASURED_TRAFFIC = {("C2", 32, 2, "folded"): 1.5225e9,   # contract_fold32_kernel<double>
                     ("C5", 64, 2, "folded"): 2.0671e10,  # contract_fold64_kernel<double>
                     ("C2", 32, 2, "full"): 4.7647e9,     # contract_stream_kernel<double, 2>
                     ("C5", 64, 2, "full"): 7.2476e10}    # contract_stream64_kernel<double, 2>

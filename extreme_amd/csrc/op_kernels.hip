@@ -13,6 +13,11 @@
 // butterfly inputs into registers before a barrier and writing after it, and writes
 // the tile back. The zero padding of the forward passes and the cropping of the inverse
 // passes are applied at load/store time, so padded rows are never read or written twice.
+//
+// The contraction kernels copy the Green tensor (read once per apply: 1.1 GB at 128 x 128 x 32,
+// 17 GB at 256 x 256 x 64) HBM -> LDS with the non-temporal hint (nt) on global_load_lds_dwordx4,
+// so the stream does not allocate in L2 and the Infinity Cache over the lines the kernel writes
+// (the folded contraction at 128 x 128 x 32: 0.337 -> 0.300 ms).
 #include <algorithm>
 #include <utility>
 #include <cstdlib>
@@ -788,7 +793,7 @@ __device__ __forceinline__ void stream_issue(const cplx<T> *__restrict__ G, cons
             src = buf + rhs * M + l * C + (q - rhs * C);
         }
         const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
-        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(m0), "v"(src) : "memory", "m0");
     }
 }
 #pragma clang diagnostic pop
@@ -985,7 +990,7 @@ __device__ __forceinline__ void s64_issue(const cplx<T> *__restrict__ G, const c
             src = G + l * GB + ubegin(U) + ulen(U) - K::EPL;     // padding lanes re-read valid data
         }
         const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
-        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(m0), "v"(src) : "memory", "m0");
     }
 }
 #pragma clang diagnostic pop
@@ -1288,7 +1293,7 @@ __device__ __forceinline__ void fold32_issue_one(const Fold32Copy<T> &cp, int t,
         src = cp.buf + h * cp.M + cp.im.l(img) * C + kk;
     }
     const unsigned m0 = __builtin_amdgcn_readfirstlane(cp.dst + (unsigned)k * 1024u);
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(m0), "v"(src) : "memory", "m0");
 }
 
 template <typename T>
@@ -1601,7 +1606,7 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
             src = G + b * GB + base + rc * NZ + 32 * half + (w & 31);
         }
         const unsigned m0 = __builtin_amdgcn_readfirstlane(dst + (unsigned)k * 1024u);
-        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(m0), "v"(src) : "memory", "m0");
     }
 }
 #pragma clang diagnostic pop
