@@ -26,8 +26,21 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-I", os.path.join(HERE, "..", "include"), "-I", CSRC]
 
 
-def _compile(src):
+def _stale(src, out):
+    """True when the object is missing or older than its source or any header in csrc/ and include/."""
+    if not os.path.exists(out):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    inc = os.path.join(HERE, "..", "include")
+    deps += [os.path.join(inc, h) for h in os.listdir(inc) if h.endswith(".h")]
+    t = os.path.getmtime(out)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, incremental=False):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, out):
+        return out
     cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -35,10 +48,12 @@ def _compile(src):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compiles every unit (incremental=True recompiles only units older than their sources or the
+    headers) and links the library."""
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -49,5 +64,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -21,6 +21,14 @@ MU0 = 4.0 * math.pi * 1.0e-07                    # OmegaModelUtils.cs:12
 EPS0 = 1.0 / (MU0 * 299792458.0 * 299792458.0)   # OmegaModelUtils.cs:13
 
 
+def depth_um(v: float) -> int:
+    """A depth or thickness in whole micrometres. The reference holds them as System.Decimal
+    (Extreme.Core/Model/Layer1D.cs), so its layer lookups add and compare exactly; on doubles
+    2.2 + 99.9 lands above 102.1. Integers of micrometres are exact for any depth written with at
+    most six decimals."""
+    return int(round(float(v) * 1e6))
+
+
 def omega_of(frequency: float) -> float:
     """OmegaModelUtils.FrequencyToOmega (OmegaModelUtils.cs:55-58)."""
     return 2 * math.pi * frequency
@@ -46,22 +54,27 @@ class Section1D:
         return self.zero_level + float(sum(self.thickness[:index]))
 
     def background_layer(self, start: float, end: float) -> int:
-        """ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52)."""
-        ss = self.zero_level
+        """ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52): the first layer
+        whose closed interval holds [start, end], compared in exact micrometres (depth_um)."""
+        ss = depth_um(self.zero_level)
+        a, b = depth_um(start), depth_um(end)
         for i, t in enumerate(self.thickness):
-            es = ss + t
-            if start >= ss and end <= es:
+            es = ss + depth_um(t)
+            if a >= ss and b <= es:
                 return i
             ss = es
         raise ValueError(f"anomaly layer [{start}, {end}] is not inside one background layer")
 
     def layer_at_depth(self, depth: float) -> int:
-        """ScalarMathUtils.DetermineCorrespondingBackgroundLayer, half-open (ScalarMathUtils.cs:15-31)."""
-        prev = self.zero_level
+        """ScalarMathUtils.DetermineCorrespondingBackgroundLayer, half-open [d1, d2)
+        (ScalarMathUtils.cs:15-31), in exact micrometres (depth_um)."""
+        prev = depth_um(self.zero_level)
+        z = depth_um(depth)
         for k, t in enumerate(self.thickness):
-            if prev <= depth < prev + t:
+            d2 = prev + depth_um(t)
+            if prev <= z < d2:
                 return k
-            prev += t
+            prev = d2
         raise ValueError("depth is not within the 1-D section")
 
     def zeta(self, omega: float) -> np.ndarray:

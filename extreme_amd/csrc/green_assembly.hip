@@ -601,12 +601,9 @@ struct Dev {
 
 int layer_at_depth(const Section &s, double depth)   // ScalarMathUtils.cs:15-31
 {
-    double prev = s.zero_level;
-    for (int k = 0; k < (int)s.thickness.size(); ++k) {
-        if (depth >= prev && depth < prev + s.thickness[k]) return k;
-        prev += s.thickness[k];
-    }
-    throw hip_error("anomaly is not within 1D section");
+    const int k = layer_index_at_depth(s, depth);
+    if (k < 0) throw hip_error("anomaly is not within 1D section");
+    return k;
 }
 
 }  // namespace

@@ -369,16 +369,12 @@ void upload(DevBuf<T> &d, const std::vector<T> &h)
     XTR_CHECK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
-// ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52)
+// ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52), in exact micrometres
 int background_layer(const xtr::Section &s, double start, double end)
 {
-    double ss = s.zero_level;
-    for (int i = 0; i < (int)s.thickness.size(); ++i) {
-        const double es = ss + s.thickness[i];
-        if (start >= ss && end <= es) return i;
-        ss = es;
-    }
-    throw std::invalid_argument("an anomaly layer is not inside one background layer");
+    const int i = xtr::background_layer_index(s, start, end);
+    if (i < 0) throw std::invalid_argument("an anomaly layer is not inside one background layer");
+    return i;
 }
 
 cd zeta_of(double sigma, double omega) { return cd(sigma, -omega * kEps0); }   // OmegaModelUtils.cs:41-42
@@ -1469,14 +1465,25 @@ int xtr_op_get_green_atoo(xtr_op *op, int electric, xtr_complex *const *comps)
     });
 }
 
-int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq)
+int xtr_op_jq_model_device(xtr_op *op, int model, const void *chi, const void *js, void *jq)
 {
     return guarded([&] {
         require(op && chi && js && jq, "null argument");
         require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before computing jQ");
-        xtr::launch_jq((const double2 *)chi, (const double2 *)js, op->R.p, op->f.p, op->g, (double2 *)jq, op->stream);
+        const double2 *R = op->model_r(model);
+        XTR_CHECK(hipDeviceSynchronize());   // chi and js may have been written on other streams
+        xtr::launch_jq((const double2 *)chi, (const double2 *)js, R, op->f.p, op->g, (double2 *)jq, op->stream);
         XTR_CHECK(hipStreamSynchronize(op->stream));
     });
+}
+
+int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq)
+{
+    if (op && op->nmodels != 1) {
+        g_last_error = "the operator holds several models: name one with xtr_op_jq_model_device";
+        return -1;
+    }
+    return xtr_op_jq_model_device(op, 0, chi, js, jq);
 }
 
 int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *field)

@@ -3,6 +3,7 @@
 
 #include "xtr_kernels.h"
 
+#include <cmath>
 #include <vector>
 
 namespace xtr {
@@ -13,6 +14,39 @@ struct Section {
     double zero_level = 0;
     std::vector<double> thickness, sigma;
 };
+
+// The reference keeps depths and thicknesses as System.Decimal (Extreme.Core/Model/Layer1D.cs), so
+// its layer lookups add and compare exactly; on doubles 2.2 + 99.9 lands above 102.1. The lookups
+// below compare whole micrometres instead, exact for any depth written with at most six decimals.
+inline long long depth_um(double v) { return std::llround(v * 1e6); }
+
+// ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52): the first layer whose
+// closed interval holds [start, end]; -1 when none does.
+inline int background_layer_index(const Section &s, double start, double end)
+{
+    long long ss = depth_um(s.zero_level);
+    const long long a = depth_um(start), b = depth_um(end);
+    for (size_t i = 0; i < s.thickness.size(); ++i) {
+        const long long es = ss + depth_um(s.thickness[i]);
+        if (a >= ss && b <= es) return (int)i;
+        ss = es;
+    }
+    return -1;
+}
+
+// ScalarMathUtils.DetermineCorrespondingBackgroundLayer (Green/Scalar/Impl/ScalarMathUtils.cs:15-31):
+// the layer whose half-open interval [d1, d2) holds depth; -1 when none does.
+inline int layer_index_at_depth(const Section &s, double depth)
+{
+    long long prev = depth_um(s.zero_level);
+    const long long z = depth_um(depth);
+    for (size_t k = 0; k < s.thickness.size(); ++k) {
+        const long long d2 = prev + depth_um(s.thickness[k]);
+        if (z >= prev && z < d2) return (int)k;
+        prev = d2;
+    }
+    return -1;
+}
 
 struct GreenInput {
     Geom g;

@@ -101,6 +101,7 @@ def lib():
         L.xtr_op_build_green_atoo.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int]
         L.xtr_op_get_green_atoo.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.xtr_op_jq_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.xtr_op_jq_model_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.xtr_op_level_field_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -190,6 +191,31 @@ class ResultInfo:
 def _dptr(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _device_vectors(ts, n, what="vector"):
+    """Checks tensors handed to a device entry point: the kernels read and write n complex128 values
+    behind each bare pointer and cannot check them, so a CPU tensor, another dtype, a strided view or a
+    short vector would make them access memory out of bounds. Raises ValueError instead."""
+    torch = _torch()
+    for t in ts:
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{what} must be a torch tensor on the GPU, got {type(t).__name__}")
+        if not t.is_cuda:
+            raise ValueError(f"{what} must be on the GPU, got a tensor on {t.device}")
+        if t.dtype != torch.complex128:
+            raise ValueError(f"{what} must be complex128, got {t.dtype}")
+        if not t.is_contiguous():
+            raise ValueError(f"{what} must be contiguous")
+        if t.numel() != n:
+            raise ValueError(f"{what} must hold {n} values, got {t.numel()}")
+
+
+def _host_vector(a, n, what="vector"):
+    if not isinstance(a, np.ndarray) or a.dtype != np.complex128 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"{what} must be a C-contiguous complex128 numpy array")
+    if a.size != n:
+        raise ValueError(f"{what} must hold {n} values, got {a.size}")
 
 
 def _ptr(t):
@@ -288,7 +314,11 @@ class ConvolutionOperator:
         """One apply of device vectors xs[r] on models[r]: the Green tensor is streamed once."""
         n = len(xs)
         if ys is None:
+            _device_vectors(xs, self.n, "x")
             ys = [x.new_empty(x.shape) for x in xs]
+        if len(models) != n or len(ys) != n:
+            raise ValueError("xs, ys and models must have the same length")
+        _device_vectors(list(xs) + list(ys), self.n, "vector")
         mi = (C.c_int * n)(*[int(m) for m in models])
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
         yp = (C.c_void_p * n)(*[y.data_ptr() for y in ys])
@@ -299,13 +329,17 @@ class ConvolutionOperator:
         """Apply(input, output): numpy arrays (host) or torch GPU tensors (device), n or [nrhs, n]."""
         nrhs = 1 if x.ndim == 1 else x.shape[0]
         if hasattr(x, "data_ptr"):
+            _device_vectors([x], nrhs * self.n, "x")
             if y is None:
                 y = x.new_empty(x.shape)
+            _device_vectors([y], nrhs * self.n, "y")
             _check(lib().xtr_op_apply_device(self._h, _ptr(x), _ptr(y), nrhs))
             return y
         x = np.ascontiguousarray(x, dtype=np.complex128)
+        _host_vector(x, nrhs * self.n, "x")
         if y is None:
             y = np.empty_like(x)
+        _host_vector(y, nrhs * self.n, "y")
         _check(lib().xtr_op_apply(self._h, _ptr(x), _ptr(y), nrhs))
         return y
 
@@ -332,20 +366,28 @@ class ConvolutionOperator:
         _check(lib().xtr_op_get_green_atoo(self._h, 1 if electric else 0, ptrs))
         return out
 
-    def jq(self, chi, js, out=None):
+    def jq(self, chi, js, out=None, model=None):
         """ForwardSolver.CalculateJqFrom (ForwardSolver.cs:364-389, 424-436) on device tensors; the
-        operator must be prepared as "A"."""
+        operator must be prepared as "A". model names the model of an operator prepared by
+        prepare_models (required when it holds more than one)."""
+        _device_vectors([chi, js], self.n, "chi and js")
         if out is None:
             out = chi.new_empty(chi.shape)
-        _check(lib().xtr_op_jq_device(self._h, _ptr(chi), _ptr(js), _ptr(out)))
+        _device_vectors([out], self.n, "out")
+        if model is None:
+            _check(lib().xtr_op_jq_device(self._h, _ptr(chi), _ptr(js), _ptr(out)))
+        else:
+            _check(lib().xtr_op_jq_model_device(self._h, int(model), _ptr(chi), _ptr(js), _ptr(out)))
         return out
 
     def level_field(self, jq, electric: bool = True, out=None):
         """AtoOCalculator.CalculateAnomalyFieldE/H at the level (AtoOCalculator.cs:63-99): device
         tensor [Nx, Ny, 3]."""
         m = self.model
+        _device_vectors([jq], self.n, "jq")
         if out is None:
             out = jq.new_empty((m.nx, m.ny, 3))
+        _device_vectors([out], 3 * m.nx * m.ny, "out")
         _check(lib().xtr_op_level_field_device(self._h, 1 if electric else 0, _ptr(jq), _ptr(out)))
         return out
 
@@ -411,11 +453,12 @@ class AnomalyCurrentFgmresSolver:
         info = _Info()
         cb = self._cb()
         if hasattr(b, "data_ptr"):
+            _device_vectors([b, x], self.op.n, "b and x")
             _check(lib().xtr_fgmres_solve_device(self.op._h, C.byref(self.params), _ptr(b), _ptr(x),
                                                  C.byref(info), cb, None))
         else:
-            if not (b.flags["C_CONTIGUOUS"] and x.flags["C_CONTIGUOUS"]):
-                raise ValueError("host vectors must be C-contiguous")
+            _host_vector(b, self.op.n, "b")
+            _host_vector(x, self.op.n, "x")
             _check(lib().xtr_fgmres_solve(self.op._h, C.byref(self.params), _ptr(b), _ptr(x), C.byref(info),
                                           cb, None))
         return ResultInfo._from(info)
@@ -423,6 +466,9 @@ class AnomalyCurrentFgmresSolver:
     def solve_learned(self, b, x, net: "LearnedPreconditioner") -> ResultInfo:
         """FGMRES with the learned network as the right preconditioner in place of the inner GMRES
         (device tensors; x holds the initial guess)."""
+        _device_vectors([b, x], self.op.n, "b and x")
+        if net.shape != (self.op.model.nx, self.op.model.ny, self.op.model.nz):
+            raise ValueError("the network's grid differs from the operator's")
         info = _Info()
         _check(lib().xtr_fgmres_solve_learned_device(self.op._h, C.byref(self.params), net._h, _ptr(b), _ptr(x),
                                                      C.byref(info)))
@@ -432,6 +478,7 @@ class AnomalyCurrentFgmresSolver:
         """solve() on device tensors that also returns the first `capacity` pairs (v_j, z_j) of the
         flexible preconditioner: the Krylov vector and the inner GMRES's result for it."""
         torch = _torch()
+        _device_vectors([b, x], self.op.n, "b and x")
         vs = [torch.empty_like(b) for _ in range(capacity)]
         zs = [torch.empty_like(b) for _ in range(capacity)]
         vp = (C.c_void_p * max(capacity, 1))(*[v.data_ptr() for v in vs])
@@ -444,6 +491,9 @@ class AnomalyCurrentFgmresSolver:
     def solve_batch(self, bs, xs) -> list:
         """Side-by-side solves sharing every operator apply (device tensors)."""
         n = len(bs)
+        if len(xs) != n:
+            raise ValueError("bs and xs must have the same length")
+        _device_vectors(list(bs) + list(xs), self.op.n, "b and x")
         infos = (_Info * n)()
         bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
@@ -455,6 +505,9 @@ class AnomalyCurrentFgmresSolver:
         """Side-by-side solves of right-hand sides on different models that share the Green tensor
         (operator prepared with prepare_models("A", ...)); device tensors."""
         n = len(bs)
+        if len(xs) != n or len(models) != n:
+            raise ValueError("bs, xs and models must have the same length")
+        _device_vectors(list(bs) + list(xs), self.op.n, "b and x")
         infos = (_Info * n)()
         mi = (C.c_int * n)(*[int(m) for m in models])
         bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
@@ -468,6 +521,9 @@ class AnomalyCurrentFgmresSolver:
         """Mixed precision (operator created with precision="mixed"): FP64 iterative refinement
         around complex64 FGMRES corrections, stopping on the FP64 true residual. Device tensors."""
         n = len(bs)
+        if len(xs) != n:
+            raise ValueError("bs and xs must have the same length")
+        _device_vectors(list(bs) + list(xs), self.op.n, "b and x")
         infos = (_RefineInfo * n)()
         bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
@@ -533,10 +589,18 @@ class LearnedPreconditioner:
 
     def apply(self, v, out=None):
         """z = preconditioner(v) for a complex128 device vector of 3 nx ny nz values."""
+        n = 3 * self.shape[0] * self.shape[1] * self.shape[2]
+        _device_vectors([v], n, "v")
         if out is None:
             out = v.new_empty(v.shape)
+        _device_vectors([out], n, "out")
         _check(lib().xtr_net_apply_device(self._h, _ptr(v), _ptr(out)))
         return out
+
+    def _check_pairs(self, vs, zs):
+        if len(vs) != len(zs) or not vs:
+            raise ValueError("vs and zs must be non-empty and of the same length")
+        _device_vectors(list(vs) + list(zs), 3 * self.shape[0] * self.shape[1] * self.shape[2], "pair vector")
 
     @staticmethod
     def _ptrs(ts):
@@ -544,6 +608,7 @@ class LearnedPreconditioner:
 
     def gradients(self, vs, zs):
         """(dL/dw1, dL/dw2, L) for the mean relative error of the correction over the pairs."""
+        self._check_pairs(vs, zs)
         g1 = np.zeros((27, 8, 16), dtype=np.float32)
         g2 = np.zeros((27, 16, 8), dtype=np.float32)
         loss = C.c_double()
@@ -554,6 +619,7 @@ class LearnedPreconditioner:
 
     def train_step(self, vs, zs, lr: float = 1e-3) -> float:
         """One Adam step on the pairs; returns the loss before the step."""
+        self._check_pairs(vs, zs)
         loss = C.c_double()
         _check(lib().xtr_net_train_device(self._h, len(vs), self._ptrs(vs), self._ptrs(zs), C.c_float(lr),
                                           C.byref(loss)))

@@ -233,6 +233,10 @@ int xtr_op_get_green_atoo(xtr_op *op, int electric, xtr_complex *const *comps);
  * jQ = E_s (zeta - zeta0) + j_s from chi and j_s (device vectors of n values). The operator must be
  * prepared as XTR_OP_A. */
 int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq);
+/* The same for model `model` of an operator prepared by xtr_op_prepare_models (its R); the
+ * reference computes jQ per model (ForwardSolver.cs:364-389), one model per operator.
+ * xtr_op_jq_device refuses an operator that holds more than one model. */
+int xtr_op_jq_model_device(xtr_op *op, int model, const void *chi, const void *js, void *jq);
 
 /* Replaces: AtoOCalculator.CalculateAnomalyFieldE/H(level, jQ, field) (AtoOCalculator.cs:63-99)
  * through ToOCalculator.CalculateAnomalyField (ToOCalculator.cs:21-45): the anomalous E (electric

@@ -102,6 +102,10 @@ def lib():
                                             f64p]
         L.xo_plan_create_range.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p]
         L.xo_atoo_field.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_int, c128p, c128p]
+        L.xo_background_layer.restype = C.c_int
+        L.xo_background_layer.argtypes = [C.POINTER(Section), C.c_double, C.c_double]
+        L.xo_layer_at_depth.restype = C.c_int
+        L.xo_layer_at_depth.argtypes = [C.POINTER(Section), C.c_double]
         L.xo_log10_rho_step.restype = C.c_double
         L.xo_hankel_lambdas.restype = C.c_int
         L.xo_hankel_lambdas.argtypes = [C.c_double, C.c_int, f64p]
@@ -137,6 +141,25 @@ def make_section(sec, keep):
 def make_grid(model, keep):
     return Grid(model.nx, model.ny, model.nz, float(model.dx), float(model.dy),
                 keep.d(model.depth), keep.d(model.thick))
+
+
+def zeta(sigma, omega):
+    """zeta = sigma - i omega eps0 of one layer (OmegaModelUtils.cs:41-42)."""
+    return _cplx(lib().xo_zeta(float(sigma), float(omega)))
+
+
+def background_layer(section, start, end):
+    """ModelUtils.FindCorrespondingBackgroundLayerIndex (Model/ModelUtils.cs:35-52); -1 when no finite
+    layer holds [start, end] (the reference throws)."""
+    k = _Keep()
+    return lib().xo_background_layer(C.byref(make_section(section, k)), float(start), float(end))
+
+
+def layer_at_depth(section, depth):
+    """ScalarMathUtils.DetermineCorrespondingBackgroundLayer, half-open (ScalarMathUtils.cs:15-31); -1
+    when no layer holds the depth (the reference throws)."""
+    k = _Keep()
+    return lib().xo_layer_at_depth(C.byref(make_section(section, k)), float(depth))
 
 
 def plane_wave_e(section, omega, depth, thick=0.0):

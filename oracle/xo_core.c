@@ -36,13 +36,19 @@ double xo_eps0(void) { return 1.0 / (xo_mu0() * 299792458.0 * 299792458.0); }
 /* zeta = sigma - i omega eps0 (OmegaModelUtils.cs:41-42) */
 xo_c xo_zeta(double sigma, double omega) { return sigma - I * (omega * xo_eps0()); }
 
+/* Depths and thicknesses are System.Decimal in the reference (Extreme.Core/Model/Layer1D.cs,
+ * Model/ModelUtils.cs:28-52), so its layer lookups add and compare exactly. The same comparisons on
+ * doubles can misjudge a layer boundary (2.2 + 99.9 > 102.1 in binary), so they run here on whole
+ * micrometres: exact for every depth written with at most six decimals. */
+static long long xo_um(double v) { return llround(v * 1e6); }
+
 /* Model/ModelUtils.cs:35-52: first layer whose closed interval [ss, es] holds [start, end] */
 int xo_background_layer(const xo_section *s, double start, double end)
 {
-    double ss = s->zero_level;
+    long long ss = xo_um(s->zero_level), a = xo_um(start), b = xo_um(end);
     for (int i = 0; i < s->nlayers; i++) {
-        double es = ss + s->thickness[i];
-        if (start >= ss && end <= es) return i;
+        long long es = ss + xo_um(s->thickness[i]);
+        if (a >= ss && b <= es) return i;
         ss = es;
     }
     return -1;
@@ -51,10 +57,10 @@ int xo_background_layer(const xo_section *s, double start, double end)
 /* Green/Scalar/Impl/ScalarMathUtils.cs:15-31: half-open [d1, d2) */
 int xo_layer_at_depth(const xo_section *s, double depth)
 {
-    double prev = s->zero_level;
+    long long prev = xo_um(s->zero_level), z = xo_um(depth);
     for (int k = 0; k < s->nlayers; k++) {
-        double d1 = prev, d2 = prev + s->thickness[k];
-        if (depth >= d1 && depth < d2) return k;
+        long long d1 = prev, d2 = prev + xo_um(s->thickness[k]);
+        if (z >= d1 && z < d2) return k;
         prev = d2;
     }
     return -1;
