@@ -148,6 +148,10 @@ inline void spin_wait(hipStream_t s, hipEvent_t ev)
 
 }  // namespace
 
+namespace {
+struct Slot;   // Krylov storage, stream and reduction scratch of one right-hand side (below)
+}
+
 struct xtr_op {
     xtr::Geom g{};
     int precision = XTR_FP64;
@@ -193,13 +197,14 @@ struct xtr_op {
     size_t ev_used = 0;
     std::mutex apply_mu;
 
-    ~xtr_op()
-    {
-        for (auto &set : ev_sets)
-            for (auto &e : set)
-                if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
+    // Per right-hand side: a stream, the Krylov bases of the outer FGMRES and the inner GMRES, the
+    // refinement vectors and the reduction scratch, allocated on first use and kept for later solves
+    // (grown when a solve needs more), so that a solve allocates nothing once the handle has run one
+    // of its shape. Slot kMaxRhs holds the FP64 norms of the mixed-precision refinement.
+    std::vector<std::unique_ptr<Slot>> slots;
+    Slot &slot(int r);
+
+    ~xtr_op();
 
     hipEvent_t *next_events()
     {
@@ -416,9 +421,77 @@ struct SolveCtx {
     SolveCtx(const SolveCtx &) = delete;
     SolveCtx &operator=(const SolveCtx &) = delete;
     void wait() { spin_wait(stream, ev); }
+    // ready for a new solve on stream s: counters, callbacks and harvest cleared
+    void reset(hipStream_t s)
+    {
+        stream = s;
+        apply = nullptr;
+        matvecs = dots = 0;
+        on_iter = nullptr;
+        user = nullptr;
+        rhs_index = 0;
+        harvest_v = harvest_z = nullptr;
+        harvest_cap = harvest_count = 0;
+    }
+};
+
+// The vectors of one Gmres instance: basis V (m columns), flexible basis Z (m columns, FGMRES
+// only), the work vector w and the residual r0.
+struct GmresBufs {
+    DevBuf<double2> V, Z, w, r0;
+};
+
+inline void grow(DevBuf<double2> &b, size_t need)
+{
+    if (b.count < need) b.alloc(need);
+}
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;   // the solve's vector is ready for the shared apply (ApplyGate)
+    SolveCtx ctx;
+    GmresBufs outer, inner;
+    DevBuf<double2> r, d;         // refinement residual and correction
+    Slot()
+    {
+        XTR_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        const hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamDefault);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(ready);
+            XTR_CHECK(e);
+        }
+    }
+    ~Slot()
+    {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (ready) (void)hipEventDestroy(ready);
+    }
+    Slot(const Slot &) = delete;
+    Slot &operator=(const Slot &) = delete;
 };
 
 }  // namespace
+
+Slot &xtr_op::slot(int r)
+{
+    require(r >= 0 && r <= xtr::kMaxRhs, "right-hand side slot out of range");
+    if ((int)slots.size() <= r) slots.resize(r + 1);
+    if (!slots[r]) slots[r] = std::make_unique<Slot>();
+    return *slots[r];
+}
+
+xtr_op::~xtr_op()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    slots.clear();
+    for (auto &set : ev_sets)
+        for (auto &e : set)
+            if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
 
 // Learned right preconditioner (no counterpart in the reference): the network's device state.
 struct xtr_net {
@@ -498,22 +571,28 @@ struct Gmres {
     SolveCtx *ctx;
     Gmres *inner = nullptr;
     xtr_net *net = nullptr;       // learned right preconditioner in place of the inner GMRES
-    DevBuf<double2> V, Z, w, r0;
+    double2 *V = nullptr, *Z = nullptr, *w = nullptr, *r0 = nullptr;   // in the slot's GmresBufs
     std::vector<cd> H;      // (m+1) x (m+1), column-major (FortranMatrix.cs:11-52)
     std::vector<double> gc;
     std::vector<cd> gs;
     int iter_counter = 0;
 
-    Gmres(int64_t n_, int m_, bool flex, SolveCtx *c)
-        : n(n_), m(m_), flexible(flex), ctx(c), V((size_t)n_ * m_), w(n_), r0(n_),
-          H((size_t)(m_ + 1) * (m_ + 1)), gc(m_), gs(m_)
+    Gmres(int64_t n_, int m_, bool flex, SolveCtx *c, GmresBufs &b)
+        : n(n_), m(m_), flexible(flex), ctx(c), H((size_t)(m_ + 1) * (m_ + 1)), gc(m_), gs(m_)
     {
-        if (flex) Z.alloc((size_t)n_ * m_);
         require(m_ >= 1 && m_ <= 31, "buffer length must be within 1..31");
+        grow(b.V, (size_t)n_ * m_);
+        if (flex) grow(b.Z, (size_t)n_ * m_);
+        grow(b.w, (size_t)n_);
+        grow(b.r0, (size_t)n_);
+        V = b.V.p;
+        Z = flex ? b.Z.p : nullptr;
+        w = b.w.p;
+        r0 = b.r0.p;
     }
 
     cd &h(int i, int j) { return H[(size_t)j * (m + 1) + i]; }
-    double2 *col(DevBuf<double2> &B, int j) { return B.p + (int64_t)j * n; }
+    double2 *col(double2 *B, int j) { return B + (int64_t)j * n; }
 
     void matvec(const double2 *x, double2 *y)
     {
@@ -534,9 +613,9 @@ struct Gmres {
     void cgs(int jH, double &dloo, double &dnormw)
     {
         const int cnt = jH + 1;
-        xtr::launch_block_dots(V.p, cnt, w.p, n, ctx->partial_c.p, ctx->stream);
+        xtr::launch_block_dots(V, cnt, w, n, ctx->partial_c.p, ctx->stream);
         xtr::launch_reduce_c(ctx->partial_c.p, cnt, ctx->h_dev.p, ctx->stream, ctx->h_host.dev);
-        xtr::launch_cgs_update(V.p, cnt, ctx->h_dev.p, w.p, n, ctx->partial_r.p, ctx->stream);
+        xtr::launch_cgs_update(V, cnt, ctx->h_dev.p, w, n, ctx->partial_r.p, ctx->stream);
         xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream, ctx->nn_host.dev);
         ctx->wait();
         double s = 0;
@@ -593,9 +672,9 @@ struct Gmres {
                 XTR_CHECK(hipMemcpyAsync(ctx->harvest_z[k], col(Z, jH), n * sizeof(double2), hipMemcpyDeviceToDevice,
                                          ctx->stream));
             }
-            matvec(col(Z, jH), w.p);
+            matvec(col(Z, jH), w);
         } else {
-            matvec(col(V, jH), w.p);
+            matvec(col(V, jH), w);
         }
     }
 
@@ -611,15 +690,15 @@ struct Gmres {
         }
         xtr::Coef32 c{};
         for (int i = 0; i < jH; ++i) c.c[i] = make_double2(y[i].real(), y[i].imag());
-        if (jH > 0) xtr::launch_update_solution(flexible ? Z.p : V.p, jH, c, x, n, ctx->stream);
+        if (jH > 0) xtr::launch_update_solution(flexible ? Z : V, jH, c, x, n, ctx->stream);
     }
 
     // CheckBackwardError (Gmres.cs:200-230)
     double check_backward_error(double bn, const double2 *rhs, const double2 *x)
     {
-        matvec(x, w.p);
-        xtr::launch_sub(rhs, w.p, w.p, n, ctx->stream);
-        const double be = norm(w.p) / bn;
+        matvec(x, w);
+        xtr::launch_sub(rhs, w, w, n, ctx->stream);
+        const double be = norm(w) / bn;
         if (be >= tol && backward_check == 2)
             throw std::runtime_error("be = " + shortest(be) + ", tolerance = " + shortest(tol));
         return be;
@@ -648,7 +727,7 @@ struct Gmres {
                     throw std::runtime_error("a Krylov vector holds NaN or Inf: the operator produced non-finite values");
             } while ((2.0 * dnormw <= std::sqrt(dloo)) && (north < 3));
             h(jH + 1, jH) = dnormw;
-            if (jH < m - 1) xtr::launch_div(w.p, dnormw, col(V, jH + 1), n, ctx->stream);
+            if (jH < m - 1) xtr::launch_div(w, dnormw, col(V, jH + 1), n, ctx->stream);
             for (int j = 0; j < jH; ++j) zrot1(h(j, jH), h(j + 1, jH), gc[j], gs[j]);
             zrotg(h(jH, jH), h(jH + 1, jH), gc[jH], gs[jH]);
             zrot1(h(jH, m), h(jH + 1, m), gc[jH], gs[jH]);
@@ -666,9 +745,9 @@ struct Gmres {
     double restart(const double2 *rhs, double2 *x, int jH)
     {
         construct(jH, x);
-        matvec(x, r0.p);
-        xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
-        return norm(w.p);
+        matvec(x, r0);
+        xtr::launch_sub(rhs, r0, w, n, ctx->stream);
+        return norm(w);
     }
 
     // GmresSolver.Solve (Gmres.cs:77-143), initial guess supplied
@@ -682,9 +761,9 @@ struct Gmres {
             bea_out = 0; be_out = 0; conv = 1;
             return;
         }
-        matvec(x0, r0.p);
-        xtr::launch_sub(rhs, r0.p, w.p, n, ctx->stream);
-        double beta = norm(w.p);
+        matvec(x0, r0);
+        xtr::launch_sub(rhs, r0, w, n, ctx->stream);
+        double beta = norm(w);
         if (!std::isfinite(beta)) throw std::runtime_error("the initial residual holds NaN or Inf (operator or initial guess)");
         if (beta == 0) {   // CheckForExactSolution: ResultInfo.Converged(0, 0, 1) (Gmres.cs:104-107)
             if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
@@ -696,7 +775,7 @@ struct Gmres {
         int iter = 0;
         if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
         for (;;) {
-            xtr::launch_mul(w.p, 1 / beta, col(V, 0), n, ctx->stream);
+            xtr::launch_mul(w, 1 / beta, col(V, 0), n, ctx->stream);
             h(0, m) = beta;
             for (int i = 1; i <= m; ++i) h(i, m) = 0;
             int jH = 0;
@@ -716,21 +795,22 @@ struct Gmres {
 };
 
 // FlexibleGmresWithGmresPreconditioner (:29-58) on one right-hand side.
-void run_fgmres(int64_t n, const xtr_gmres_params &p, SolveCtx &ctx, const double2 *b, double2 *x,
+void run_fgmres(int64_t n, const xtr_gmres_params &p, Slot &sl, const double2 *b, double2 *x,
                 xtr_result_info &info, xtr_net *net = nullptr)
 {
+    SolveCtx &ctx = sl.ctx;
     require(p.outer_buffer_length >= 1 && p.outer_buffer_length <= 31, "OuterBufferLength must be 1..31");
     require(p.inner_buffer_length >= 1 && p.inner_buffer_length <= 31, "InnerBufferLength must be 1..31");
     require(p.max_repeat_number >= 1, "MaxRepeatsNumber must be >= 1");
     auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<Gmres> inner;
     if (!net) {
-        inner = std::make_unique<Gmres>(n, p.inner_buffer_length, false, &ctx);
+        inner = std::make_unique<Gmres>(n, p.inner_buffer_length, false, &ctx, sl.inner);
         inner->tol = p.tolerance;
         inner->max_repeat = 1;
         inner->backward_check = 0;
     }
-    Gmres outer(n, p.outer_buffer_length, true, &ctx);
+    Gmres outer(n, p.outer_buffer_length, true, &ctx, sl.outer);
     outer.tol = p.tolerance;
     outer.max_repeat = p.max_repeat_number;
     outer.backward_check = p.backward_error_checking;
@@ -820,14 +900,13 @@ struct ApplyGate {
 // C ABI
 // =============================================================================================
 
-// Side-by-side FGMRES solves of nrhs right-hand sides with one ApplyGate (shared applies).
+// Side-by-side FGMRES solves of nrhs right-hand sides with one ApplyGate (shared applies). Right-hand
+// side r runs on its own host thread with the handle's slot r (stream, bases, scratch).
 static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, const void *const *b, void *const *x,
                         xtr_result_info *info, xtr_iteration_fn on_iteration, void *user, const int *model = nullptr)
 {
     ApplyGate gate(op, nrhs);
-    std::vector<std::unique_ptr<SolveCtx>> ctxs;
-    std::vector<hipStream_t> streams(nrhs, nullptr);
-    std::vector<hipEvent_t> ready(nrhs, nullptr);
+    std::vector<Slot *> sl(nrhs, nullptr);
     std::vector<std::exception_ptr> errs(nrhs);
     std::mutex cb_mu;
     struct CbUser { xtr_iteration_fn fn; void *user; std::mutex *mu; };
@@ -839,21 +918,19 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
     };
     try {
         for (int r = 0; r < nrhs; ++r) {
-            XTR_CHECK(hipStreamCreateWithFlags(&streams[r], hipStreamDefault));
-            XTR_CHECK(hipEventCreateWithFlags(&ready[r], hipEventDisableTiming));
-            auto c = std::make_unique<SolveCtx>();
-            c->stream = streams[r];
-            c->rhs_index = r;
+            sl[r] = &op->slot(r);
+            SolveCtx &c = sl[r]->ctx;
+            c.reset(sl[r]->stream);
+            c.rhs_index = r;
             if (on_iteration) {
-                c->on_iter = cb;
-                c->user = &cbu;
+                c.on_iter = cb;
+                c.user = &cbu;
             }
-            hipStream_t s = streams[r];
-            hipEvent_t ev = ready[r];
+            hipStream_t s = sl[r]->stream;
+            hipEvent_t ev = sl[r]->ready;
             ApplyGate *gp = &gate;
             const double2 *Rp = model ? op->model_r(model[r]) : nullptr;   // null: the handle's R
-            c->apply = [gp, s, ev, Rp](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev, Rp); };
-            ctxs.push_back(std::move(c));
+            c.apply = [gp, s, ev, Rp](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev, Rp); };
         }
         // inputs written on the caller's default stream must be visible to the solve streams
         XTR_CHECK(hipDeviceSynchronize());
@@ -862,7 +939,7 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
             th.emplace_back([&, r] {
                 try {
                     std::memset(&info[r], 0, sizeof(info[r]));
-                    run_fgmres(op->g.n(), *params, *ctxs[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
+                    run_fgmres(op->g.n(), *params, *sl[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
                 } catch (...) {
                     errs[r] = std::current_exception();
                 }
@@ -872,17 +949,11 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
         for (auto &t : th) t.join();
         XTR_CHECK(hipStreamSynchronize(op->stream));
     } catch (...) {
-        for (auto &s : streams)
-            if (s) (void)hipStreamSynchronize(s);
-        for (auto &e : ready)
-            if (e) (void)hipEventDestroy(e);
-        for (auto &s : streams)
-            if (s) (void)hipStreamDestroy(s);
+        for (auto *p : sl)
+            if (p) (void)hipStreamSynchronize(p->stream);
         throw;
     }
-    for (auto &s : streams) (void)hipStreamSynchronize(s);
-    for (auto &e : ready) (void)hipEventDestroy(e);
-    for (auto &s : streams) (void)hipStreamDestroy(s);
+    for (auto *p : sl) XTR_CHECK(hipStreamSynchronize(p->stream));
     for (auto &e : errs)
         if (e) std::rethrow_exception(e);
 }
@@ -1264,6 +1335,19 @@ int xtr_op_green_layout(xtr_op *op, int *folded, int64_t *blocks)
     });
 }
 
+int xtr_op_green_mirror_check(xtr_op *op, double *out12)
+{
+    return guarded([&] {
+        require(op && out12, "null argument");
+        require(op->green_ready && op->G64.p, "the FP64 Green tensor is not set");
+        require(!op->folded, "the Green tensor is folded: the check needs every wavenumber's block");
+        DevBuf<double> stat(12);
+        xtr::launch_green_mirror_check(op->G64.p, op->g, stat.p, op->stream);
+        XTR_CHECK(hipMemcpyAsync(out12, stat.p, 12 * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    });
+}
+
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes)
 {
     return guarded([&] {
@@ -1281,8 +1365,9 @@ int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const vo
     return guarded([&] {
         require(op && params && b && x && info, "null argument");
         require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
-        SolveCtx ctx;
-        ctx.stream = op->stream;
+        Slot &sl = op->slot(0);
+        SolveCtx &ctx = sl.ctx;
+        ctx.reset(op->stream);
         ctx.on_iter = on_iteration;
         ctx.user = user;
         xtr_op *o = op;
@@ -1293,7 +1378,7 @@ int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const vo
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));
-        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info);
+        run_fgmres(op->g.n(), *params, sl, (const double2 *)b, (double2 *)x, *info);
     });
 }
 
@@ -1349,12 +1434,15 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
         const int64_t n = op->g.n();
         const auto t0 = std::chrono::steady_clock::now();
         XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
-        SolveCtx ctx;                        // reduction scratch for the FP64 norms
-        ctx.stream = op->stream;
-        std::vector<DevBuf<double2>> r(nrhs), d(nrhs);
+        SolveCtx &ctx = op->slot(xtr::kMaxRhs).ctx;   // reduction scratch for the FP64 norms
+        ctx.reset(op->stream);
+        std::vector<DevBuf<double2> *> r(nrhs), d(nrhs);
         for (int k = 0; k < nrhs; ++k) {
-            r[k].alloc(n);
-            d[k].alloc(n);
+            Slot &sk = op->slot(k);
+            grow(sk.r, (size_t)n);
+            grow(sk.d, (size_t)n);
+            r[k] = &sk.r;
+            d[k] = &sk.d;
         }
         auto norm = [&](const double2 *v) {
             xtr::launch_norm2(v, n, ctx.partial_r.p, ctx.stream);
@@ -1379,14 +1467,14 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
             xtr::RhsPtrs v{};
             for (size_t i = 0; i < live.size(); ++i) {
                 v.x[i] = (const double2 *)x[live[i]];
-                v.y[i] = r[live[i]].p;
+                v.y[i] = r[live[i]]->p;
             }
             op->apply_raw(v, (int)live.size(), op->stream);
             std::vector<int> next;
             for (int k : live) {
                 info[k].matvecs64++;
-                xtr::launch_sub((const double2 *)b[k], r[k].p, r[k].p, n, op->stream);
-                rn[k] = bn[k] > 0 ? norm(r[k].p) : 0.0;
+                xtr::launch_sub((const double2 *)b[k], r[k]->p, r[k]->p, n, op->stream);
+                rn[k] = bn[k] > 0 ? norm(r[k]->p) : 0.0;
                 if (!std::isfinite(rn[k])) throw std::runtime_error("the FP64 residual holds NaN or Inf");
                 info[k].residual = bn[k] > 0 ? rn[k] / bn[k] : 0.0;
                 if (info[k].residual <= params->tolerance) info[k].converged = 1;
@@ -1401,9 +1489,9 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
             xtr_gmres_params ip = params->inner;
             double tol_in = 1.0;
             for (int k : live) {
-                XTR_CHECK(hipMemsetAsync(d[k].p, 0, n * sizeof(double2), op->stream));
-                rb.push_back(r[k].p);
-                dx.push_back(d[k].p);
+                XTR_CHECK(hipMemsetAsync(d[k]->p, 0, n * sizeof(double2), op->stream));
+                rb.push_back(r[k]->p);
+                dx.push_back(d[k]->p);
                 tol_in = std::min(tol_in, std::max(params->inner_floor, 0.5 * params->tolerance * bn[k] / rn[k]));
             }
             ip.tolerance = tol_in;
@@ -1417,7 +1505,7 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
                 info[k].inner_iterations += ii[i].number_of_iterations;
                 xtr::Coef32 one{};
                 one.c[0] = make_double2(1.0, 0.0);
-                xtr::launch_update_solution(d[k].p, 1, one, (double2 *)x[k], n, op->stream);   // x += d
+                xtr::launch_update_solution(d[k]->p, 1, one, (double2 *)x[k], n, op->stream);   // x += d
             }
         }
         XTR_CHECK(hipStreamSynchronize(op->stream));
@@ -1667,8 +1755,9 @@ int xtr_fgmres_solve_learned_device(xtr_op *op, const xtr_gmres_params *params, 
                 "the network was made for another grid");
         std::lock_guard<std::mutex> lk(net->mu);
         XTR_CHECK(hipDeviceSynchronize());
-        SolveCtx ctx;
-        ctx.stream = op->stream;
+        Slot &sl = op->slot(0);
+        SolveCtx &ctx = sl.ctx;
+        ctx.reset(op->stream);
         xtr_op *o = op;
         ctx.apply = [o](const double2 *xx, double2 *yy) {
             xtr::RhsPtrs v{};
@@ -1677,7 +1766,7 @@ int xtr_fgmres_solve_learned_device(xtr_op *op, const xtr_gmres_params *params, 
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));
-        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info, net);
+        run_fgmres(op->g.n(), *params, sl, (const double2 *)b, (double2 *)x, *info, net);
     });
 }
 
@@ -1690,8 +1779,9 @@ int xtr_fgmres_solve_harvest_device(xtr_op *op, const xtr_gmres_params *params, 
         require(capacity == 0 || (v_out && z_out), "null harvest buffers");
         require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
         XTR_CHECK(hipDeviceSynchronize());
-        SolveCtx ctx;
-        ctx.stream = op->stream;
+        Slot &sl = op->slot(0);
+        SolveCtx &ctx = sl.ctx;
+        ctx.reset(op->stream);
         ctx.harvest_v = (double2 *const *)v_out;
         ctx.harvest_z = (double2 *const *)z_out;
         ctx.harvest_cap = capacity;
@@ -1703,7 +1793,7 @@ int xtr_fgmres_solve_harvest_device(xtr_op *op, const xtr_gmres_params *params, 
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));
-        run_fgmres(op->g.n(), *params, ctx, (const double2 *)b, (double2 *)x, *info);
+        run_fgmres(op->g.n(), *params, sl, (const double2 *)b, (double2 *)x, *info);
         XTR_CHECK(hipStreamSynchronize(op->stream));
         *count = ctx.harvest_count;
     });

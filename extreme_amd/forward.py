@@ -138,6 +138,22 @@ def apparent_resistivity_phase(z, omega):
     return np.abs(z) ** 2 / (omega * cm.MU0), np.angle(z)
 
 
+def result_errors(got: dict, ref: dict) -> dict:
+    """Relative difference of each observed quantity between two level_responses() results, per
+    quantity over all sites and both polarisations: |got - ref|_2 / |ref|_2 for E and H (full
+    fields), and for Z, rho_a and phase over the off-diagonal elements xy and yx (the diagonal ones
+    vanish over a 1-D earth, so their relative error is not meaningful)."""
+    out = {}
+    for q in ("E", "H"):
+        a, b = np.stack(got[q]), np.stack(ref[q])
+        out[q] = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    off = (Ellipsis, [0, 1], [1, 0])
+    for q in ("Z", "rho", "phase"):
+        a, b = got[q][off], ref[q][off]
+        out[q] = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    return out
+
+
 def level_normal_fields(section: cm.Section1D, omega: float, level_z: float):
     """Mt3DForwardSolver.CalculateNormalFieldE/H(level) (Solver/Mt3DForwardSolver.cs:236-271): pol X
     has E = (E0, 0, 0), H = (0, H0, 0); pol Y has E = (0, E0, 0), H = (-H0, 0, 0)."""
@@ -171,6 +187,7 @@ class FrequencySolve:
         self.op.prepare_operator("A")
         self.solver = X.AnomalyCurrentFgmresSolver(self.op, self.settings)
         self.chi = self.chi0.clone()
+        self.level_z = None
 
     def solve(self):
         """SolveEquationFor for pol X and pol Y: chi <- chi0, then FGMRES (ForwardSolver.cs:300-307)."""
@@ -183,18 +200,30 @@ class FrequencySolve:
         jQ from chi, anomalous E and H by the AtoO convolution (AtoOCalculator.cs:63-160), plus the
         level's normal field; then Z, rho_a and phase per site (the cell centres shifted by
         (x_shift, y_shift) metres)."""
+        self.build_level(level_z, x_shift, y_shift)
+        return self.level_responses()
+
+    def build_level(self, level_z: float = 0.0, x_shift: float = 0.0, y_shift: float = 0.0):
+        """The E and H level tensors of the observation level (AtoOCalculator.cs:124-160), kept in
+        the operator handle for level_responses()."""
+        t0 = time.perf_counter()
+        self.op.calculate_green_atoo(level_z, electric=True, x_shift=x_shift, y_shift=y_shift)
+        self.op.calculate_green_atoo(level_z, electric=False, x_shift=x_shift, y_shift=y_shift)
+        self.atoo_green_seconds += time.perf_counter() - t0          # AtoOGreenCalc, summed over levels
+        self.level_z = level_z
+
+    def level_responses(self, chi=None) -> dict:
+        """E, H, Z, rho_a and phase at the level of the latest build_level(), from chi (both
+        polarisations; the latest solution by default)."""
         torch = self.torch
         op = self.op
-        t0 = time.perf_counter()
-        op.calculate_green_atoo(level_z, electric=True, x_shift=x_shift, y_shift=y_shift)
-        op.calculate_green_atoo(level_z, electric=False, x_shift=x_shift, y_shift=y_shift)
-        self.atoo_green_seconds += time.perf_counter() - t0          # AtoOGreenCalc, summed over levels
-        normals = level_normal_fields(self.model.section, self.omega, level_z)
+        chi = self.chi if chi is None else chi
+        normals = level_normal_fields(self.model.section, self.omega, self.level_z)
         E, H, Ea, Ha = [], [], [], []
         for pol in (0, 1):
             t0 = time.perf_counter()
             js = torch.from_numpy(self.js[pol]).to("cuda")
-            jq = op.jq(self.chi[pol], js)
+            jq = op.jq(chi[pol], js)
             ea = op.level_field(jq, electric=True).cpu().numpy()
             ha = op.level_field(jq, electric=False).cpu().numpy()
             self.atoo_field_seconds[pol] += time.perf_counter() - t0  # AtoOFields of this polarisation

@@ -108,6 +108,7 @@ def lib():
         L.xtr_op_fft_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.xtr_op_green_mirror_check.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
         f32p = C.POINTER(C.c_float)
         vpp = C.POINTER(C.c_void_p)
@@ -419,6 +420,12 @@ class ConvolutionOperator:
         n, gb, bb = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().xtr_op_sizes(self._h, C.byref(n), C.byref(gb), C.byref(bb)))
         return n.value, gb.value, bb.value
+
+    def green_mirror_check(self):
+        """{component: (max mirror deviation, max |G|)} of a handle holding every block (layout "full")."""
+        out = (C.c_double * 12)()
+        _check(lib().xtr_op_green_mirror_check(self._h, out))
+        return {c: (out[2 * k], out[2 * k + 1]) for k, c in enumerate(("xx", "xy", "yy", "zz", "xz", "yz"))}
 
     def green_layout(self):
         """("folded", (Nx+1)(Ny+1)) when the device keeps only the blocks kx <= Nx, ky <= Ny (the

@@ -257,6 +257,14 @@ int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_b
  * blocks = the number of blocks stored. */
 int xtr_op_green_layout(xtr_op *op, int *folded, int64_t *blocks);
 
+/* The mirror symmetry that folding relies on, measured on a handle holding every block (built or set
+ * with XTR_GREEN_LAYOUT=full, or set from a tensor without the symmetry). No counterpart in the
+ * reference, whose populator writes the mirrored blocks (GreenTensorFftPopulator.cs:88-148).
+ * out12[2c] = max over wavenumbers l with kx > Nx or ky > Ny of |G_c(l) - s_c G_c(mirror source)|
+ * (real and imaginary parts separately), out12[2c + 1] = max |G_c|, for c in device block order
+ * xx, xy, yy, zz, xz, yz, with the signs s_c of the component's parities. */
+int xtr_op_green_mirror_check(xtr_op *op, double *out12);
+
 /* Replaces: ConvolutionOperator.Dispose / GreenTensor.Dispose (ConvolutionOperator.cs:338-344). */
 void xtr_op_destroy(xtr_op *op);
 
