@@ -165,6 +165,7 @@ struct xtr_op {
     int number_of_hankels = 10;
     std::vector<double> hank0, hank1;
 
+    int device = 0;               // the device current at xtr_op_create (the stream's)
     hipStream_t stream = nullptr;
     DevBuf<double2> G64;          // [nblk][gblk] FP64 Green tensor
     DevBuf<float2> G32;           // [nblk][gblk] complex64 copy for XTR_FP32 and XTR_MIXED
@@ -1002,6 +1003,7 @@ int xtr_op_create(const xtr_op_desc *d, xtr_op **out)
             op->hank1.assign(d->hank1, d->hank1 + d->ntaps);
         }
         for (int k = 0; k < d->nz; ++k) background_layer(op->section, op->depth[k], op->depth[k] + op->thick[k]);
+        XTR_CHECK(hipGetDevice(&op->device));
         XTR_CHECK(hipStreamCreateWithFlags(&op->stream, hipStreamDefault));
         const xtr::Geom &g = op->g;
         // packed (rc, tr) table in CombineSymm order (TranceiverUtils.cs:65-74)
@@ -1572,6 +1574,59 @@ int xtr_op_jq_device(xtr_op *op, const void *chi, const void *js, void *jq)
         return -1;
     }
     return xtr_op_jq_model_device(op, 0, chi, js, jq);
+}
+
+}  // extern "C"
+
+// ---- allocation-free level field ---------------------------------------------------------------
+// Scratch of xtr_op_level_field_noalloc_device: the zero-padded spectrum of jQ (4 Nx Ny * 3 Nz) and
+// the level spectrum (4 Nx Ny * 3), complex128, in one file-scope device array of 128 MiB: grids with
+// 4 Nx Ny * 3 (Nz + 1) <= 8 Mi elements (128 x 128 x 32 needs 103.8 MB). Each GPU the library runs on
+// holds its own copy until the process ends. In host code the array's name is not a device address:
+// the host uses only the address hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr size_t kScratchElems = (size_t)8 << 20;                     // complex128 elements
+static __device__ __align__(256) double scratch[2 * kScratchElems];   // re, im interleaved
+std::mutex g_scratch_mu;
+std::map<int, double2 *> g_scratch_ptr;                               // device ordinal -> address
+
+// The current device's copy of scratch, looked up once per device (with that device current).
+double2 *scratch_on_device(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *p = nullptr;
+    XTR_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (double2 *)p;
+    return (double2 *)p;
+}
+}  // namespace
+
+extern "C" {
+
+int xtr_op_level_field_noalloc_device(xtr_op *op, int electric, const void *jq, void *field)
+{
+    return guarded([&] {
+        require(op && jq && field, "null argument");
+        const xtr::Geom &g = op->g;
+        const size_t need = (size_t)g.M() + 3 * (size_t)g.L();
+        if (need > kScratchElems)
+            throw std::invalid_argument("the level field of this grid needs " + std::to_string(need * 16 >> 20) +
+                                        " MiB of scratch, more than the 128 MiB of the allocation-free entry point "
+                                        "(4 Nx Ny * 3 (Nz + 1) must not exceed 8388608); use xtr_op_level_field_device");
+        const int e = electric ? 1 : 0;
+        require(op->go_ready[e], "build the level tensor first (xtr_op_build_green_atoo)");
+        int dev = 0;
+        XTR_CHECK(hipGetDevice(&dev));
+        require(dev == op->device, "the handle was created on another device than the current one");
+        double2 *ptr = scratch_on_device(dev);
+        XTR_CHECK(hipDeviceSynchronize());   // jQ may have been written on another stream
+        xtr::launch_level_field(op->Go[e].p, e, (const double2 *)jq, g, ptr, ptr + g.M(), op->tw64_x.p, op->tw64_y.p,
+                                (double2 *)field, op->stream);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    });
 }
 
 int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *field)

@@ -103,6 +103,7 @@ def lib():
         L.xtr_op_jq_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.xtr_op_jq_model_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.xtr_op_level_field_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.xtr_op_level_field_noalloc_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_fft_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int)]

@@ -243,6 +243,15 @@ int xtr_op_jq_model_device(xtr_op *op, int model, const void *chi, const void *j
  * = 1) or H at the level, field [Nx][Ny][3] on the device, from the tensor built above. */
 int xtr_op_level_field_device(xtr_op *op, int electric, const void *jq, void *field);
 
+/* xtr_op_level_field_device that allocates nothing, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid allocation). No counterpart in
+ * the reference. Its kernels take their scratch (the zero-padded spectrum of jQ and the level
+ * spectrum, 4 Nx Ny * 3 (Nz + 1) complex128) from a 128 MiB array the library holds on each device
+ * until the process ends; grids needing more are refused with an error (4 Nx Ny * 3 (Nz + 1) <=
+ * 8388608: 128 x 128 x 32 fits, 256 x 256 x 64 does not). Every call on a device shares that array:
+ * calls that overlap on one device race, so a caller must not run two at once on one GPU. */
+int xtr_op_level_field_noalloc_device(xtr_op *op, int electric, const void *jq, void *field);
+
 /* Sizes: n = 3 Nx Ny Nz, the Green tensor's device bytes, the spectrum buffer bytes. */
 int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_bytes);
 
