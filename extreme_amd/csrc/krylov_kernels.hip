@@ -1,5 +1,10 @@
-// Krylov-basis kernels for the FGMRES driver (FP64): block dot V^H w, the classical
-// Gram-Schmidt update with the new norm, norms, scaling and the solution update.
+// Krylov-basis kernels for the FGMRES driver: block dot V^H w, the classical Gram-Schmidt update
+// with the new norm, norms, scaling and the solution update.
+//
+// Vectors are complex128 (the reference's arithmetic) or complex64 (the correction solves of the
+// mixed-precision refinement): every kernel is a template on the stored element type and widens each
+// element to FP64 on load, so products, dot sums and norms accumulate in FP64 either way; complex64
+// results are rounded once, on store.
 //
 // Reductions are two-stage and deterministic: kRedBlocks workgroups write per-block
 // partials (grid-stride loops, wavefront shuffles, then LDS), and one workgroup per output
@@ -11,6 +16,11 @@ namespace xtr {
 
 constexpr int kThreads = 256;
 constexpr int kDotChunk = 8;   // basis columns per pass of block_dots
+
+__device__ __forceinline__ double2 ld(const double2 &v) { return v; }
+__device__ __forceinline__ double2 ld(const float2 &v) { return to_d(v); }
+__device__ __forceinline__ void st(double2 &d, double2 v) { d = v; }
+__device__ __forceinline__ void st(float2 &d, double2 v) { d = to_f(v); }
 
 __device__ inline double warp_sum(double v)
 {
@@ -34,8 +44,9 @@ __device__ inline double block_sum(double v, double *sh)
 }
 
 // partial[(k0 + k) * kRedBlocks + blk] = sum_i conj(V[k][i]) w[i] over this block's i
-__global__ __launch_bounds__(kThreads) void block_dots_kernel(const double2 *__restrict__ V, int cnt,
-                                                              const double2 *__restrict__ w, int64_t n,
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void block_dots_kernel(const VT *__restrict__ V, int cnt,
+                                                              const VT *__restrict__ w, int64_t n,
                                                               double2 *partial)
 {
     __shared__ double sh[kThreads / 64];
@@ -45,10 +56,10 @@ __global__ __launch_bounds__(kThreads) void block_dots_kernel(const double2 *__r
 #pragma unroll
     for (int k = 0; k < kDotChunk; ++k) acc[k] = make_double2(0, 0);
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const double2 wv = w[i];
+        const double2 wv = ld(w[i]);
 #pragma unroll
         for (int k = 0; k < kDotChunk; ++k)
-            if (k < kc) acc[k] = cfma_conj<double>(acc[k], V[(int64_t)(k0 + k) * n + i], wv);
+            if (k < kc) acc[k] = cfma_conj<double>(acc[k], ld(V[(int64_t)(k0 + k) * n + i]), wv);
     }
 #pragma unroll
     for (int k = 0; k < kDotChunk; ++k) {
@@ -59,11 +70,20 @@ __global__ __launch_bounds__(kThreads) void block_dots_kernel(const double2 *__r
     }
 }
 
-void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s)
+template <typename VT>
+static void block_dots(const VT *V, int cnt, const VT *w, int64_t n, double2 *partial, hipStream_t s)
 {
     dim3 grid(kRedBlocks, (unsigned)ceil_div(cnt, kDotChunk));
-    hipLaunchKernelGGL(block_dots_kernel, grid, dim3(kThreads), 0, s, V, cnt, w, n, partial);
+    hipLaunchKernelGGL(block_dots_kernel<VT>, grid, dim3(kThreads), 0, s, V, cnt, w, n, partial);
     XTR_LAUNCH_CHECK("block_dots_kernel");
+}
+void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s)
+{
+    block_dots(V, cnt, w, n, partial, s);
+}
+void launch_block_dots(const float2 *V, int cnt, const float2 *w, int64_t n, double2 *partial, hipStream_t s)
+{
+    block_dots(V, cnt, w, n, partial, s);
 }
 
 // out[k] = sum_b partial[k][b], one workgroup per k, fixed order; also into host_out[k] (pinned,
@@ -92,9 +112,10 @@ void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t 
     XTR_LAUNCH_CHECK("reduce_c_kernel");
 }
 
-// w[i] -= sum_k V[k][i] h[k]; partial[blk] = sum |w_new|^2
-__global__ __launch_bounds__(kThreads) void cgs_update_kernel(const double2 *__restrict__ V, int cnt,
-                                                              const double2 *__restrict__ h, double2 *w,
+// w[i] -= sum_k V[k][i] h[k]; partial[blk] = sum |w_new|^2 (of the stored, rounded w)
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void cgs_update_kernel(const VT *__restrict__ V, int cnt,
+                                                              const double2 *__restrict__ h, VT *w,
                                                               int64_t n, double *partial)
 {
     __shared__ double sh[kThreads / 64];
@@ -104,9 +125,11 @@ __global__ __launch_bounds__(kThreads) void cgs_update_kernel(const double2 *__r
     double nn = 0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         double2 a = make_double2(0, 0);
-        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, V[(int64_t)k * n + i], hs[k]);
-        const double2 nw = csub<double>(w[i], a);
-        w[i] = nw;
+        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, ld(V[(int64_t)k * n + i]), hs[k]);
+        VT t;
+        st(t, csub<double>(ld(w[i]), a));
+        w[i] = t;
+        const double2 nw = ld(t);
         nn = fma(nw.x, nw.x, nn);
         nn = fma(nw.y, nw.y, nn);
     }
@@ -114,20 +137,31 @@ __global__ __launch_bounds__(kThreads) void cgs_update_kernel(const double2 *__r
     if (threadIdx.x == 0) partial[blockIdx.x] = nn;
 }
 
+template <typename VT>
+static void cgs_update(const VT *V, int cnt, const double2 *h, VT *w, int64_t n, double *partial, hipStream_t s)
+{
+    if (cnt > 32) throw hip_error("cgs_update: more than 32 basis vectors");
+    hipLaunchKernelGGL(cgs_update_kernel<VT>, dim3(kRedBlocks), dim3(kThreads), 0, s, V, cnt, h, w, n, partial);
+    XTR_LAUNCH_CHECK("cgs_update_kernel");
+}
 void launch_cgs_update(const double2 *V, int cnt, const double2 *h, double2 *w, int64_t n, double *partial,
                        hipStream_t s)
 {
-    if (cnt > 32) throw hip_error("cgs_update: more than 32 basis vectors");
-    hipLaunchKernelGGL(cgs_update_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, V, cnt, h, w, n, partial);
-    XTR_LAUNCH_CHECK("cgs_update_kernel");
+    cgs_update(V, cnt, h, w, n, partial, s);
+}
+void launch_cgs_update(const float2 *V, int cnt, const double2 *h, float2 *w, int64_t n, double *partial,
+                       hipStream_t s)
+{
+    cgs_update(V, cnt, h, w, n, partial, s);
 }
 
-__global__ __launch_bounds__(kThreads) void norm2_kernel(const double2 *__restrict__ w, int64_t n, double *partial)
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void norm2_kernel(const VT *__restrict__ w, int64_t n, double *partial)
 {
     __shared__ double sh[kThreads / 64];
     double nn = 0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const double2 v = w[i];
+        const double2 v = ld(w[i]);
         nn = fma(v.x, v.x, nn);
         nn = fma(v.y, v.y, nn);
     }
@@ -137,7 +171,12 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double2 *__restri
 
 void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s)
 {
-    hipLaunchKernelGGL(norm2_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, w, n, partial);
+    hipLaunchKernelGGL(norm2_kernel<double2>, dim3(kRedBlocks), dim3(kThreads), 0, s, w, n, partial);
+    XTR_LAUNCH_CHECK("norm2_kernel");
+}
+void launch_norm2(const float2 *w, int64_t n, double *partial, hipStream_t s)
+{
+    hipLaunchKernelGGL(norm2_kernel<float2>, dim3(kRedBlocks), dim3(kThreads), 0, s, w, n, partial);
     XTR_LAUNCH_CHECK("norm2_kernel");
 }
 
@@ -159,60 +198,94 @@ void launch_reduce_r(const double *partial, double *out, hipStream_t s, double *
     XTR_LAUNCH_CHECK("reduce_r_kernel");
 }
 
-__global__ void div_kernel(const double2 *src, double d, double2 *dst, int64_t n)
+template <typename VT>
+__global__ void div_kernel(const VT *src, double d, VT *dst, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 v = src[i];
-        dst[i] = make_double2(v.x / d, v.y / d);
+        const double2 v = ld(src[i]);
+        st(dst[i], make_double2(v.x / d, v.y / d));
     }
 }
 
 void launch_div(const double2 *src, double d, double2 *dst, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(div_kernel, dim3(2048), dim3(256), 0, s, src, d, dst, n);
+    hipLaunchKernelGGL(div_kernel<double2>, dim3(2048), dim3(256), 0, s, src, d, dst, n);
+    XTR_LAUNCH_CHECK("div_kernel");
+}
+void launch_div(const float2 *src, double d, float2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(div_kernel<float2>, dim3(2048), dim3(256), 0, s, src, d, dst, n);
     XTR_LAUNCH_CHECK("div_kernel");
 }
 
-__global__ void mul_kernel(const double2 *src, double a, double2 *dst, int64_t n)
+template <typename VS, typename VD>
+__global__ void mul_kernel(const VS *src, double a, VD *dst, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 v = src[i];
-        dst[i] = make_double2(a * v.x, a * v.y);
+        const double2 v = ld(src[i]);
+        st(dst[i], make_double2(a * v.x, a * v.y));
     }
 }
 
 void launch_mul(const double2 *src, double a, double2 *dst, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(mul_kernel, dim3(2048), dim3(256), 0, s, src, a, dst, n);
+    hipLaunchKernelGGL((mul_kernel<double2, double2>), dim3(2048), dim3(256), 0, s, src, a, dst, n);
+    XTR_LAUNCH_CHECK("mul_kernel");
+}
+void launch_mul(const float2 *src, double a, float2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((mul_kernel<float2, float2>), dim3(2048), dim3(256), 0, s, src, a, dst, n);
+    XTR_LAUNCH_CHECK("mul_kernel");
+}
+void launch_mul(const double2 *src, double a, float2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((mul_kernel<double2, float2>), dim3(2048), dim3(256), 0, s, src, a, dst, n);
     XTR_LAUNCH_CHECK("mul_kernel");
 }
 
-__global__ void sub_kernel(const double2 *a, const double2 *b, double2 *dst, int64_t n)
+template <typename VT>
+__global__ void sub_kernel(const VT *a, const VT *b, VT *dst, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = csub<double>(a[i], b[i]);
+        st(dst[i], csub<double>(ld(a[i]), ld(b[i])));
 }
 
 void launch_sub(const double2 *a, const double2 *b, double2 *dst, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(sub_kernel, dim3(2048), dim3(256), 0, s, a, b, dst, n);
+    hipLaunchKernelGGL(sub_kernel<double2>, dim3(2048), dim3(256), 0, s, a, b, dst, n);
+    XTR_LAUNCH_CHECK("sub_kernel");
+}
+void launch_sub(const float2 *a, const float2 *b, float2 *dst, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(sub_kernel<float2>, dim3(2048), dim3(256), 0, s, a, b, dst, n);
     XTR_LAUNCH_CHECK("sub_kernel");
 }
 
 // x[i] += sum_k B[k][i] y[k]  (ConstructCurrentSolution: r0 = Z y; x = x + r0)
-__global__ __launch_bounds__(kThreads) void update_solution_kernel(const double2 *__restrict__ B, int cnt, Coef32 y,
-                                                                   double2 *x, int64_t n)
+template <typename VB, typename VX>
+__global__ __launch_bounds__(kThreads) void update_solution_kernel(const VB *__restrict__ B, int cnt, Coef32 y,
+                                                                   VX *x, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         double2 a = make_double2(0, 0);
-        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, B[(int64_t)k * n + i], y.c[k]);
-        x[i] = cadd<double>(x[i], a);
+        for (int k = 0; k < cnt; ++k) a = cfma<double>(a, ld(B[(int64_t)k * n + i]), y.c[k]);
+        st(x[i], cadd<double>(ld(x[i]), a));
     }
 }
 
 void launch_update_solution(const double2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(update_solution_kernel, dim3(2048), dim3(kThreads), 0, s, B, cnt, y, x, n);
+    hipLaunchKernelGGL((update_solution_kernel<double2, double2>), dim3(2048), dim3(kThreads), 0, s, B, cnt, y, x, n);
+    XTR_LAUNCH_CHECK("update_solution_kernel");
+}
+void launch_update_solution(const float2 *B, int cnt, const Coef32 &y, float2 *x, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((update_solution_kernel<float2, float2>), dim3(2048), dim3(kThreads), 0, s, B, cnt, y, x, n);
+    XTR_LAUNCH_CHECK("update_solution_kernel");
+}
+void launch_update_solution(const float2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((update_solution_kernel<float2, double2>), dim3(2048), dim3(kThreads), 0, s, B, cnt, y, x, n);
     XTR_LAUNCH_CHECK("update_solution_kernel");
 }
 

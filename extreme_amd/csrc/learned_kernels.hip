@@ -289,6 +289,193 @@ __global__ void weights_kernel(const float *__restrict__ w1, const float *__rest
     }
 }
 
+// ---- spectral branch ---------------------------------------------------------------------------------
+// Per-lateral-mode channel mixing on the zero-padded 2Nx x 2Ny spectrum, modes |kx|, |ky| <= K. The
+// truncated transforms are separable sums with FP32 twiddle tables (t = k n mod 2N); the mixing of
+// the 3 Nz complex channels of each mode is a real GEMM on v_mfma_f32_16x16x16_bf16 in the block
+// form [Wr -Wi; Wi Wr] [Re U; Im U], A = BF16(W0 + phi W1) per element, B = BF16(U).
+constexpr int kSpecMaxK = 12;
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cmulf_conj(float2 a, float2 b)   // a conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ int twidx(int kk, int n, int p)   // (kk n) mod p, kk may be negative
+{
+    int t = (kk * n) % p;
+    return t < 0 ? t + p : t;
+}
+__device__ __forceinline__ float chan(const unsigned short *src, int64_t e) { return bf2f(src[e]); }
+__device__ __forceinline__ float chan(const float *src, int64_t e) { return src[e]; }
+
+// T[i][b][q] = sum_j u(i, j, q) e^{-2 pi i ky j / 2Ny}, ky = b - K; u from channels 2c, 2c + 1 of
+// src[cells][8] (q = c Nz + k). One thread per (i, q).
+template <typename SRC>
+__global__ __launch_bounds__(256) void spec_dft_y_kernel(SpecGeom g, const SRC *__restrict__ src,
+                                                         const float2 *__restrict__ twy, float2 *__restrict__ T)
+{
+    const int Q = g.q(), nb = 2 * g.K + 1, py = 2 * g.ny;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)g.nx * Q) return;
+    const int i = (int)(e / Q), q = (int)(e % Q), c = q / g.nz, k = q % g.nz;
+    float2 acc[2 * kSpecMaxK + 1];
+#pragma unroll
+    for (int b = 0; b < 2 * kSpecMaxK + 1; ++b) acc[b] = make_float2(0, 0);
+    for (int j = 0; j < g.ny; ++j) {
+        const int64_t cell = ((int64_t)i * g.ny + j) * g.nz + k;
+        const float2 u = make_float2(chan(src, cell * 8 + 2 * c), chan(src, cell * 8 + 2 * c + 1));
+#pragma unroll
+        for (int b = 0; b < 2 * kSpecMaxK + 1; ++b)
+            if (b < nb) {
+                const float2 w = twy[twidx(b - g.K, j, py)];
+                acc[b].x = fmaf(u.x, w.x, fmaf(-u.y, w.y, acc[b].x));
+                acc[b].y = fmaf(u.x, w.y, fmaf(u.y, w.x, acc[b].y));
+            }
+    }
+#pragma unroll
+    for (int b = 0; b < 2 * kSpecMaxK + 1; ++b)
+        if (b < nb) T[((int64_t)i * nb + b) * Q + q] = acc[b];
+}
+
+// U[a nb + b][q] = s0 sum_i T[i][b][q] e^{-2 pi i kx i / 2Nx}, kx = a - K, rounded to BF16 values when
+// round (the MFMA operand). One thread per (b, q).
+__global__ __launch_bounds__(256) void spec_dft_x_kernel(SpecGeom g, const float2 *__restrict__ T,
+                                                         const float2 *__restrict__ twx, float s0,
+                                                         float2 *__restrict__ U, int round)
+{
+    const int Q = g.q(), nb = 2 * g.K + 1, px = 2 * g.nx;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * Q) return;
+    const int b = e / Q, q = e % Q;
+    float2 acc[2 * kSpecMaxK + 1];
+#pragma unroll
+    for (int a = 0; a < 2 * kSpecMaxK + 1; ++a) acc[a] = make_float2(0, 0);
+    for (int i = 0; i < g.nx; ++i) {
+        const float2 t = T[((int64_t)i * nb + b) * Q + q];
+#pragma unroll
+        for (int a = 0; a < 2 * kSpecMaxK + 1; ++a)
+            if (a < nb) {
+                const float2 w = twx[twidx(a - g.K, i, px)];
+                acc[a].x = fmaf(t.x, w.x, fmaf(-t.y, w.y, acc[a].x));
+                acc[a].y = fmaf(t.x, w.y, fmaf(t.y, w.x, acc[a].y));
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 2 * kSpecMaxK + 1; ++a)
+        if (a < nb) {
+            float2 v = make_float2(s0 * acc[a].x, s0 * acc[a].y);
+            if (round) v = make_float2(bf2f(f2bf(v.x)), bf2f(f2bf(v.y)));
+            U[((int64_t)a * nb + b) * Q + q] = v;
+        }
+}
+
+// Y[m] = A U[m] with A the 2Q x 2Q real block form of W0[m] + phi W1[m] (BF16 per element): one wave
+// per (mode, 16-row tile), K = 2Q in chunks of 16, one live B column (lane % 16 == 0).
+__global__ __launch_bounds__(256) void spec_mix_kernel(SpecGeom g, const unsigned short *__restrict__ WB, float phi,
+                                                       const float2 *__restrict__ U, float2 *__restrict__ Y)
+{
+    const int Q = g.q(), Q2 = 2 * Q, tiles = Q2 / 16, nm = g.nm();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t wg = (int64_t)blockIdx.x * 4 + wave;
+    if (wg >= (int64_t)nm * tiles) return;             // wave-uniform
+    const int m = (int)(wg / tiles), tile = (int)(wg % tiles);
+    const int row = lane & 15, kg = lane >> 4;
+    const int R = tile * 16 + row, qr = R % Q;
+    const bool top = R < Q;
+    const unsigned short *w0 = WB + ((int64_t)m * Q + qr) * Q * 2;
+    const unsigned short *w1 = WB + (((int64_t)nm + m) * Q + qr) * Q * 2;
+    const float2 *u = U + (int64_t)m * Q;
+    f32x4 acc = {0, 0, 0, 0};
+    for (int ch = 0; ch < tiles; ++ch) {
+        const int k0 = ch * 16 + 4 * kg;
+        bf16x4 a, bv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kk = k0 + r, qk = kk % Q;
+            const bool left = kk < Q;
+            const float wr = bf2f(w0[2 * qk]) + phi * bf2f(w1[2 * qk]);
+            const float wi = bf2f(w0[2 * qk + 1]) + phi * bf2f(w1[2 * qk + 1]);
+            const float val = top ? (left ? wr : -wi) : (left ? wi : wr);
+            a[r] = (short)f2bf(val);
+            const float ub = row == 0 ? (left ? u[qk].x : u[qk].y) : 0.0f;
+            bv[r] = (short)f2bf(ub);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    }
+    if (row == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {                      // D[4 kg + r][0]
+            const int RR = tile * 16 + 4 * kg + r;
+            float *dst = reinterpret_cast<float *>(Y + (int64_t)m * Q + (RR % Q)) + (RR < Q ? 0 : 1);
+            *dst = acc[r];
+        }
+    }
+}
+
+// T[i][b][q] = sum_a Y[a nb + b][q] e^{+2 pi i kx i / 2Nx}. One thread per (i, b, q).
+__global__ __launch_bounds__(256) void spec_idft_x_kernel(SpecGeom g, const float2 *__restrict__ Y,
+                                                          const float2 *__restrict__ twx, float2 *__restrict__ T)
+{
+    const int Q = g.q(), nb = 2 * g.K + 1, px = 2 * g.nx;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)g.nx * nb * Q) return;
+    const int q = (int)(e % Q), b = (int)((e / Q) % nb), i = (int)(e / ((int64_t)Q * nb));
+    float2 acc = make_float2(0, 0);
+    for (int a = 0; a < nb; ++a) {
+        const float2 y = Y[((int64_t)a * nb + b) * Q + q];
+        const float2 w = twx[twidx(a - g.K, i, px)];
+        acc = make_float2(fmaf(y.x, w.x, fmaf(y.y, w.y, acc.x)), fmaf(y.y, w.x, fmaf(-y.x, w.y, acc.y)));   // y conj(w)
+    }
+    T[e] = acc;
+}
+
+// y[cell][2c], y[cell][2c + 1] += Re, Im of s0 sum_b T[i][b][q] e^{+2 pi i ky j / 2Ny}. One thread per
+// (i, j, q).
+__global__ __launch_bounds__(256) void spec_idft_y_kernel(SpecGeom g, const float2 *__restrict__ T,
+                                                          const float2 *__restrict__ twy, float s0, float *y)
+{
+    const int Q = g.q(), nb = 2 * g.K + 1, py = 2 * g.ny;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)g.nx * g.ny * Q) return;
+    const int q = (int)(e % Q);
+    const int64_t ij = e / Q;
+    const int j = (int)(ij % g.ny), i = (int)(ij / g.ny), c = q / g.nz, k = q % g.nz;
+    float2 acc = make_float2(0, 0);
+    for (int b = 0; b < nb; ++b) {
+        const float2 t = T[((int64_t)i * nb + b) * Q + q];
+        const float2 w = twy[twidx(b - g.K, j, py)];
+        acc = make_float2(fmaf(t.x, w.x, fmaf(t.y, w.y, acc.x)), fmaf(t.y, w.x, fmaf(-t.x, w.y, acc.y)));
+    }
+    const int64_t cell = ij * g.nz + k;
+    y[cell * 16 + 2 * c] += s0 * acc.x;
+    y[cell * 16 + 2 * c + 1] += s0 * acc.y;
+}
+
+// grad[w][m][a][b] (+)= scale f_w gY[m][a] conj(U[m][b]), f_0 = 1, f_1 = phi; complex entries (re, im).
+__global__ __launch_bounds__(256) void spec_wgrad_kernel(SpecGeom g, const float2 *__restrict__ gY,
+                                                         const float2 *__restrict__ U, float phi, float scale,
+                                                         float *__restrict__ grad, int accumulate)
+{
+    const int Q = g.q();
+    const int64_t per = (int64_t)g.nm() * Q * Q;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per) return;
+    const int b = (int)(e % Q), a = (int)((e / Q) % Q);
+    const int64_t m = e / ((int64_t)Q * Q);
+    const float2 v = cmulf_conj(gY[m * Q + a], U[m * Q + b]);
+    float2 *g0 = reinterpret_cast<float2 *>(grad) + e, *g1 = reinterpret_cast<float2 *>(grad) + per + e;
+    const float2 d0 = make_float2(scale * v.x, scale * v.y), d1 = make_float2(scale * phi * v.x, scale * phi * v.y);
+    *g0 = accumulate ? make_float2(g0->x + d0.x, g0->y + d0.y) : d0;
+    *g1 = accumulate ? make_float2(g1->x + d1.x, g1->y + d1.y) : d1;
+}
+
+__global__ void spec_weights_kernel(int64_t n, const float *__restrict__ w, unsigned short *__restrict__ WB)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) WB[e] = f2bf(w[e]);
+}
+
 }  // namespace lp
 
 using namespace lp;
@@ -372,6 +559,57 @@ void net_weights(const float *w1, const float *w2, const NetBuffers &b, hipStrea
 {
     hipLaunchKernelGGL(weights_kernel, dim3(grid_of(27 * 16 * 16, 256)), dim3(256), 0, s, w1, w2, b.W1, b.W2, b.WT2);
     XTR_LAUNCH_CHECK("weights_kernel");
+}
+
+// ---- spectral branch launchers ------------------------------------------------------------------------
+
+template <typename SRC>
+static void spec_transform(const SpecGeom &g, const SRC *src, const SpecBuffers &sb, float2 *out, int round,
+                           hipStream_t s)
+{
+    if (g.K < 1 || g.K > kSpecMaxK) throw hip_error("spectral modes K must be 1..12");
+    const int Q = g.q(), nb = 2 * g.K + 1;
+    const float s0 = (float)(1.0 / std::sqrt((double)g.nx * (double)g.ny));
+    hipLaunchKernelGGL(spec_dft_y_kernel<SRC>, dim3(grid_of((int64_t)g.nx * Q, 256)), dim3(256), 0, s, g, src, sb.twy,
+                       sb.T);
+    XTR_LAUNCH_CHECK("spec_dft_y_kernel");
+    hipLaunchKernelGGL(spec_dft_x_kernel, dim3(grid_of((int64_t)nb * Q, 256)), dim3(256), 0, s, g, sb.T, sb.twx, s0,
+                       out, round);
+    XTR_LAUNCH_CHECK("spec_dft_x_kernel");
+}
+
+void spec_forward(const SpecGeom &g, const NetBuffers &b, const SpecBuffers &sb, float phi, hipStream_t s)
+{
+    if (g.q() % 8 != 0) throw hip_error("the spectral branch needs Nz divisible by 8");
+    const int Q = g.q(), nb = 2 * g.K + 1;
+    spec_transform(g, b.x0, sb, sb.U, 1, s);
+    const int64_t waves = (int64_t)g.nm() * (2 * Q / 16);
+    hipLaunchKernelGGL(spec_mix_kernel, dim3(grid_of(waves, 4)), dim3(256), 0, s, g, sb.WB, phi, sb.U, sb.Y);
+    XTR_LAUNCH_CHECK("spec_mix_kernel");
+    hipLaunchKernelGGL(spec_idft_x_kernel, dim3(grid_of((int64_t)g.nx * nb * Q, 256)), dim3(256), 0, s, g, sb.Y, sb.twx,
+                       sb.T);
+    XTR_LAUNCH_CHECK("spec_idft_x_kernel");
+    const float s0 = (float)(1.0 / std::sqrt((double)g.nx * (double)g.ny));
+    hipLaunchKernelGGL(spec_idft_y_kernel, dim3(grid_of((int64_t)g.nx * g.ny * Q, 256)), dim3(256), 0, s, g, sb.T,
+                       sb.twy, s0, b.y);
+    XTR_LAUNCH_CHECK("spec_idft_y_kernel");
+}
+
+void spec_backward(const SpecGeom &g, const NetBuffers &b, const SpecBuffers &sb, float phi, float *grad, float scale,
+                   int accumulate, hipStream_t s)
+{
+    spec_transform(g, b.dy, sb, sb.gY, 0, s);
+    const int64_t per = (int64_t)g.nm() * g.q() * g.q();
+    hipLaunchKernelGGL(spec_wgrad_kernel, dim3(grid_of(per, 256)), dim3(256), 0, s, g, sb.gY, sb.U, phi, scale, grad,
+                       accumulate);
+    XTR_LAUNCH_CHECK("spec_wgrad_kernel");
+}
+
+void spec_weights(const SpecGeom &g, const float *w, unsigned short *WB, hipStream_t s)
+{
+    const int64_t n = 2 * g.weights();
+    hipLaunchKernelGGL(spec_weights_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, n, w, WB);
+    XTR_LAUNCH_CHECK("spec_weights_kernel");
 }
 
 }  // namespace xtr

@@ -191,10 +191,14 @@ struct IoPlain {
     }
 };
 
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+__device__ __forceinline__ double2 widen(float2 v) { return to_d(v); }
+
 // First forward pass along y, fused with ApplyOperatorR and PrepareForForwardFft
-// (ConvolutionOperator.cs:233-272): reads x and R, writes f * R * x into rows i < Nx of the
-// padded buffer; columns c = i*3Nz + col; points j < Ny are data, j >= Ny are the padding.
-template <typename T>
+// (ConvolutionOperator.cs:233-272): reads x (complex128, or complex64 when V = float) and R, writes
+// f * R * x into rows i < Nx of the padded buffer; columns c = i*3Nz + col; points j < Ny are data,
+// j >= Ny are the padding. R * x is formed in FP64.
+template <typename T, typename V>
 struct IoFwdY {
     RhsPtrs v;                // x, and R per right-hand side
     const double *f;
@@ -208,7 +212,8 @@ struct IoFwdY {
         const int64_t i = c / C, col = c % C;
         const int k = (int)(col % nz);
         const int64_t ij = i * ny + p;
-        double2 rx = cmul<double>(v.R[rhs][ij * nz + k], v.x[rhs][ij * C + col]);
+        const cplx<V> *x = reinterpret_cast<const cplx<V> *>(v.x[rhs]);
+        double2 rx = cmul<double>(v.R[rhs][ij * nz + k], widen(x[ij * C + col]));
         return from_d<T>(cscale<double>(rx, f[k]));
     }
     __device__ void store(int rhs, int64_t c, int p, cplx<T> val) const
@@ -222,8 +227,9 @@ struct IoFwdY {
 
 // Last inverse pass along y, fused with ExtractData and PerformLastStep
 // (ConvolutionOperator.cs:187-231): keeps j < Ny of rows i < Nx, scales by b, and writes
-// y = x - b*conv - R*x (A) or y = b*conv + R*x (Chi0) in FP64.
-template <typename T>
+// y = x - b*conv - R*x (A) or y = b*conv + R*x (Chi0), formed in FP64 and stored as complex128 (or
+// complex64 when V = float).
+template <typename T, typename V>
 struct IoInvY {
     RhsPtrs v;                // x, y, and R per right-hand side
     const double *b;
@@ -247,12 +253,12 @@ struct IoInvY {
         double2 conv;
         if constexpr (sizeof(T) == 8) conv = cscale<double>(val, b[k]);
         else conv = cscale<double>(to_d(val), b[k]);
-        const double2 xv = v.x[rhs][t];
+        const double2 xv = widen(reinterpret_cast<const cplx<V> *>(v.x[rhs])[t]);
         const double2 rx = cmul<double>(v.R[rhs][ij * nz + k], xv);
         double2 out;
         if (type == 0) out = csub<double>(csub<double>(xv, conv), rx);
         else out = cadd<double>(conv, rx);
-        v.y[rhs][t] = out;
+        reinterpret_cast<cplx<V> *>(v.y[rhs])[t] = from_d<V>(out);
     }
 };
 
@@ -560,10 +566,10 @@ static RhsPtrs with_r(const ApplyArgs &a)
     return v;
 }
 
-template <typename T>
+template <typename T, typename V>
 static void fwd_y(const ApplyArgs &a, hipStream_t s)
 {
-    IoFwdY<T> io{with_r(a), a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
+    IoFwdY<T, V> io{with_r(a), a.f, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz};
     launch_pencil<T, -1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s, a.fft_tile[0]);
 }
 
@@ -583,17 +589,35 @@ static void inv_x(const ApplyArgs &a, hipStream_t s)
     launch_pencil<T, +1>(2 * a.g.nx, io, ncols, a.nrhs, a.tw_x, s, a.fft_tile[2]);
 }
 
-template <typename T>
+template <typename T, typename V>
 static void inv_y(const ApplyArgs &a, hipStream_t s)
 {
-    IoInvY<T> io{with_r(a), a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
+    IoInvY<T, V> io{with_r(a), a.b, (cplx<T> *)a.buf, a.g.M(), a.g.nx, a.g.ny, a.g.nz, a.type};
     launch_pencil<T, +1>(2 * a.g.ny, io, (int64_t)a.g.nx * a.g.C(), a.nrhs, a.tw_y, s, a.fft_tile[3]);
 }
 
-void launch_fwd_y(const ApplyArgs &a, hipStream_t s) { a.precision ? fwd_y<float>(a, s) : fwd_y<double>(a, s); }
+// complex64 vectors go with the FP32 apply only
+static void check_vec32(const ApplyArgs &a)
+{
+    if (a.vec32 && !a.precision) throw hip_error("complex64 vectors need the FP32 apply");
+}
+
+void launch_fwd_y(const ApplyArgs &a, hipStream_t s)
+{
+    check_vec32(a);
+    if (a.vec32) fwd_y<float, float>(a, s);
+    else if (a.precision) fwd_y<float, double>(a, s);
+    else fwd_y<double, double>(a, s);
+}
 void launch_fwd_x(const ApplyArgs &a, hipStream_t s) { a.precision ? fwd_x<float>(a, s) : fwd_x<double>(a, s); }
 void launch_inv_x(const ApplyArgs &a, hipStream_t s) { a.precision ? inv_x<float>(a, s) : inv_x<double>(a, s); }
-void launch_inv_y(const ApplyArgs &a, hipStream_t s) { a.precision ? inv_y<float>(a, s) : inv_y<double>(a, s); }
+void launch_inv_y(const ApplyArgs &a, hipStream_t s)
+{
+    check_vec32(a);
+    if (a.vec32) inv_y<float, float>(a, s);
+    else if (a.precision) inv_y<float, double>(a, s);
+    else inv_y<double, double>(a, s);
+}
 
 void launch_fft2_plain(double2 *data, int n1, int n2, int64_t ncols, const double2 *tw1, const double2 *tw2,
                        int sign, hipStream_t s)

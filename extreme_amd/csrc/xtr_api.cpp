@@ -29,6 +29,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 using cd = std::complex<double>;
@@ -300,15 +301,17 @@ struct xtr_op {
         return best;
     }
 
-    // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr).
-    void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s)
+    // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr);
+    // vec32: the vectors are complex64 (the FP32 apply of a mixed-precision correction solve).
+    void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s, bool vec32 = false)
     {
         require(green_ready, "the Green tensor is not set");
         require(type == XTR_OP_A || type == XTR_OP_CHI0, "the operator is not prepared");
         require(nrhs >= 1 && nrhs <= max_rhs, "nrhs out of range");
+        require(!vec32 || arith() == XTR_FP32, "complex64 vectors need the FP32 apply");
         // the last pass reads x while it writes y, element by element, and the tile measurement
         // repeats that pass: y must not overlap any x
-        const size_t vb = (size_t)g.n() * sizeof(double2);
+        const size_t vb = (size_t)g.n() * (vec32 ? sizeof(float2) : sizeof(double2));
         for (int r = 0; r < nrhs; ++r)
             for (int q = 0; q < nrhs; ++q) {
                 const char *xb = (const char *)v.x[q], *yb = (const char *)v.y[r];
@@ -330,6 +333,7 @@ struct xtr_op {
         a.buf = buf.p;
         a.tw_x = p == XTR_FP64 ? (const void *)tw64_x.p : (const void *)tw32_x.p;
         a.tw_y = p == XTR_FP64 ? (const void *)tw64_y.p : (const void *)tw32_y.p;
+        a.vec32 = vec32 ? 1 : 0;
         a.v = v;
         last_tiles = fft_tiles(a, s);
         for (int k = 0; k < 4; ++k) a.fft_tile[k] = last_tiles[k];
@@ -389,7 +393,9 @@ cd zeta_of(double sigma, double omega) { return cd(sigma, -omega * kEps0); }   /
 // FGMRES / GMRES on device vectors
 // ---------------------------------------------------------------------------------------------
 
-using ApplyFn = std::function<void(const double2 *x, double2 *y)>;
+// One operator apply of a solve's vector x into y: complex128 vectors, or complex64 ones in the
+// correction solves of the mixed-precision refinement.
+using ApplyFn = std::function<void(const void *x, void *y)>;
 
 struct SolveCtx {
     hipStream_t stream;
@@ -438,11 +444,13 @@ struct SolveCtx {
 
 // The vectors of one Gmres instance: basis V (m columns), flexible basis Z (m columns, FGMRES
 // only), the work vector w and the residual r0.
+template <typename VT>
 struct GmresBufs {
-    DevBuf<double2> V, Z, w, r0;
+    DevBuf<VT> V, Z, w, r0;
 };
 
-inline void grow(DevBuf<double2> &b, size_t need)
+template <typename VT>
+inline void grow(DevBuf<VT> &b, size_t need)
 {
     if (b.count < need) b.alloc(need);
 }
@@ -451,8 +459,12 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ready = nullptr;   // the solve's vector is ready for the shared apply (ApplyGate)
     SolveCtx ctx;
-    GmresBufs outer, inner;
-    DevBuf<double2> r, d;         // refinement residual and correction
+    GmresBufs<double2> outer, inner;
+    GmresBufs<float2> outer32, inner32;   // complex64 bases of the mixed-precision correction solves
+    DevBuf<double2> r, d;         // refinement residual and correction (complex128)
+    DevBuf<float2> r32, d32;      // the same in complex64, for correction solves on complex64 vectors
+    template <typename VT> GmresBufs<VT> &outer_bufs();
+    template <typename VT> GmresBufs<VT> &inner_bufs();
     Slot()
     {
         XTR_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
@@ -475,6 +487,11 @@ struct Slot {
 };
 
 }  // namespace
+
+template <> GmresBufs<double2> &Slot::outer_bufs<double2>() { return outer; }
+template <> GmresBufs<double2> &Slot::inner_bufs<double2>() { return inner; }
+template <> GmresBufs<float2> &Slot::outer_bufs<float2>() { return outer32; }
+template <> GmresBufs<float2> &Slot::inner_bufs<float2>() { return inner32; }
 
 Slot &xtr_op::slot(int r)
 {
@@ -504,8 +521,16 @@ struct xtr_net {
     DevBuf<double> partial_r, nn, lpart;
     DevBuf<float> feat;                               // [cells][2] feature channels, when set
     bool has_feat = false;
+    float phi = 0.0f;                                 // the frequency feature (log10 f), 0 without features
     int step = 0;
     std::mutex mu;
+    // spectral branch (K = 0: none): FP32 master weights [2][NM][Q][Q][2], Adam moments, gradient,
+    // BF16 copies, twiddle tables and the transform buffers
+    xtr::SpecGeom sg{};
+    DevBuf<float> sw, sm, sv, sgrad;
+    DevBuf<unsigned short> swb;
+    DevBuf<float2> twx, twy, sT, sU, sY, sgY;
+    xtr::SpecBuffers sbufs() { return xtr::SpecBuffers{twx.p, twy.p, sT.p, sU.p, sY.p, sgY.p, swb.p}; }
 
     ~xtr_net()
     {
@@ -529,6 +554,7 @@ void net_precondition(xtr_net *net, const double2 *v, double2 *z, hipStream_t s)
     const xtr::NetBuffers b = net->bufs();
     xtr::net_pack(net->g, v, nullptr, net->nn.p, b, s);
     xtr::net_forward(net->g, b, s);
+    if (net->sg.K > 0) xtr::spec_forward(net->sg, b, net->sbufs(), net->phi, s);
     xtr::net_unpack(net->g, v, net->nn.p, b, z, s);
 }
 
@@ -546,6 +572,7 @@ double net_gradients(xtr_net *net, int ns, const double2 *const *v, const double
         xtr::launch_reduce_r(net->partial_r.p, net->nn.p, s);
         xtr::net_pack(net->g, v[k], z[k], net->nn.p, b, s);
         xtr::net_forward(net->g, b, s);
+        if (net->sg.K > 0) xtr::spec_forward(net->sg, b, net->sbufs(), net->phi, s);
         xtr::net_loss(net->g, b, net->lpart.p, s);
         XTR_CHECK(hipMemcpyAsync(lp.data(), net->lpart.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         XTR_CHECK(hipStreamSynchronize(s));
@@ -556,12 +583,17 @@ double net_gradients(xtr_net *net, int ns, const double2 *const *v, const double
         }
         const float scale = st > 0 ? (float)(1.0 / (st * ns)) : 0.0f;
         xtr::net_backward(net->g, b, net->g1.p, net->g2.p, scale, k > 0, s);
+        if (net->sg.K > 0) xtr::spec_backward(net->sg, b, net->sbufs(), net->phi, net->sgrad.p, scale, k > 0, s);
         total += st > 0 ? se / st : 0.0;
     }
     XTR_CHECK(hipStreamSynchronize(s));
     return total / ns;
 }
 
+// GmresSolver (Gmres.cs) on vectors of element type VT: double2 (complex128, the reference's
+// arithmetic) or float2 (complex64, the correction solves of the mixed-precision refinement). Dots,
+// norms, the Hessenberg matrix and the Givens rotations are FP64 either way.
+template <typename VT>
 struct Gmres {
     int64_t n;
     int m;
@@ -570,15 +602,15 @@ struct Gmres {
     int max_repeat = 1;
     int backward_check = 0;
     SolveCtx *ctx;
-    Gmres *inner = nullptr;
+    Gmres<VT> *inner = nullptr;
     xtr_net *net = nullptr;       // learned right preconditioner in place of the inner GMRES
-    double2 *V = nullptr, *Z = nullptr, *w = nullptr, *r0 = nullptr;   // in the slot's GmresBufs
+    VT *V = nullptr, *Z = nullptr, *w = nullptr, *r0 = nullptr;   // in the slot's GmresBufs
     std::vector<cd> H;      // (m+1) x (m+1), column-major (FortranMatrix.cs:11-52)
     std::vector<double> gc;
     std::vector<cd> gs;
     int iter_counter = 0;
 
-    Gmres(int64_t n_, int m_, bool flex, SolveCtx *c, GmresBufs &b)
+    Gmres(int64_t n_, int m_, bool flex, SolveCtx *c, GmresBufs<VT> &b)
         : n(n_), m(m_), flexible(flex), ctx(c), H((size_t)(m_ + 1) * (m_ + 1)), gc(m_), gs(m_)
     {
         require(m_ >= 1 && m_ <= 31, "buffer length must be within 1..31");
@@ -593,15 +625,15 @@ struct Gmres {
     }
 
     cd &h(int i, int j) { return H[(size_t)j * (m + 1) + i]; }
-    double2 *col(double2 *B, int j) { return B + (int64_t)j * n; }
+    VT *col(VT *B, int j) { return B + (int64_t)j * n; }
 
-    void matvec(const double2 *x, double2 *y)
+    void matvec(const VT *x, VT *y)
     {
         ctx->apply(x, y);
         ctx->matvecs++;
     }
 
-    double norm(const double2 *v)
+    double norm(const VT *v)
     {
         xtr::launch_norm2(v, n, ctx->partial_r.p, ctx->stream);
         xtr::launch_reduce_r(ctx->partial_r.p, ctx->norm_dev.p, ctx->stream, ctx->nn_host.dev);
@@ -660,7 +692,10 @@ struct Gmres {
     {
         if (flexible) {
             if (net) {
-                net_precondition(net, col(V, jH), col(Z, jH), ctx->stream);
+                if constexpr (std::is_same<VT, double2>::value)
+                    net_precondition(net, col(V, jH), col(Z, jH), ctx->stream);
+                else
+                    throw std::invalid_argument("the learned preconditioner runs on complex128 vectors");
             } else {
                 double bea, be;
                 int conv;
@@ -668,9 +703,9 @@ struct Gmres {
             }
             if (ctx->harvest_count < ctx->harvest_cap) {
                 const int k = ctx->harvest_count++;
-                XTR_CHECK(hipMemcpyAsync(ctx->harvest_v[k], col(V, jH), n * sizeof(double2), hipMemcpyDeviceToDevice,
+                XTR_CHECK(hipMemcpyAsync(ctx->harvest_v[k], col(V, jH), n * sizeof(VT), hipMemcpyDeviceToDevice,
                                          ctx->stream));
-                XTR_CHECK(hipMemcpyAsync(ctx->harvest_z[k], col(Z, jH), n * sizeof(double2), hipMemcpyDeviceToDevice,
+                XTR_CHECK(hipMemcpyAsync(ctx->harvest_z[k], col(Z, jH), n * sizeof(VT), hipMemcpyDeviceToDevice,
                                          ctx->stream));
             }
             matvec(col(Z, jH), w);
@@ -680,7 +715,7 @@ struct Gmres {
     }
 
     // ConstructCurrentSolution (Gmres.cs:187-196): y = H^-1 g, x += B y
-    void construct(int jH, double2 *x)
+    void construct(int jH, VT *x)
     {
         std::vector<cd> y(jH);
         for (int i = 0; i < jH; ++i) y[i] = h(i, m);
@@ -695,7 +730,7 @@ struct Gmres {
     }
 
     // CheckBackwardError (Gmres.cs:200-230)
-    double check_backward_error(double bn, const double2 *rhs, const double2 *x)
+    double check_backward_error(double bn, const VT *rhs, const VT *x)
     {
         matvec(x, w);
         xtr::launch_sub(rhs, w, w, n, ctx->stream);
@@ -705,7 +740,7 @@ struct Gmres {
         return be;
     }
 
-    double final_actions(int jH, double bn, const double2 *rhs, double2 *x)
+    double final_actions(int jH, double bn, const VT *rhs, VT *x)
     {
         construct(jH, x);
         if (backward_check != 0) return check_backward_error(bn, rhs, x);
@@ -743,7 +778,7 @@ struct Gmres {
     }
 
     // PrepareRestart, ComputeTheTrue (Gmres.cs:145-162)
-    double restart(const double2 *rhs, double2 *x, int jH)
+    double restart(const VT *rhs, VT *x, int jH)
     {
         construct(jH, x);
         matvec(x, r0);
@@ -752,7 +787,7 @@ struct Gmres {
     }
 
     // GmresSolver.Solve (Gmres.cs:77-143), initial guess supplied
-    void solve(const double2 *rhs, const double2 *x0, double2 *result, double &bea_out, double &be_out, int &conv)
+    void solve(const VT *rhs, const VT *x0, VT *result, double &bea_out, double &be_out, int &conv)
     {
         iter_counter = 0;
         const double bn = norm(rhs);
@@ -767,14 +802,14 @@ struct Gmres {
         double beta = norm(w);
         if (!std::isfinite(beta)) throw std::runtime_error("the initial residual holds NaN or Inf (operator or initial guess)");
         if (beta == 0) {   // CheckForExactSolution: ResultInfo.Converged(0, 0, 1) (Gmres.cs:104-107)
-            if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
+            if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(VT), hipMemcpyDeviceToDevice, ctx->stream));
             bea_out = 0; be_out = 0; conv = 1;
             iter_counter = 1;
             return;
         }
         double bea = -1, be = -1;
         int iter = 0;
-        if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
+        if (result != x0) XTR_CHECK(hipMemcpyAsync(result, x0, n * sizeof(VT), hipMemcpyDeviceToDevice, ctx->stream));
         for (;;) {
             xtr::launch_mul(w, 1 / beta, col(V, 0), n, ctx->stream);
             h(0, m) = beta;
@@ -796,22 +831,25 @@ struct Gmres {
 };
 
 // FlexibleGmresWithGmresPreconditioner (:29-58) on one right-hand side.
-void run_fgmres(int64_t n, const xtr_gmres_params &p, Slot &sl, const double2 *b, double2 *x,
-                xtr_result_info &info, xtr_net *net = nullptr)
+template <typename VT>
+void run_fgmres(int64_t n, const xtr_gmres_params &p, Slot &sl, const VT *b, VT *x, xtr_result_info &info,
+                xtr_net *net = nullptr)
 {
     SolveCtx &ctx = sl.ctx;
+    if (!std::is_same<VT, double2>::value)
+        require(ctx.harvest_cap == 0, "pairs are harvested from complex128 solves only");
     require(p.outer_buffer_length >= 1 && p.outer_buffer_length <= 31, "OuterBufferLength must be 1..31");
     require(p.inner_buffer_length >= 1 && p.inner_buffer_length <= 31, "InnerBufferLength must be 1..31");
     require(p.max_repeat_number >= 1, "MaxRepeatsNumber must be >= 1");
     auto t0 = std::chrono::steady_clock::now();
-    std::unique_ptr<Gmres> inner;
+    std::unique_ptr<Gmres<VT>> inner;
     if (!net) {
-        inner = std::make_unique<Gmres>(n, p.inner_buffer_length, false, &ctx, sl.inner);
+        inner = std::make_unique<Gmres<VT>>(n, p.inner_buffer_length, false, &ctx, sl.inner_bufs<VT>());
         inner->tol = p.tolerance;
         inner->max_repeat = 1;
         inner->backward_check = 0;
     }
-    Gmres outer(n, p.outer_buffer_length, true, &ctx, sl.outer);
+    Gmres<VT> outer(n, p.outer_buffer_length, true, &ctx, sl.outer_bufs<VT>());
     outer.tol = p.tolerance;
     outer.max_repeat = p.max_repeat_number;
     outer.backward_check = p.backward_error_checking;
@@ -836,17 +874,21 @@ void run_fgmres(int64_t n, const xtr_gmres_params &p, Slot &sl, const double2 *b
 // is streamed once per round for all of them.
 struct ApplyGate {
     xtr_op *op;
+    bool vec32;                   // the solves' vectors are complex64
     std::mutex mu;
     std::condition_variable cv;
     int active;
     int pending = 0;
     uint64_t round = 0;
-    struct Sub { const double2 *x; double2 *y; hipStream_t s; hipEvent_t ready; const double2 *R; };
+    struct Sub { const void *x; void *y; hipStream_t s; hipEvent_t ready; const double2 *R; };
     std::vector<Sub> subs;
     hipEvent_t done = nullptr;
     std::exception_ptr err;
 
-    ApplyGate(xtr_op *o, int n) : op(o), active(n) { XTR_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming)); }
+    ApplyGate(xtr_op *o, int n, bool v32) : op(o), vec32(v32), active(n)
+    {
+        XTR_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    }
     ~ApplyGate()
     {
         if (done) (void)hipEventDestroy(done);
@@ -858,11 +900,11 @@ struct ApplyGate {
             xtr::RhsPtrs v{};
             for (size_t i = 0; i < subs.size(); ++i) {
                 XTR_CHECK(hipStreamWaitEvent(op->stream, subs[i].ready, 0));
-                v.x[i] = subs[i].x;
-                v.y[i] = subs[i].y;
+                v.x[i] = (const double2 *)subs[i].x;
+                v.y[i] = (double2 *)subs[i].y;
                 v.R[i] = subs[i].R;
             }
-            op->apply_raw(v, (int)subs.size(), op->stream);
+            op->apply_raw(v, (int)subs.size(), op->stream, vec32);
             XTR_CHECK(hipEventRecord(done, op->stream));
             for (auto &s : subs) XTR_CHECK(hipStreamWaitEvent(s.s, done, 0));
         } catch (...) {
@@ -874,7 +916,7 @@ struct ApplyGate {
         cv.notify_all();
     }
 
-    void submit(const double2 *x, double2 *y, hipStream_t s, hipEvent_t ready, const double2 *R)
+    void submit(const void *x, void *y, hipStream_t s, hipEvent_t ready, const double2 *R)
     {
         XTR_CHECK(hipEventRecord(ready, s));
         std::unique_lock<std::mutex> lk(mu);
@@ -903,10 +945,11 @@ struct ApplyGate {
 
 // Side-by-side FGMRES solves of nrhs right-hand sides with one ApplyGate (shared applies). Right-hand
 // side r runs on its own host thread with the handle's slot r (stream, bases, scratch).
+template <typename VT = double2>
 static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, const void *const *b, void *const *x,
                         xtr_result_info *info, xtr_iteration_fn on_iteration, void *user, const int *model = nullptr)
 {
-    ApplyGate gate(op, nrhs);
+    ApplyGate gate(op, nrhs, std::is_same<VT, float2>::value);
     std::vector<Slot *> sl(nrhs, nullptr);
     std::vector<std::exception_ptr> errs(nrhs);
     std::mutex cb_mu;
@@ -931,7 +974,7 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
             hipEvent_t ev = sl[r]->ready;
             ApplyGate *gp = &gate;
             const double2 *Rp = model ? op->model_r(model[r]) : nullptr;   // null: the handle's R
-            c.apply = [gp, s, ev, Rp](const double2 *xx, double2 *yy) { gp->submit(xx, yy, s, ev, Rp); };
+            c.apply = [gp, s, ev, Rp](const void *xx, void *yy) { gp->submit(xx, yy, s, ev, Rp); };
         }
         // inputs written on the caller's default stream must be visible to the solve streams
         XTR_CHECK(hipDeviceSynchronize());
@@ -940,7 +983,7 @@ static void solve_batch(xtr_op *op, const xtr_gmres_params *params, int nrhs, co
             th.emplace_back([&, r] {
                 try {
                     std::memset(&info[r], 0, sizeof(info[r]));
-                    run_fgmres(op->g.n(), *params, *sl[r], (const double2 *)b[r], (double2 *)x[r], info[r]);
+                    run_fgmres<VT>(op->g.n(), *params, *sl[r], (const VT *)b[r], (VT *)x[r], info[r]);
                 } catch (...) {
                     errs[r] = std::current_exception();
                 }
@@ -1373,10 +1416,10 @@ int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const vo
         ctx.on_iter = on_iteration;
         ctx.user = user;
         xtr_op *o = op;
-        ctx.apply = [o](const double2 *xx, double2 *yy) {
+        ctx.apply = [o](const void *xx, void *yy) {
             xtr::RhsPtrs v{};
-            v.x[0] = xx;
-            v.y[0] = yy;
+            v.x[0] = (const double2 *)xx;
+            v.y[0] = (double2 *)yy;
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));
@@ -1438,13 +1481,23 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
         XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
         SolveCtx &ctx = op->slot(xtr::kMaxRhs).ctx;   // reduction scratch for the FP64 norms
         ctx.reset(op->stream);
+        const bool vec32 = params->vector_precision == XTR_FP32;
+        require(params->vector_precision == XTR_FP64 || vec32, "vector_precision must be XTR_FP64 or XTR_FP32");
         std::vector<DevBuf<double2> *> r(nrhs), d(nrhs);
+        std::vector<DevBuf<float2> *> r32(nrhs), d32(nrhs);
         for (int k = 0; k < nrhs; ++k) {
             Slot &sk = op->slot(k);
             grow(sk.r, (size_t)n);
-            grow(sk.d, (size_t)n);
             r[k] = &sk.r;
-            d[k] = &sk.d;
+            if (vec32) {
+                grow(sk.r32, (size_t)n);
+                grow(sk.d32, (size_t)n);
+                r32[k] = &sk.r32;
+                d32[k] = &sk.d32;
+            } else {
+                grow(sk.d, (size_t)n);
+                d[k] = &sk.d;
+            }
         }
         auto norm = [&](const double2 *v) {
             xtr::launch_norm2(v, n, ctx.partial_r.p, ctx.stream);
@@ -1484,30 +1537,45 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
             }
             live.swap(next);
             if (live.empty()) break;
-            // complex64 FGMRES corrections A d = r, side by side, from d = 0
+            // FGMRES corrections A d = r with the complex64 operator, side by side, from d = 0; on
+            // complex64 vectors the residual goes in scaled to unit norm, r / |r|, and the correction
+            // comes back scaled by |r|
             op->active = XTR_FP32;
             std::vector<const void *> rb;
             std::vector<void *> dx;
             xtr_gmres_params ip = params->inner;
             double tol_in = 1.0;
             for (int k : live) {
-                XTR_CHECK(hipMemsetAsync(d[k]->p, 0, n * sizeof(double2), op->stream));
-                rb.push_back(r[k]->p);
-                dx.push_back(d[k]->p);
+                if (vec32) {
+                    xtr::launch_mul(r[k]->p, 1.0 / rn[k], r32[k]->p, n, op->stream);
+                    XTR_CHECK(hipMemsetAsync(d32[k]->p, 0, n * sizeof(float2), op->stream));
+                    rb.push_back(r32[k]->p);
+                    dx.push_back(d32[k]->p);
+                } else {
+                    XTR_CHECK(hipMemsetAsync(d[k]->p, 0, n * sizeof(double2), op->stream));
+                    rb.push_back(r[k]->p);
+                    dx.push_back(d[k]->p);
+                }
                 tol_in = std::min(tol_in, std::max(params->inner_floor, 0.5 * params->tolerance * bn[k] / rn[k]));
             }
             ip.tolerance = tol_in;
             XTR_CHECK(hipStreamSynchronize(op->stream));
             std::vector<xtr_result_info> ii(live.size());
-            solve_batch(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
+            if (vec32) solve_batch<float2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
+            else solve_batch<double2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
             for (size_t i = 0; i < live.size(); ++i) {
                 const int k = live[i];
                 info[k].refinements++;
                 info[k].matvecs32 += ii[i].matvecs;
                 info[k].inner_iterations += ii[i].number_of_iterations;
-                xtr::Coef32 one{};
-                one.c[0] = make_double2(1.0, 0.0);
-                xtr::launch_update_solution(d[k]->p, 1, one, (double2 *)x[k], n, op->stream);   // x += d
+                xtr::Coef32 c{};
+                if (vec32) {
+                    c.c[0] = make_double2(rn[k], 0.0);
+                    xtr::launch_update_solution(d32[k]->p, 1, c, (double2 *)x[k], n, op->stream);   // x += |r| d
+                } else {
+                    c.c[0] = make_double2(1.0, 0.0);
+                    xtr::launch_update_solution(d[k]->p, 1, c, (double2 *)x[k], n, op->stream);     // x += d
+                }
             }
         }
         XTR_CHECK(hipStreamSynchronize(op->stream));
@@ -1717,11 +1785,13 @@ int xtr_net_set_features(xtr_net *net, const float *feat)
         std::lock_guard<std::mutex> lk(net->mu);
         if (!feat) {
             net->has_feat = false;
+            net->phi = 0.0f;
             return;
         }
         if (!net->feat.p) net->feat.alloc((size_t)net->g.cells * 2);
         XTR_CHECK(hipMemcpy(net->feat.p, feat, (size_t)net->g.cells * 2 * sizeof(float), hipMemcpyHostToDevice));
         net->has_feat = true;
+        net->phi = feat[1];   // the frequency feature, the same in every cell, modulates the spectral branch
     });
 }
 
@@ -1755,6 +1825,86 @@ int xtr_net_get_weights(xtr_net *net, float *w1, float *w2)
         XTR_CHECK(hipStreamSynchronize(net->stream));
         XTR_CHECK(hipMemcpy(w1, net->w1.p, xtr::kNetW1 * sizeof(float), hipMemcpyDeviceToHost));
         XTR_CHECK(hipMemcpy(w2, net->w2.p, xtr::kNetW2 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_net_set_spectral(xtr_net *net, int modes)
+{
+    return guarded([&] {
+        require(net, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        const xtr::NetGeom &g = net->g;
+        require(modes >= 0 && modes <= 12, "spectral modes must be 0..12");
+        require(modes == 0 || (g.nz % 8 == 0 && modes < g.nx && modes < g.ny),
+                "the spectral branch needs Nz divisible by 8 and modes below Nx and Ny");
+        net->sg = xtr::SpecGeom{g.nx, g.ny, g.nz, modes};
+        if (modes == 0) return;
+        const xtr::SpecGeom &sg = net->sg;
+        const int64_t nw = 2 * sg.weights();   // floats
+        for (auto *d : {&net->sw, &net->sm, &net->sv, &net->sgrad}) {
+            d->alloc(nw);
+            XTR_CHECK(hipMemset(d->p, 0, nw * sizeof(float)));   // zero: the branch starts as no change
+        }
+        net->swb.alloc(nw);
+        XTR_CHECK(hipMemset(net->swb.p, 0, nw * sizeof(unsigned short)));
+        auto table = [](int p) {   // e^{-2 pi i t / p}, t < p, from FP64
+            std::vector<float2> t(p);
+            for (int k = 0; k < p; ++k) {
+                const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)p;
+                t[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+            }
+            return t;
+        };
+        upload(net->twx, table(2 * g.nx));
+        upload(net->twy, table(2 * g.ny));
+        const int nb = 2 * modes + 1, Q = sg.q();
+        net->sT.alloc((size_t)g.nx * nb * Q);
+        net->sU.alloc((size_t)sg.nm() * Q);
+        net->sY.alloc((size_t)sg.nm() * Q);
+        net->sgY.alloc((size_t)sg.nm() * Q);
+    });
+}
+
+int xtr_net_spectral_size(xtr_net *net, int *modes, int64_t *floats)
+{
+    return guarded([&] {
+        require(net, "null argument");
+        if (modes) *modes = net->sg.K;
+        if (floats) *floats = net->sg.K > 0 ? 2 * net->sg.weights() : 0;
+    });
+}
+
+int xtr_net_set_spectral_weights(xtr_net *net, const float *w)
+{
+    return guarded([&] {
+        require(net && w, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        require(net->sg.K > 0, "the network has no spectral branch (xtr_net_set_spectral)");
+        XTR_CHECK(hipMemcpy(net->sw.p, w, 2 * net->sg.weights() * sizeof(float), hipMemcpyHostToDevice));
+        xtr::spec_weights(net->sg, net->sw.p, net->swb.p, net->stream);
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+    });
+}
+
+int xtr_net_get_spectral_weights(xtr_net *net, float *w)
+{
+    return guarded([&] {
+        require(net && w, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        require(net->sg.K > 0, "the network has no spectral branch (xtr_net_set_spectral)");
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+        XTR_CHECK(hipMemcpy(w, net->sw.p, 2 * net->sg.weights() * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int xtr_net_spectral_gradient(xtr_net *net, float *g)
+{
+    return guarded([&] {
+        require(net && g, "null argument");
+        std::lock_guard<std::mutex> lk(net->mu);
+        require(net->sg.K > 0, "the network has no spectral branch (xtr_net_set_spectral)");
+        XTR_CHECK(hipStreamSynchronize(net->stream));
+        XTR_CHECK(hipMemcpy(g, net->sgrad.p, 2 * net->sg.weights() * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1795,6 +1945,11 @@ int xtr_net_train_device(xtr_net *net, int nsamples, const void *const *v, const
         xtr::net_adam(net->w1.p, net->m1.p, net->s1.p, net->g1.p, xtr::kNetW1, 1.0f, lr, net->step, net->stream);
         xtr::net_adam(net->w2.p, net->m2.p, net->s2.p, net->g2.p, xtr::kNetW2, 1.0f, lr, net->step, net->stream);
         xtr::net_weights(net->w1.p, net->w2.p, net->bufs(), net->stream);
+        if (net->sg.K > 0) {
+            xtr::net_adam(net->sw.p, net->sm.p, net->sv.p, net->sgrad.p, (int)(2 * net->sg.weights()), 1.0f, lr,
+                          net->step, net->stream);
+            xtr::spec_weights(net->sg, net->sw.p, net->swb.p, net->stream);
+        }
         XTR_CHECK(hipStreamSynchronize(net->stream));
         if (loss) *loss = l;
     });
@@ -1814,10 +1969,10 @@ int xtr_fgmres_solve_learned_device(xtr_op *op, const xtr_gmres_params *params, 
         SolveCtx &ctx = sl.ctx;
         ctx.reset(op->stream);
         xtr_op *o = op;
-        ctx.apply = [o](const double2 *xx, double2 *yy) {
+        ctx.apply = [o](const void *xx, void *yy) {
             xtr::RhsPtrs v{};
-            v.x[0] = xx;
-            v.y[0] = yy;
+            v.x[0] = (const double2 *)xx;
+            v.y[0] = (double2 *)yy;
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));
@@ -1841,10 +1996,10 @@ int xtr_fgmres_solve_harvest_device(xtr_op *op, const xtr_gmres_params *params, 
         ctx.harvest_z = (double2 *const *)z_out;
         ctx.harvest_cap = capacity;
         xtr_op *o = op;
-        ctx.apply = [o](const double2 *xx, double2 *yy) {
+        ctx.apply = [o](const void *xx, void *yy) {
             xtr::RhsPtrs v{};
-            v.x[0] = xx;
-            v.y[0] = yy;
+            v.x[0] = (const double2 *)xx;
+            v.y[0] = (double2 *)yy;
             o->apply_raw(v, 1, o->stream);
         };
         std::memset(info, 0, sizeof(*info));

@@ -77,7 +77,8 @@ __host__ __device__ inline FoldImages fold_images(int64_t b, int nx, int ny)
 __host__ __device__ inline int asym_swizzle_mask(int nz) { return nz % 16 == 0 ? 15 : (nz % 8 == 0 ? 7 : 0); }
 __host__ __device__ inline int asym_pos(int rc, int tr, int nz, int mask) { return rc * nz + (tr ^ (rc & mask)); }
 
-// Per-call vector pointers (device, FP64) for up to kMaxRhs right-hand sides.
+// Per-call vector pointers (device; complex128, or complex64 when the apply's vec32 is set) for up
+// to kMaxRhs right-hand sides.
 // R[i] is the R function of right-hand side i's model ([nx*ny*nz]); models that share the grid,
 // background and frequency share the Green tensor, so one apply can serve several of them.
 struct RhsPtrs {
@@ -104,6 +105,7 @@ struct ApplyArgs {
     const void *tw_x;         // twiddles for P = 2 nx
     const void *tw_y;         // twiddles for P = 2 ny
     int fft_tile[4];          // column tile of fwd_y, fwd_x, inv_x, inv_y (FFT_TILE_*)
+    int vec32;                // v.x and v.y point to complex64 vectors (float2; precision FP32 only)
     RhsPtrs v;
 };
 
@@ -151,25 +153,37 @@ void launch_level_field(const double2 *G, int electric, const double2 *jq, const
                         double2 *small, const double2 *tw_x, const double2 *tw_y, double2 *field, hipStream_t s);
 
 // ---- Krylov (krylov_kernels.hip) -----------------------------------------------------------
+// Vectors are complex128 (double2) or complex64 (float2, the mixed-precision correction solves);
+// every kernel accumulates in FP64.
 constexpr int kRedBlocks = 512;   // partial-sum blocks of the reductions
 // partial[k][blk] of conj(V_k) . w, k < cnt (V columns contiguous, stride n)
 void launch_block_dots(const double2 *V, int cnt, const double2 *w, int64_t n, double2 *partial, hipStream_t s);
+void launch_block_dots(const float2 *V, int cnt, const float2 *w, int64_t n, double2 *partial, hipStream_t s);
 // out[k] = sum_blk partial[k][blk]
 void launch_reduce_c(const double2 *partial, int cnt, double2 *out, hipStream_t s, double2 *host_out = nullptr);
 // w -= sum_k V_k h_k (h on device), partial norms of the new w
 void launch_cgs_update(const double2 *V, int cnt, const double2 *h, double2 *w, int64_t n, double *partial,
                        hipStream_t s);
+void launch_cgs_update(const float2 *V, int cnt, const double2 *h, float2 *w, int64_t n, double *partial,
+                       hipStream_t s);
 // partial sums of |w|^2 ; out = sum of partials
 void launch_norm2(const double2 *w, int64_t n, double *partial, hipStream_t s);
+void launch_norm2(const float2 *w, int64_t n, double *partial, hipStream_t s);
 void launch_reduce_r(const double *partial, double *out, hipStream_t s, double *host_out = nullptr);
 // dst = src / d  (v_{j+1} = w / |w|, Gmres.cs:267-268)
 void launch_div(const double2 *src, double d, double2 *dst, int64_t n, hipStream_t s);
-// dst = a * src  (V_0 = (1/beta) w, Gmres.cs:117)
+void launch_div(const float2 *src, double d, float2 *dst, int64_t n, hipStream_t s);
+// dst = a * src  (V_0 = (1/beta) w, Gmres.cs:117; and the complex64 copy of a scaled residual)
 void launch_mul(const double2 *src, double a, double2 *dst, int64_t n, hipStream_t s);
+void launch_mul(const float2 *src, double a, float2 *dst, int64_t n, hipStream_t s);
+void launch_mul(const double2 *src, double a, float2 *dst, int64_t n, hipStream_t s);
 // dst = a - b
 void launch_sub(const double2 *a, const double2 *b, double2 *dst, int64_t n, hipStream_t s);
+void launch_sub(const float2 *a, const float2 *b, float2 *dst, int64_t n, hipStream_t s);
 // x += sum_k B_k y_k  (ConstructCurrentSolution, y passed by value)
 struct Coef32 { double2 c[32]; };
 void launch_update_solution(const double2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s);
+void launch_update_solution(const float2 *B, int cnt, const Coef32 &y, float2 *x, int64_t n, hipStream_t s);
+void launch_update_solution(const float2 *B, int cnt, const Coef32 &y, double2 *x, int64_t n, hipStream_t s);
 
 }  // namespace xtr

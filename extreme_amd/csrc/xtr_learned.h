@@ -47,4 +47,32 @@ void net_adam(float *w, float *m, float *v2, const float *grad, int n, float gsc
               hipStream_t s);
 void net_weights(const float *w1, const float *w2, const NetBuffers &b, hipStream_t s);
 
+// ---- spectral branch: per-lateral-mode channel mixing on the zero-padded lateral spectrum ----------
+// Modes kx, ky in -K..K of the 2Nx x 2Ny padded plane, mode m = (kx + K)(2K + 1) + (ky + K); channels
+// q = c Nz + k (component c, layer k), Q = 3 Nz. U[m][q] = s0 sum_{i<Nx, j<Ny} u(i, j, q)
+// e^{-2 pi i (kx i / 2Nx + ky j / 2Ny)} with s0 = 1 / sqrt(Nx Ny), Y[m] = (W0[m] + phi W1[m]) bf16(U[m]),
+// y(i, j, q) += s0 sum_m Y[m][q] e^{+2 pi i (...)}. W0, W1: complex Q x Q per mode (row = output).
+struct SpecGeom {
+    int nx, ny, nz, K;
+    __host__ __device__ int nm() const { return (2 * K + 1) * (2 * K + 1); }
+    __host__ __device__ int q() const { return 3 * nz; }
+    __host__ __device__ int64_t weights() const { return 2LL * nm() * q() * q(); }   // complex entries of W0, W1
+};
+struct SpecBuffers {
+    const float2 *twx, *twy;   // e^{-2 pi i t / 2Nx}, t < 2Nx; e^{-2 pi i t / 2Ny}, t < 2Ny (FP32)
+    float2 *T;                 // [Nx][2K+1][Q] partial transform
+    float2 *U;                 // [NM][Q] transform of the input, rounded to BF16 values (the MFMA operand)
+    float2 *Y;                 // [NM][Q] mixed modes
+    float2 *gY;                // [NM][Q] transform of the loss gradient
+    unsigned short *WB;        // [2][NM][Q][Q][2] BF16 copies of W0, W1 (re, im)
+};
+// the network's input channels 0..5 (BF16 [cells][8]) -> U; mixing -> Y; y[cells][16] channels 0..5 += inverse
+void spec_forward(const SpecGeom &g, const NetBuffers &b, const SpecBuffers &sb, float phi, hipStream_t s);
+// dY[m] = transform of dy (FP32 [cells][8]); grad (+)= scale dY conj(U)^T into [2][NM][Q][Q] complex
+// (W1's part times phi)
+void spec_backward(const SpecGeom &g, const NetBuffers &b, const SpecBuffers &sb, float phi, float *grad, float scale,
+                   int accumulate, hipStream_t s);
+// BF16 copies of the FP32 master weights w [2][NM][Q][Q][2]
+void spec_weights(const SpecGeom &g, const float *w, unsigned short *WB, hipStream_t s);
+
 }  // namespace xtr

@@ -236,12 +236,14 @@ class FrequencySolve:
         return {"E": E, "H": H, "Z": z, "rho": rho, "phase": phase, "E_anomalous": Ea, "H_anomalous": Ha,
                 "E_normal": [normals[p][0] for p in (0, 1)], "H_normal": [normals[p][1] for p in (0, 1)]}
 
-    def solve_mixed(self, inner_floor=1e-8, max_refinements=10):
+    def solve_mixed(self, inner_floor=1e-6, max_refinements=10, vectors="fp32"):
         """The same two solves in mixed precision (needs precision="mixed"): chi <- chi0, then FP64
-        refinement around complex64 FGMRES until the FP64 true residual reaches the tolerance."""
+        refinement around complex64 FGMRES (complex64 Krylov vectors unless vectors="fp64") until the
+        FP64 true residual reaches the tolerance."""
         self.chi.copy_(self.chi0)
         return self.solver.solve_refined_batch([self.chi0[0], self.chi0[1]], [self.chi[0], self.chi[1]],
-                                               inner_floor=inner_floor, max_refinements=max_refinements)
+                                               inner_floor=inner_floor, max_refinements=max_refinements,
+                                               vectors=vectors)
 
 
 class ModelGroupSolve:

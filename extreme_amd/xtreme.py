@@ -55,7 +55,7 @@ class _Info(C.Structure):
 
 class _RefineParams(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("inner_floor", C.c_double), ("max_refinements", C.c_int),
-                ("inner", _Params)]
+                ("inner", _Params), ("vector_precision", C.c_int)]
 
 
 class _RefineInfo(C.Structure):
@@ -123,6 +123,11 @@ def lib():
         L.xtr_net_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.xtr_net_gradients_device.argtypes = [C.c_void_p, C.c_int, vpp, vpp, f32p, f32p, C.POINTER(C.c_double)]
         L.xtr_net_train_device.argtypes = [C.c_void_p, C.c_int, vpp, vpp, C.c_float, C.POINTER(C.c_double)]
+        L.xtr_net_set_spectral.argtypes = [C.c_void_p, C.c_int]
+        L.xtr_net_spectral_size.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.xtr_net_set_spectral_weights.argtypes = [C.c_void_p, f32p]
+        L.xtr_net_get_spectral_weights.argtypes = [C.c_void_p, f32p]
+        L.xtr_net_spectral_gradient.argtypes = [C.c_void_p, f32p]
         L.xtr_fgmres_solve_learned_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p,
                                                       C.c_void_p, C.POINTER(_Info)]
         L.xtr_fgmres_solve_harvest_device.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p,
@@ -525,9 +530,12 @@ class AnomalyCurrentFgmresSolver:
                                                     None))
         return [ResultInfo._from(infos[i]) for i in range(n)]
 
-    def solve_refined_batch(self, bs, xs, tolerance=None, inner_floor=1e-8, max_refinements=10) -> list:
+    def solve_refined_batch(self, bs, xs, tolerance=None, inner_floor=1e-6, max_refinements=10,
+                            vectors: str = "fp32") -> list:
         """Mixed precision (operator created with precision="mixed"): FP64 iterative refinement
-        around complex64 FGMRES corrections, stopping on the FP64 true residual. Device tensors."""
+        around complex64 FGMRES corrections, stopping on the FP64 true residual. The corrections'
+        Krylov vectors are complex64 (vectors="fp32") or complex128 ("fp64"); their dots, norms and
+        Hessenberg matrix are FP64 either way. Device tensors."""
         n = len(bs)
         if len(xs) != n:
             raise ValueError("bs and xs must have the same length")
@@ -536,7 +544,7 @@ class AnomalyCurrentFgmresSolver:
         bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
         tol = float(self.params.tolerance if tolerance is None else tolerance)
-        rp = _RefineParams(tol, float(inner_floor), int(max_refinements), self.params)
+        rp = _RefineParams(tol, float(inner_floor), int(max_refinements), self.params, _PRECISION[vectors])
         _check(lib().xtr_refined_solve_batch_device(self.op._h, C.byref(rp), n, bp, xp, infos))
         return [RefineInfo._from(infos[i]) for i in range(n)]
 
@@ -555,14 +563,58 @@ class LearnedPreconditioner:
     features (log10 sigma, log10 f) once set_features is called, else zero.
     linear=True drops the hidden layer's relu: the preconditioner is then a fixed linear map."""
 
-    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1, linear: bool = False):
+    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1, linear: bool = False, modes: int = 0):
         h = C.c_void_p()
         _check(lib().xtr_net_create(nx, ny, nz, C.c_uint64(seed), C.byref(h)))
         self._h = h
         self.shape = (nx, ny, nz)
         self.linear = bool(linear)
+        self.modes = 0
         if linear:
             _check(lib().xtr_net_set_linear(self._h, 1))
+        if modes:
+            self.set_spectral(modes)
+
+    # --- spectral branch: per-lateral-mode mixing of the 3 nz complex channels, modes |kx|, |ky| <= K
+    def set_spectral(self, modes: int):
+        """Adds (modes > 0, zero weights) or removes (0) the spectral branch."""
+        _check(lib().xtr_net_set_spectral(self._h, int(modes)))
+        self.modes = int(modes)
+
+    def _spec_shape(self):
+        nm = (2 * self.modes + 1) ** 2
+        q = 3 * self.shape[2]
+        return (2, nm, q, q)
+
+    def parameter_count(self) -> int:
+        """Trainable FP32 parameters: the two convolutions and the spectral branch."""
+        k, f = C.c_int(), C.c_int64()
+        _check(lib().xtr_net_spectral_size(self._h, C.byref(k), C.byref(f)))
+        return 27 * 8 * 16 + 27 * 16 * 8 + f.value
+
+    def spectral_weights(self):
+        """(W0, W1), complex64 [NM, 3nz, 3nz] each."""
+        w = np.zeros(self._spec_shape() + (2,), dtype=np.float32)
+        _check(lib().xtr_net_get_spectral_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float))))
+        c = w[..., 0] + 1j * w[..., 1]
+        return c[0].astype(np.complex64), c[1].astype(np.complex64)
+
+    def set_spectral_weights(self, w0, w1):
+        shp = self._spec_shape()[1:]
+        w = np.zeros((2,) + shp + (2,), dtype=np.float32)
+        for k, a in enumerate((w0, w1)):
+            a = np.asarray(a).reshape(shp)
+            w[k, ..., 0] = a.real
+            w[k, ..., 1] = a.imag
+        _check(lib().xtr_net_set_spectral_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def spectral_gradient(self):
+        """Gradient of the last gradients()/train_step() call: (dL/dW0, dL/dW1) as complex [NM, 3nz, 3nz]
+        (real part: derivative by the real part of the weight, imaginary part: by its imaginary part)."""
+        g = np.zeros(self._spec_shape() + (2,), dtype=np.float32)
+        _check(lib().xtr_net_spectral_gradient(self._h, g.ctypes.data_as(C.POINTER(C.c_float))))
+        c = g[..., 0] + 1j * g[..., 1]
+        return c[0], c[1]
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -174,6 +174,9 @@ typedef struct {
     int max_refinements;          /* correction solves per right-hand side (>= 1) */
     xtr_gmres_params inner;       /* FGMRES(m) / GMRES(m_in) settings of the correction solves;
                                      inner.tolerance is ignored (set per refinement) */
+    int vector_precision;         /* Krylov vectors of the correction solves: XTR_FP64 (complex128)
+                                     or XTR_FP32 (complex64, the apply reading and writing complex64;
+                                     dots, norms, Hessenberg and rotations stay FP64) */
 } xtr_refine_params;
 
 typedef struct {
@@ -310,6 +313,23 @@ int xtr_net_set_features(xtr_net *net, const float *feat);
  * of component c (x, y, z). */
 int xtr_net_set_weights(xtr_net *net, const float *w1, const float *w2);
 int xtr_net_get_weights(xtr_net *net, float *w1, float *w2);
+/* Spectral branch: a per-lateral-mode channel mixing on the zero-padded 2 nx x 2 ny spectrum added to
+ * the network's output, for the non-local reach a 3 x 3 x 3 convolution lacks. For the modes kx, ky in
+ * -K..K and the 3 nz complex channels q = c nz + k of the scaled residual u = v / s:
+ *   U[m][q] = s0 sum_{i<nx, j<ny} u(i, j, q) e^{-2 pi i (kx i / 2nx + ky j / 2ny)}, s0 = 1 / sqrt(nx ny),
+ *   Y[m] = (W0[m] + phi W1[m]) U[m],  N(u)(i, j, q) += s0 sum_m Y[m][q] e^{+2 pi i (kx i / 2nx + ky j / 2ny)},
+ * with phi the frequency feature (log10 f, channel 7 of xtr_net_set_features; 0 without features),
+ * mode m = (kx + K)(2K + 1) + (ky + K), W0, W1 complex 3nz x 3nz per mode (row = output channel). The
+ * mixing runs on BF16 MFMA with FP32 accumulation (BF16 weights and BF16-rounded U), the transforms in
+ * FP32. modes = K in 1..12 (0 removes the branch) needs nz divisible by 8 and K < nx, ny; the
+ * weights start at zero. Weight layout: [2 (W0, W1)][NM][3nz][3nz][2 (re, im)] FP32, NM = (2K + 1)^2;
+ * *floats = 2 * 2 NM (3nz)^2. The gradient is that of the last xtr_net_gradients_device or
+ * xtr_net_train_device call (before its Adam step), for W1 already multiplied by phi. */
+int xtr_net_set_spectral(xtr_net *net, int modes);
+int xtr_net_spectral_size(xtr_net *net, int *modes, int64_t *floats);
+int xtr_net_set_spectral_weights(xtr_net *net, const float *w);
+int xtr_net_get_spectral_weights(xtr_net *net, float *w);
+int xtr_net_spectral_gradient(xtr_net *net, float *g);
 /* z = the preconditioner applied to v (device vectors of n = 3 nx ny nz complex values). */
 int xtr_net_apply_device(xtr_net *net, const void *v, void *z);
 /* Gradient of the mean over the samples of |N(v/s) - (z - v)/s|^2 / |(z - v)/s|^2 (relative

@@ -122,6 +122,144 @@ def test_forward_and_gradients_match_the_cpu_restatement(torch_cuda, linear, fea
     assert e1 < 1e-4 and e2 < 1e-4
 
 
+def _twiddles(p):
+    """e^{-2 pi i t / p}, t < p, as the library tabulates them: FP64 values rounded to complex64."""
+    return np.exp(-2j * np.pi * np.arange(p) / p).astype(np.complex64).astype(np.complex128)
+
+
+def _spectral_matrices(shape, K):
+    """Fx[a, i] = e^{-2 pi i kx i / 2nx}, Fy[b, j] likewise, kx = a - K, ky = b - K: the truncated
+    transforms of the zero-padded plane as matrices of the tabulated twiddles."""
+    nx, ny, _ = shape
+    tx, ty = _twiddles(2 * nx), _twiddles(2 * ny)
+    ks = np.arange(-K, K + 1)
+    Fx = tx[np.mod(np.outer(ks, np.arange(nx)), 2 * nx)]
+    Fy = ty[np.mod(np.outer(ks, np.arange(ny)), 2 * ny)]
+    return Fx, Fy
+
+
+def _spectral_reference(shape, K, w0, w1, phi, x0, dy=None):
+    """The spectral branch as the GPU computes it: U = s0 Fx u Fy^T (FP32, then BF16), Y = A U with
+    A = BF16(BF16(W0) + phi BF16(W1)) per real part, y = s0 Fx^H Y conj(Fy); with dy, the weight
+    gradient's factors gY = s0 Fx dy_c Fy^T (per channel) and U. u and dy are [nx, ny, nz, 8] with
+    channels (Re, Im) x (x, y, z); U, Y, gY are [NM, 3nz]."""
+    nx, ny, nz = shape
+    Fx, Fy = _spectral_matrices(shape, K)
+    s0 = 1.0 / np.sqrt(nx * ny)
+
+    def cplx(a):          # [nx, ny, nz, 8] -> [nx, ny, 3nz] complex, q = c nz + k
+        return np.concatenate([a[..., 2 * c] + 1j * a[..., 2 * c + 1] for c in range(3)], axis=-1)
+
+    def fwd(u):
+        U = s0 * np.einsum("ai,ijq,bj->abq", Fx, u, Fy)
+        return U.reshape(-1, 3 * nz)
+
+    def b16c(z):
+        z = np.asarray(z, dtype=np.complex64)
+        return bf16(z.real) + 1j * bf16(z.imag)
+
+    U = b16c(fwd(cplx(x0)))
+    A = (bf16(np.float32(bf16(w0.real) + phi * bf16(w1.real))) +
+         1j * bf16(np.float32(bf16(w0.imag) + phi * bf16(w1.imag))))
+    Y = np.einsum("mab,mb->ma", A, U)
+    nb = 2 * K + 1
+    Yg = Y.reshape(nb, nb, 3 * nz)
+    y = s0 * np.einsum("ai,abq,bj->ijq", np.conj(Fx), Yg, np.conj(Fy))
+    out = np.zeros((nx, ny, nz, 8))
+    for c in range(3):
+        out[..., 2 * c] = y[..., c * nz:(c + 1) * nz].real
+        out[..., 2 * c + 1] = y[..., c * nz:(c + 1) * nz].imag
+    gY = None if dy is None else fwd(cplx(dy))
+    return out, U, gY
+
+
+def test_spectral_branch_matches_the_cpu_restatement(torch_cuda):
+    """The network with the spectral branch (relu hidden layer, features set): forward output, loss
+    and the gradients of both convolutions and of W0, W1 against the CPU restatement."""
+    torch = torch_cuda
+    shape, K = (6, 5, 8), 2
+    nx, ny, nz = shape
+    n = 3 * nx * ny * nz
+    rng = np.random.default_rng(17)
+    w1 = (rng.standard_normal((27, 8, 16)) * 0.2).astype(np.float32)
+    w2 = (rng.standard_normal((27, 16, 8)) * 0.2).astype(np.float32)
+    nm, q = (2 * K + 1) ** 2, 3 * nz
+    sw0 = ((rng.standard_normal((nm, q, q)) + 1j * rng.standard_normal((nm, q, q))) * 0.1).astype(np.complex64)
+    sw1 = ((rng.standard_normal((nm, q, q)) + 1j * rng.standard_normal((nm, q, q))) * 0.05).astype(np.complex64)
+    sigma = 10.0 ** rng.uniform(-3, 0, shape)
+    freq = 0.3
+    feat = np.stack([np.log10(sigma), np.full(shape, np.log10(freq))], axis=-1).astype(np.float32)
+    phi = float(np.float32(np.log10(freq)))
+    vs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)]
+    zs = [v + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for v in vs]
+    net = X.LearnedPreconditioner(*shape, seed=3, modes=K)
+    net.set_weights(w1, w2)
+    net.set_spectral_weights(sw0, sw1)
+    net.set_features(sigma, freq)
+    g0, g1_ = net.spectral_weights()
+    assert np.array_equal(g0, sw0) and np.array_equal(g1_, sw1)
+    print(f"parameters: {net.parameter_count()}", flush=True)
+    assert net.parameter_count() == 27 * 8 * 16 + 27 * 16 * 8 + 4 * nm * q * q
+
+    # CPU restatement: the convolution network (as _reference) plus the spectral branch
+    W1, W2 = bf16(w1.astype(np.float64)), bf16(w2.astype(np.float64))
+    f64 = feat.astype(np.float64)
+    outs, loss = [], 0.0
+    gw1, gw2 = np.zeros((27, 8, 16)), np.zeros((27, 16, 8))
+    gs0 = np.zeros((nm, q, q), dtype=np.complex128)
+    for v, z in zip(vs, zs):
+        s = np.sqrt(np.vdot(v, v).real / n)
+        x0 = bf16(np.float32(_channels(v, shape, s, f64)))
+        pre = _conv(x0, W1)
+        h = bf16(np.maximum(pre, 0))
+        ys, U, _ = _spectral_reference(shape, K, sw0.astype(np.complex128), sw1.astype(np.complex128), phi, x0)
+        y = np.float32(np.float32(_conv(h, W2)) + ys)
+        out = v.reshape(nx, ny, 3, nz).copy()
+        for c in range(3):
+            out[:, :, c, :] += s * (y[..., 2 * c] + 1j * y[..., 2 * c + 1])
+        outs.append(out.reshape(-1))
+        t = np.float32(_channels(z - v, shape, s))
+        d = np.float32(y - t)
+        d[..., 6:] = 0
+        st = float((t.astype(np.float64) ** 2).sum())
+        loss += float((d.astype(np.float64) ** 2).sum()) / st
+        dy = np.float32(2.0 * d).astype(np.float64)
+        scale = 1.0 / (st * len(vs))
+        gw2 += scale * np.stack([_shift(h, t_).reshape(-1, 16).T @ dy.reshape(-1, 8) for t_ in range(27)])
+        dh = sum(_shift(bf16(np.float32(dy)), t_, -1) @ W2[t_].T for t_ in range(27))
+        dp = np.where(h > 0, dh, 0.0)
+        gw1 += scale * np.stack([_shift(x0, t_).reshape(-1, 8).T @ dp.reshape(-1, 16) for t_ in range(27)])
+        _, _, gY = _spectral_reference(shape, K, sw0.astype(np.complex128), sw1.astype(np.complex128), phi, x0, dy)
+        gs0 += scale * np.einsum("ma,mb->mab", gY, np.conj(U))
+    loss /= len(vs)
+
+    dv = [torch.from_numpy(v).to("cuda") for v in vs]
+    dz = [torch.from_numpy(z).to("cuda") for z in zs]
+    for v, ref in zip(dv, outs):
+        o = net.apply(v).cpu().numpy()
+        e = np.linalg.norm(o - ref) / np.linalg.norm(ref - v.cpu().numpy())
+        print(f"forward with the spectral branch: correction error {e:.2e}", flush=True)
+        assert e < 1e-5
+    G1, G2, L = net.gradients(dv, dz)
+    S0, S1 = net.spectral_gradient()
+    e1 = np.linalg.norm(G1 - gw1) / np.linalg.norm(gw1)
+    e2 = np.linalg.norm(G2 - gw2) / np.linalg.norm(gw2)
+    es0 = np.linalg.norm(S0 - gs0) / np.linalg.norm(gs0)
+    es1 = np.linalg.norm(S1 - phi * gs0) / np.linalg.norm(phi * gs0)
+    print(f"loss {L:.6f} vs {loss:.6f}; gradient errors w1 {e1:.2e} w2 {e2:.2e} W0 {es0:.2e} W1 {es1:.2e}", flush=True)
+    assert abs(L - loss) <= 1e-5 * loss
+    assert e1 < 1e-4 and e2 < 1e-4 and es0 < 1e-4 and es1 < 1e-4
+    # a wrong sign convention of the inverse transform must fail this check: conjugating the
+    # restatement's Y path changes the output far beyond the tolerance
+    ys_bad = _spectral_reference(shape, K, np.conj(sw0.astype(np.complex128)), np.conj(sw1.astype(np.complex128)),
+                                 phi, bf16(np.float32(_channels(vs[0], shape, np.sqrt(np.vdot(vs[0], vs[0]).real / n),
+                                                                f64))))[0]
+    ys_ok = _spectral_reference(shape, K, sw0.astype(np.complex128), sw1.astype(np.complex128), phi,
+                                bf16(np.float32(_channels(vs[0], shape, np.sqrt(np.vdot(vs[0], vs[0]).real / n),
+                                                          f64))))[0]
+    assert np.linalg.norm(ys_bad - ys_ok) > 0.1 * np.linalg.norm(ys_ok)
+
+
 def _problem(model, pol):
     import oracle.oracle as O
     w = cm.omega_of(model.frequencies[0])
@@ -237,3 +375,4 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda, monkeypatch):
         print(f"{name} vs none: {gain:.2f}x fewer operator applies, {results['identity'][1] / results[name][1]:.2f}x "
               f"the speed; vs inner GMRES(10): {results['inner GMRES(10)'][1] / results[name][1]:.2f}x the speed",
               flush=True)
+

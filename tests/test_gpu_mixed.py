@@ -12,15 +12,40 @@ from tests.conftest import rel
 pytestmark = pytest.mark.gpu
 
 
-def test_mixed_c1_reaches_fp64_true_residual(oracle, torch_cuda):
+def _oracle_responses(oracle, m, w, chis, level_z=0.0):
+    """E, H, Z, rho_a and phase at the level from the oracle's AtoO path (AtoOCalculator.cs:63-160,
+    Mt3DForwardSolver.cs:236-271, ResponseFunctionsCalculator.cs:36-51) for the two polarisations'
+    solutions chis."""
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    Ge = oracle.green_atoo_level(m, w, level_z, True)
+    Gh = oracle.green_atoo_level(m, w, level_z, False)
+    e0 = oracle.plane_wave_e(m.section, w, level_z)
+    h0 = oracle.plane_wave_h(m.section, w, level_z)
+    en = oracle.normal_field(m, w)
+    E, H = [], []
+    for pol in (0, 1):
+        js = oracle.jscattered(m, pol, z, z0, en)
+        _, jq = oracle.escattered_jq(m, z, z0, chis[pol], js)
+        ea = oracle.atoo_field(m, Ge, True, jq)
+        ha = oracle.atoo_field(m, Gh, False, jq)
+        E.append(ea + np.array([e0 if k == pol else 0 for k in range(3)]))
+        H.append(ha + (np.array([0, h0, 0]) if pol == 0 else np.array([-h0, 0, 0])))
+    Z = F.impedance(E[0], E[1], H[0], H[1])
+    rho, phase = F.apparent_resistivity_phase(Z, w)
+    return {"E": E, "H": H, "Z": Z, "rho": rho, "phase": phase}
+
+
+@pytest.mark.parametrize("vectors", ["fp32", "fp64"])
+def test_mixed_c1_reaches_fp64_true_residual(oracle, torch_cuda, vectors):
     m = cm.config_c1()
     p = F.FrequencySolve(m, m.frequencies[0], precision="mixed")
-    infos = p.solve_mixed()
+    infos = p.solve_mixed(vectors=vectors)
     chi_mixed = p.chi.clone()
     w = p.omega
     G = p.op.green_tensor()
     oop = oracle.Operator(m, G)
     z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    refs = []
     for pol in (0, 1):
         chi0 = p.chi0[pol].cpu().numpy()
         oop.prepare("A", z, z0)
@@ -36,6 +61,14 @@ def test_mixed_c1_reaches_fp64_true_residual(oracle, torch_cuda):
         assert true_res <= 1.05e-8
         assert abs(true_res - infos[pol].residual) <= 1e-3 * true_res + 1e-12
         assert err < 1e-6
+        refs.append(x_ref)
+    # every observed result of the mixed solve against the oracle's FP64 solve, per quantity
+    p.build_level(0.0)
+    got = p.level_responses()
+    errs = F.result_errors(got, _oracle_responses(oracle, m, w, refs))
+    print(f"{vectors} vectors: result errors vs the FP64 oracle {errs}", flush=True)
+    for q, e in errs.items():
+        assert e < 1e-6, (q, e)
 
 
 def test_mixed_operator_fp64_applies_unchanged(oracle, torch_cuda):
