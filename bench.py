@@ -80,6 +80,14 @@ def main():
     ap.add_argument("--no-models", action="store_true",
                     help="skip the two-model grouping measurement on the 128x128x32 model")
     ap.add_argument("--cpu-applies", type=int, default=0, help="oracle applies timed (0: 2 at C5, 3 otherwise)")
+    ap.add_argument("--c3-models", type=int, default=64,
+                    help="models of the batched uneven-grid workload (256 in full; 0 skips it)")
+    ap.add_argument("--profile-applies", type=int, default=0,
+                    help="only run this many two-right-hand-side FP64 applies, then as many complex64 applies on "
+                         "complex64 vectors, of the workload's first frequency on the main thread (for kernel traces)")
+    ap.add_argument("--profile-krylov", action="store_true",
+                    help="only run one single-right-hand-side FGMRES solve of the workload's first frequency on the "
+                         "main thread (for traces and counters of the Krylov kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -101,9 +109,14 @@ def main():
     from extreme_amd import cartesian_model as cm
 
     model = {"c5": cm.config_c5, "c2": cm.config_c2, "c1": cm.config_c1}[args.config]()
+    if args.profile_applies or args.profile_krylov:
+        profile_run(model, args)
+        return
     line = sweep(model, args)
     if not args.no_models and args.config != "c1":
         line["model_batching"] = model_batching(line["_settings"])
+    if args.c3_models > 0:
+        line["batched_uneven_models"] = batched_uneven(args.c3_models, line["_settings"])
     line.pop("_settings")
     torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
@@ -301,6 +314,65 @@ def model_batching(settings):
     del grp, p0, p1
     torch.cuda.empty_cache()
     return out
+
+
+def batched_uneven(nmodels, settings):
+    """The batched uneven-grid workload: seeded models of {32,48,64,96}^2 x {8,12,16,24} cells, each at
+    two of {0.01, 0.1, 1} Hz, both polarisations. Solved one model and frequency at a time (the
+    reference's order, on one handle at a time) and by the batch scheduler (groups of up to four models
+    sharing grid and frequency, four groups in flight). Green-tensor assembly is included in both."""
+    import torch
+    from extreme_amd import batch as B
+    from extreme_amd import cartesian_model as cm
+    models = [cm.config_c3(k) for k in range(nmodels)]
+    B.run(models[:2], settings, group_size=1, workers=1)          # warm: FFT tile measurement of two shapes
+    torch.cuda.synchronize()
+    seq = B.run(models, settings, group_size=1, workers=1)
+    torch.cuda.synchronize()
+    grp = B.run(models, settings, group_size=4, workers=4)
+    torch.cuda.synchronize()
+    same = all(seq.iterations[k] == grp.iterations[k] for k in seq.iterations)
+    log(f"batched uneven models: {seq.solves} solves, one at a time {seq.seconds:.2f} s, grouped {grp.seconds:.2f} s")
+    return {"workload": f"C3: {nmodels} of 256 seeded models, (Nx, Ny) in {{32,48,64,96}}^2, Nz in {{8,12,16,24}}, "
+                        "2 of {0.01, 0.1, 1} Hz each, both polarisations, FP64, tol 1e-8",
+            "right_hand_sides": seq.solves, "one_at_a_time_s": seq.seconds, "grouped_s": grp.seconds,
+            "solves_per_s_one_at_a_time": seq.solves / seq.seconds, "solves_per_s_grouped": grp.solves / grp.seconds,
+            "speedup": seq.seconds / grp.seconds, "groups": grp.groups, "group_size": 4, "workers": 4,
+            "iterations_equal": same, "converged": seq.converged and grp.converged,
+            "true_residual_max": max(seq.max_true_residual, grp.max_true_residual)}
+
+
+def profile_run(model, args):
+    """Kernel-trace workloads on one host thread: operator applies of both polarisations' chi0 (FP64,
+    then complex64 on complex64 vectors), or one FGMRES solve of one polarisation."""
+    import torch
+    from extreme_amd import forward as F
+    from extreme_amd import xtreme as X
+    f = model.frequencies[0]
+    p = F.FrequencySolve(model, f, precision="mixed")
+    torch.cuda.synchronize()
+    if args.profile_krylov:
+        x = p.chi0[0].clone()
+        t0 = time.perf_counter()
+        info = X.AnomalyCurrentFgmresSolver(p.op).solve(p.chi0[0], x)
+        log(f"profile: one solve at {f:g} Hz, {info.number_of_iterations} iterations, {info.matvecs} applies, "
+            f"{time.perf_counter() - t0:.3f} s")
+        return
+    y = torch.empty_like(p.chi0)
+    t0 = time.perf_counter()
+    for _ in range(args.profile_applies):
+        p.op.apply(p.chi0, y)
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.profile_applies
+    # complex64 applies on complex64 vectors, through the mixed handle's FP32 path
+    x32 = p.chi0.to(torch.complex64)
+    y32 = torch.empty_like(x32)
+    t0 = time.perf_counter()
+    for _ in range(args.profile_applies):
+        p.op.apply_c64(x32, y32)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / args.profile_applies
+    log(f"profile: {args.profile_applies} applies at {f:g} Hz, {t64 * 1e3:.3f} ms FP64, {t32 * 1e3:.3f} ms complex64")
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):

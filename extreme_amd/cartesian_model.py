@@ -211,6 +211,45 @@ def config_c2(model_index: int = 0) -> CartesianModel:
     return CartesianModel(nx, ny, 200.0, 200.0, depth, thick, sigma, sec, "C2", freqs)
 
 
+def config_c3(model_index: int = 0) -> CartesianModel:
+    """Batched, uneven sizes: model k of 256, Nx and Ny drawn from {32, 48, 64, 96} and Nz from
+    {8, 12, 16, 24}, dx = dy = 250 m, dz = 100 m (2 km is a cell face), the 3-layer background and
+    20 seeded boxes as in the 128x128x32 model (sides 4-32 cells, log-uniform 1-1000 ohm-m); two
+    distinct frequencies of {0.01, 0.1, 1} Hz per model."""
+    rng = SplitMix64(seed_for(3, model_index))
+    sizes, depths = (32, 48, 64, 96), (8, 12, 16, 24)
+    nx = sizes[rng.randint(0, 3)]
+    ny = sizes[rng.randint(0, 3)]
+    nz = depths[rng.randint(0, 3)]
+    skip = rng.randint(0, 2)                        # the frequency of the three not used
+    freqs = [f for k, f in enumerate((0.01, 0.1, 1.0)) if k != skip]
+    depth = np.arange(nz) * 100.0
+    thick = np.full(nz, 100.0)
+    sec = _three_layer_section(nz * 100.0)
+    sigma = np.empty((nx, ny, nz))
+    for k in range(nz):
+        sigma[:, :, k] = sec.sigma[sec.background_layer(depth[k], depth[k] + thick[k])]
+    _paint_boxes(rng, sigma, 20, 4, 32, 0.0, 3.0)
+    return CartesianModel(nx, ny, 250.0, 250.0, depth, thick, sigma, sec, "C3", freqs)
+
+
+def config_c4(model_index: int = 0) -> CartesianModel:
+    """Learned-preconditioner training: model k of 4096, 64x64x16 cells, dx = dy = 250 m, dz = 100 m,
+    5-30 seeded boxes (sides 4-32 cells, log-uniform 1-1000 ohm-m) over the 3-layer background;
+    frequencies 0.01, 0.1, 1 and 10 Hz."""
+    nx = ny = 64
+    nz = 16
+    rng = SplitMix64(seed_for(4, model_index))
+    depth = np.arange(nz) * 100.0
+    thick = np.full(nz, 100.0)
+    sec = _three_layer_section(nz * 100.0)
+    sigma = np.empty((nx, ny, nz))
+    for k in range(nz):
+        sigma[:, :, k] = sec.sigma[sec.background_layer(depth[k], depth[k] + thick[k])]
+    _paint_boxes(rng, sigma, rng.randint(5, 30), 4, 32, 0.0, 3.0)
+    return CartesianModel(nx, ny, 250.0, 250.0, depth, thick, sigma, sec, "C4", [0.01, 0.1, 1.0, 10.0])
+
+
 def config_c5(model_index: int = 0) -> CartesianModel:
     """Large: 256x256x64, dx=dy=100 m, dz 25 -> 800 m geometric with faces snapped to 2 and 10 km,
     the 3-layer background plus 60 seeded boxes (log-uniform 0.3-3000 ohm-m); 12 frequencies."""

@@ -1310,6 +1310,30 @@ int xtr_op_apply_device(xtr_op *op, const void *x, void *y, int nrhs)
     });
 }
 
+int xtr_op_apply_c64_device(xtr_op *op, const void *x, void *y, int nrhs)
+{
+    return guarded([&] {
+        require(op && x && y, "null argument");
+        require(op->precision == XTR_FP32 || op->precision == XTR_MIXED,
+                "complex64 applies need a handle created with XTR_FP32 or XTR_MIXED");
+        require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs out of range");
+        xtr::RhsPtrs v{};
+        const int64_t n = op->g.n();
+        for (int r = 0; r < nrhs; ++r) {
+            v.x[r] = (const double2 *)((const float2 *)x + r * n);
+            v.y[r] = (double2 *)((float2 *)y + r * n);
+        }
+        struct ActiveGuard {
+            xtr_op *o;
+            int prev;
+            ~ActiveGuard() { o->active = prev; }
+        } guard{op, op->active};
+        op->active = XTR_FP32;
+        op->apply_raw(v, nrhs, op->stream, true);
+        XTR_CHECK(hipStreamSynchronize(op->stream));
+    });
+}
+
 int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs)
 {
     return guarded([&] {

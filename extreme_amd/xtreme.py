@@ -83,6 +83,7 @@ def lib():
         L.xtr_op_prepare.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.xtr_op_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.xtr_op_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.xtr_op_apply_c64_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.xtr_fgmres_solve.argtypes = [C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p, C.POINTER(_Info),
                                        ITER_FN, C.c_void_p]
         L.xtr_fgmres_solve_device.argtypes = L.xtr_fgmres_solve.argtypes
@@ -348,6 +349,22 @@ class ConvolutionOperator:
             y = np.empty_like(x)
         _host_vector(y, nrhs * self.n, "y")
         _check(lib().xtr_op_apply(self._h, _ptr(x), _ptr(y), nrhs))
+        return y
+
+    def apply_c64(self, x, y=None):
+        """The apply in complex64 on complex64 device tensors (n or [nrhs, n]); the operator must be
+        created with precision "fp32" or "mixed"."""
+        torch = _torch()
+        nrhs = 1 if x.ndim == 1 else x.shape[0]
+        for t, what in ((x, "x"), (y, "y")):
+            if t is None:
+                continue
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous()
+                    and t.numel() == nrhs * self.n):
+                raise ValueError(f"{what} must be a contiguous complex64 GPU tensor of {nrhs * self.n} values")
+        if y is None:
+            y = x.new_empty(x.shape)
+        _check(lib().xtr_op_apply_c64_device(self._h, _ptr(x), _ptr(y), nrhs))
         return y
 
     # --- observation level (AtoO) ----------------------------------------------------------

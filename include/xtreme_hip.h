@@ -134,6 +134,13 @@ int xtr_op_apply(xtr_op *op, const xtr_complex *x, xtr_complex *y, int nrhs);
  * and complete on return. y must not overlap x (an error otherwise). */
 int xtr_op_apply_device(xtr_op *op, const void *x, void *y, int nrhs);
 
+/* The apply in complex64 on complex64 vectors (interleaved {float re, im}, nrhs consecutive device
+ * vectors of n values): the FP32 Green tensor, FFTs and contraction, with R x and the last step
+ * formed in FP64 and rounded once on store. Needs a handle created with XTR_FP32 or XTR_MIXED. No
+ * counterpart in the reference (complex128 throughout); the operator of the mixed-precision
+ * correction solves, for callers that run their own. */
+int xtr_op_apply_c64_device(xtr_op *op, const void *x, void *y, int nrhs);
+
 /* Replaces: AnomalyCurrentFgmresSolver.Solve -> FlexibleGmresWithGmresPreconditioner.Solve
  * (Extreme.Cartesian/Forward/AnomalyCurrentFgmresSolver.cs:59-85,
  *  Extreme.Fgmres/FlexibleGmresWithGmresPreconditioner.cs:29-63): FGMRES with an inner GMRES

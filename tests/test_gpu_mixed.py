@@ -89,3 +89,19 @@ def test_mixed_operator_fp64_applies_unchanged(oracle, torch_cuda):
     back = ops["mixed"].green_tensor()
     for c in X.ConvolutionOperator.COMPONENTS:
         assert np.abs(back[c] - G[c]).max() <= 1e-14 * np.abs(G[c]).max()
+
+
+def test_complex64_vector_apply(oracle, torch_cuda):
+    """The complex64 apply on complex64 vectors (the correction solves' operator) against the FP64
+    apply of the same handle: within the complex64 rounding of the tensor, FFTs and vectors."""
+    torch = torch_cuda
+    m = cm.config_c1()
+    p = F.FrequencySolve(m, m.frequencies[0], precision="mixed")
+    x = p.chi0.clone()
+    y64 = p.op.apply(x)
+    y32 = p.op.apply_c64(x.to(torch.complex64))
+    e = float(torch.linalg.norm(y32.to(torch.complex128) - y64) / torch.linalg.norm(y64))
+    print(f"complex64 apply on complex64 vectors: rel error {e:.2e} against FP64", flush=True)
+    assert e < 2e-6
+    # the handle goes back to FP64 applies afterwards
+    assert torch.equal(p.op.apply(x), y64)
