@@ -378,34 +378,46 @@ def test_training_on_krylov_pairs_and_what_it_gains(torch_cuda, monkeypatch):
 
 
 @pytest.mark.timeout(900)
-def test_spectral_network_trained_on_c4_models(torch_cuda, monkeypatch):
-    """The network with the spectral branch (K = 6: 1.56M parameters at 64 x 64 x 16), trained on
-    the preconditioner pairs of seeded 64 x 64 x 16 models (5-30 boxes, 1-1000 ohm-m) at the four
-    training frequencies, then compared on a held-out model with the reference's inner GMRES(10)
-    and with no preconditioner, in operator applies and wall time. Every solve reaches the FP64
-    true residual and the same solution; the printed table is the measurement."""
+@pytest.mark.parametrize("workload", ["C4", "C2"])
+def test_spectral_network_trained_on_seeded_models(torch_cuda, monkeypatch, workload):
+    """The network with the spectral branch, trained on the preconditioner pairs of seeded models, then
+    compared on a held-out model with the reference's inner GMRES(10) and with no preconditioner, in
+    operator applies and wall time. C4: 64 x 64 x 16 models (5-30 boxes, 1-1000 ohm-m) at the four
+    training frequencies, K = 6 (1.56M parameters). C2: 128 x 128 x 32 models at two of their
+    frequencies, K = 3 (1.81M parameters). Every solve reaches the FP64 true residual; the printed
+    table is the measurement."""
     torch = torch_cuda
     monkeypatch.setenv("XTR_FFT_TILES", "dddd")
     from extreme_amd import learned as Lrn
     s = X.ForwardSettings(residual=1e-8, inner_buffer_length=10, outer_buffer_length=30, max_repeats_number=100)
-    models = [cm.config_c4(k) for k in range(1, 9)]
+    if workload == "C4":
+        models, freqs, K, held, tests = [cm.config_c4(k) for k in range(1, 9)], [0.01, 0.1, 1.0, 10.0], 6, \
+            cm.config_c4(0), (0.1, 10.0)
+    else:
+        c2 = cm.config_c2()
+        models, freqs, K, held, tests = [cm.config_c2(k) for k in range(1, 5)], [c2.frequencies[0], c2.frequencies[6]], \
+            3, cm.config_c2(0), (c2.frequencies[0], c2.frequencies[6])
     t0 = time.perf_counter()
-    sets = Lrn.harvest(models, models[0].frequencies, s, capacity=16)
-    print(f"{sum(len(p.vs) for p in sets)} pairs from {len(sets)} solves in {time.perf_counter() - t0:.1f} s",
+    sets = Lrn.harvest(models, freqs, s, capacity=16)
+    print(f"{workload}: {sum(len(p.vs) for p in sets)} pairs from {len(sets)} solves in {time.perf_counter() - t0:.1f} s",
           flush=True)
-    net = X.LearnedPreconditioner(64, 64, 16, seed=5, linear=True, modes=6)
+    m0 = models[0]
+    net = X.LearnedPreconditioner(m0.nx, m0.ny, m0.nz, seed=5, linear=True, modes=K)
     print(f"parameters: {net.parameter_count()}", flush=True)
-    assert 1.4e6 < net.parameter_count() < 1.7e6
+    assert 1.4e6 < net.parameter_count() < 1.9e6
     first = Lrn.mean_loss(net, sets)
     t0 = time.perf_counter()
     Lrn.train(net, sets, steps=400, lr=1e-3, batch=32, log=lambda m: print(m, flush=True))
     final = Lrn.mean_loss(net, sets)
     print(f"training loss {first:.4f} -> {final:.4f} ({time.perf_counter() - t0:.1f} s)", flush=True)
     assert final < 0.9 * first
-    held = cm.config_c4(0)
-    for f in (0.1, 10.0):
+    # the same loss on the held-out model's own pairs: how the fit carries over to unseen Krylov vectors
+    held_sets = Lrn.harvest([held], tests, s, capacity=16)
+    print(f"held-out pairs: loss {Lrn.mean_loss(net, held_sets):.4f} (the untrained network, z = v: 1.0)", flush=True)
+    for f in tests:
         trials = Lrn.compare(held, f, s, {"learned (conv + spectral)": net}, repeats=2)
         for t in trials:
-            print(f"C4 held-out, {f:g} Hz, {t.name}: converged {t.converged}, outer iterations {t.outer_iterations}, "
-                  f"applies {t.applies}, true residual {t.true_residual:.2e}, {t.seconds * 1e3:.1f} ms", flush=True)
+            print(f"{workload} held-out, {f:g} Hz, {t.name}: converged {t.converged}, outer iterations "
+                  f"{t.outer_iterations}, applies {t.applies}, true residual {t.true_residual:.2e}, "
+                  f"{t.seconds * 1e3:.1f} ms", flush=True)
             assert t.converged and t.true_residual < 1e-8
