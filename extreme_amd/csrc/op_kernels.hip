@@ -1541,21 +1541,25 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
 // ---- folded contraction on MFMA, nz = 64 -------------------------------------------------------------
 // A folded block (264 KB in FP64) is streamed through LDS in nine units: the spectrum columns of its
 // images, xx, xy, yy, zz, and the xz and yz blocks each as a diagonal and an off-diagonal pair of
-// 32 x 32 quadrants. Each wave owns one 16-row tile of each component (rows 16w..16w+15 of x, y and
-// z), and every unit then gives the four tile owners the same number of MFMAs; eight waves (two per
-// SIMD, so one computes while the other waits) pair up: waves w and w+4 own the same tiles and take
-// half of each unit's k-steps each, their partial sums meet through LDS at the end of the block. One
-// accumulator per tile:
-// with B1 = [Re U | Im U] and B2 = [-Im U | Re U], Re G . B1 + Im G . B2 = [Re Y | Im Y]. The
-// spectrum unit is copied (mirror signs applied, columns rotated over the LDS banks) into an area that
-// lives for the whole block; the others are contracted from their ring slot while the next NSLOT-1
-// units are in flight (FP64: 4 slots of 33 wave-loads, FP32: 8 of 17).
+// 32 x 32 quadrants, through a ring of slots (FP64: 4 of 33 wave-loads, FP32: 8 of 17) with NSLOT-1
+// units in flight. Wave w = 4 g + t owns rows 16t..16t+15 of x, y and z and the k-half g (k in
+// 32g..32g+31 of every input component); waves t and 4 + t add their partial sums at the end of the
+// block. The spectrum unit is read once per block into registers: lane (kl, n) keeps B[k][n] of its
+// 24 k-steps, B = [Re U | Im U] with the image's mirror signs applied. Two accumulators per tile,
+// D1 = Re G . B and D2 = Im G . B, give Re Y = D1[:, :8] - D2[:, 8:] and Im Y = D1[:, 8:] + D2[:, :8]
+// through one lane exchange at the end; xy's A values serve both of its products. The -xz^T, -yz^T
+// blocks enter through the FP64 MFMA's negate modifier (FP32: negated values). A quadrant pair holds
+// the quadrants (row half h, column half h) (diagonal unit) or (h, 1 - h) (off-diagonal unit); the
+// direct and the transposed product of wave (t, g) both need the quadrant (t / 2, g) or (g, t / 2), so
+// each quadrant unit is contracted by the four waves with g == t / 2 (diagonal) or g != t / 2, one per
+// SIMD.
 namespace fold64 {
 constexpr int NZ = 64, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
 constexpr int MASK = 15;                  // asym_swizzle_mask(64)
 constexpr int NCOL = 8, UCOLS = NCOL * C;
 constexpr int NUNIT = 9;                  // spectrum, xx, xy, yy, zz, xz diag, xz off, yz diag, yz off
 constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
+constexpr int KS = 8;                     // k-steps of one component's k-half (32 values, 4 per step)
 __host__ __device__ constexpr int uelems(int u) { return u == 0 ? UCOLS : u <= 4 ? S : A / 2; }
 template <typename T>
 struct Cfg {
@@ -1566,6 +1570,7 @@ struct Cfg {
     __host__ __device__ static constexpr int nw(int u) { return (uelems(u) + EPC - 1) / EPC; }
     static constexpr int SLOTW = (S + EPC - 1) / EPC;       // the largest unit, in wave-loads
     static constexpr int SLOT = SLOTW * EPC;
+    static constexpr int EXCH = 4 * 3 * 64;                 // exchange: [t][tile][lane] of 4 T
     // wave-loads of unit u issued by wave w
     __host__ __device__ static constexpr int cnt(int u, int w) { return (nw(u) - w + NWAVE - 1) / NWAVE; }
     // wave-loads a wave has issued after unit u's when step u waits (units u+1 .. u+NSLOT-2: unit
@@ -1580,14 +1585,17 @@ struct Cfg {
 };
 static_assert(Cfg<double>::nw(0) <= Cfg<double>::SLOTW && Cfg<double>::nw(5) <= Cfg<double>::SLOTW, "slot (FP64)");
 static_assert(Cfg<float>::nw(0) <= Cfg<float>::SLOTW && Cfg<float>::nw(5) <= Cfg<float>::SLOTW, "slot (FP32)");
-static_assert(4 * Cfg<double>::SLOT * 16 + UCOLS * 16 <= 160 * 1024, "LDS (FP64)");
-static_assert(8 * Cfg<float>::SLOT * 8 + UCOLS * 8 <= 160 * 1024, "LDS (FP32)");
+static_assert(4 * Cfg<double>::SLOT * 16 + Cfg<double>::EXCH * 4 * 8 <= 160 * 1024, "LDS (FP64)");
+static_assert(8 * Cfg<float>::SLOT * 8 + Cfg<float>::EXCH * 4 * 4 <= 160 * 1024, "LDS (FP32)");
 static_assert(Cfg<double>::ahead(0, true, 0) < 64 && Cfg<float>::ahead(0, true, 0) < 64, "vmcnt range");
 static_assert(Cfg<double>::NSLOT - 1 < NUNIT && Cfg<float>::NSLOT - 1 < NUNIT, "look-ahead within one block");
 static_assert(S % 2 == 0 && C % 2 == 0, "two-element lanes stay inside one run");
+static_assert(Cfg<double>::nw(0) * Cfg<double>::EPC == UCOLS && Cfg<float>::nw(0) * Cfg<float>::EPC == UCOLS,
+              "the spectrum unit has no padding lanes");
 }  // namespace fold64
 
-// Step U's wait: wave w lets through the wave-loads it issued after unit U's (counts differ by wave).
+// Step U's wait: wave w lets through the wave-loads it issued after unit U's (counts differ by wave;
+// `wave` is uniform, so the dispatch is scalar).
 template <typename T, int U, bool HN, int W = 0>
 __device__ __forceinline__ void fold64_wait_wave(int wave)
 {
@@ -1599,7 +1607,9 @@ __device__ __forceinline__ void fold64_wait_wave(int wave)
 
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-// Copy of unit UT of block b into the slot at LDS address dst.
+// Copy of unit UT of block b into the slot at LDS address dst. In the spectrum unit, column q's
+// element kk sits at position (kk + 2q) mod C, which spreads the columns over the LDS banks for the
+// B reads.
 template <typename T, int UT>
 __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, const cplx<T> *buf, int64_t M, int64_t b,
                                              const FoldImages &im, int nrhs, unsigned dst, int wave, int lane)
@@ -1615,9 +1625,11 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
         const cplx<T> *src;
         if constexpr (UT == 0) {
             int q = e / C, j = e - q * C;
-            if (q >= im.n * nrhs) q = j = 0;                 // unused column / padding: valid data
+            if (q >= im.n * nrhs) q = j = 0;                 // unused column: valid data
+            int kk = j - 2 * q;                              // even, so a lane's two FP32 elements
+            if (kk < 0) kk += C;                             // stay contiguous
             const int img = q / nrhs, h = q - img * nrhs;
-            src = buf + h * M + im.l(img) * C + j;
+            src = buf + h * M + im.l(img) * C + kk;
         } else if constexpr (UT <= 4) {
             src = G + b * GB + (UT - 1) * S + (e < S ? e : S - K::EPL);   // padding lanes re-read
         } else {
@@ -1635,115 +1647,137 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
 }
 #pragma clang diagnostic pop
 
-// Wave w's MFMAs on one unit for its tile of components O (rows 16w..) over k-steps s0 .. s0+NS-1 of
-// component CI. KIND 0: packed symmetric unit (offset 0); 1: quadrant pair, B(r, c); 2: quadrant
-// pair, -B(c, r).
-template <typename T, int O, int CI, int KIND, int NS>
-__device__ __forceinline__ void fold64_job(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua, int s0,
-                                           int r, int kl, int q, int part, typename MfmaAcc<T>::type &acc)
+// Per-lane byte offsets of the A values of k-step s (k = 32 g + 4 s + kl, local to the component)
+// for row r = 16 t + i: packed symmetric unit (po), the quadrant of the direct product (qo: row r,
+// column k) and of the transposed one (to: row k, column r). They depend only on the lane and the
+// wave, so each wave computes them once.
+template <typename T>
+__device__ __forceinline__ void fold64_offsets(int t, int g, int lane, unsigned (&po)[fold64::KS],
+                                               unsigned (&qo)[fold64::KS], unsigned (&to)[fold64::KS])
 {
-    using namespace fold64;
-    auto a_at = [&](int s) -> cplx<T> {
-        const int c = 4 * s + kl;
-        if constexpr (KIND == 0) {
-            return blk[c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2];
-        } else if constexpr (KIND == 1) {
-            return blk[(r >> 5) * 1024 + (r & 31) * 32 + ((c ^ (r & MASK)) & 31)];
-        } else {
-            const cplx<T> v = blk[(c >> 5) * 1024 + (c & 31) * 32 + ((r ^ (c & MASK)) & 31)];
-            return mk<T>(-v.x, -v.y);
-        }
-    };
-    auto b_at = [&](int s) -> cplx<T> {
-        int j = CI * NZ + 4 * s + kl + 2 * q;             // rotated position of k = CI NZ + 4 s + kl
-        if (j >= C) j -= C;
-        return ua[q * C + j];
-    };
-    cplx<T> ga = a_at(s0), gb = b_at(s0);
+    constexpr unsigned E = sizeof(cplx<T>);
+    const int r = 16 * t + (lane & 15), kl = lane >> 4;
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-        cplx<T> na = ga, nb = gb;
-        if (j + 1 < NS) {
-            na = a_at(s0 + j + 1);
-            nb = b_at(s0 + j + 1);
-        }
-        const T b1 = part ? gb.y : gb.x, b2 = part ? gb.x : -gb.y;
-        acc = mfma16(ga.x, b1, acc);
-        acc = mfma16(ga.y, b2, acc);
-        ga = na;
-        gb = nb;
+    for (int s = 0; s < fold64::KS; ++s) {
+        const int c = 32 * g + 4 * s + kl, cq = 4 * s + kl;  // cq = c & 31
+        po[s] = E * (unsigned)(c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2);
+        qo[s] = E * (unsigned)((r >> 5) * 1024 + (r & 31) * 32 + (cq ^ (r & fold64::MASK)));
+        to[s] = E * (unsigned)(g * 1024 + cq * 32 + ((r & 31) ^ (cq & fold64::MASK)));
     }
 }
 
-// Wave w: tiles of rows 16 (w % 4) .. of each component; waves w and w + 4 split each job's k-steps
-// (first and second half).
-template <typename T, int UT>
-__device__ __forceinline__ void fold64_compute(const cplx<T> *__restrict__ blk, const cplx<T> *__restrict__ ua,
-                                               int wave, int lane, typename MfmaAcc<T>::type (&acc)[3])
+// D1 += (NEG ? -Re a : Re a) b, D2 += (NEG ? -Im a : Im a) b over the 8 k-steps of one A tile.
+template <typename T, bool NEG>
+__device__ __forceinline__ void fold64_job(const cplx<T> *__restrict__ blk, const unsigned (&off)[fold64::KS],
+                                           const T (&b)[fold64::KS], typename MfmaAcc<T>::type &d1,
+                                           typename MfmaAcc<T>::type &d2)
 {
-    const int tw = wave & 3, g = wave >> 2;
-    const int r = 16 * tw + (lane & 15), kl = lane >> 4, q = lane & 7, part = (lane >> 3) & 1;
-    const int h = tw >> 1;                                 // row half of the wave's tiles
+    cplx<T> a[fold64::KS];
+#pragma unroll
+    for (int s = 0; s < fold64::KS; ++s) a[s] = lds_cplx<T>(blk, off[s]);
+#pragma unroll
+    for (int s = 0; s < fold64::KS; ++s) {
+        d1 = mfma16s<NEG>(a[s].x, b[s], d1);
+        d2 = mfma16s<NEG>(a[s].y, b[s], d2);
+    }
+}
+
+// The MFMAs of unit UT for wave (t, g).
+template <typename T, int UT>
+__device__ __forceinline__ void fold64_compute(const cplx<T> *__restrict__ blk, const T (&bf)[3][fold64::KS],
+                                               const unsigned (&po)[fold64::KS], const unsigned (&qo)[fold64::KS],
+                                               const unsigned (&to)[fold64::KS], int t, int g,
+                                               typename MfmaAcc<T>::type (&d1)[3], typename MfmaAcc<T>::type (&d2)[3])
+{
+    using namespace fold64;
     if constexpr (UT == 1) {
-        fold64_job<T, 0, 0, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[0]);
+        fold64_job<T, false>(blk, po, bf[0], d1[0], d2[0]);          // xx . u_x -> y_x
     } else if constexpr (UT == 2) {
-        fold64_job<T, 0, 1, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[0]);
-        fold64_job<T, 1, 0, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[1]);
+        cplx<T> a[KS];                                               // xy . u_y -> y_x, xy . u_x -> y_y
+#pragma unroll
+        for (int s = 0; s < KS; ++s) a[s] = lds_cplx<T>(blk, po[s]);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            d1[0] = mfma16s<false>(a[s].x, bf[1][s], d1[0]);
+            d2[0] = mfma16s<false>(a[s].y, bf[1][s], d2[0]);
+            d1[1] = mfma16s<false>(a[s].x, bf[0][s], d1[1]);
+            d2[1] = mfma16s<false>(a[s].y, bf[0][s], d2[1]);
+        }
     } else if constexpr (UT == 3) {
-        fold64_job<T, 1, 1, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[1]);
+        fold64_job<T, false>(blk, po, bf[1], d1[1], d2[1]);          // yy . u_y -> y_y
     } else if constexpr (UT == 4) {
-        fold64_job<T, 2, 2, 0, 8>(blk, ua, 8 * g, r, kl, q, part, acc[2]);
+        fold64_job<T, false>(blk, po, bf[2], d1[2], d2[2]);          // zz . u_z -> y_z
     } else {
-        constexpr int o = UT <= 6 ? 0 : 1;                   // x (xz) or y (yz) rows and k of the transpose
-        const int hs = (UT == 5 || UT == 7) ? h : 1 - h;      // column half: diag same as rows, off other
-        fold64_job<T, o, 2, 1, 4>(blk, ua, 8 * hs + 4 * g, r, kl, q, part, acc[o]);
-        fold64_job<T, 2, o, 2, 4>(blk, ua, 8 * hs + 4 * g, r, kl, q, part, acc[2]);
+        constexpr int o = UT <= 6 ? 0 : 1;                           // xz (x) or yz (y)
+        constexpr bool diag = UT == 5 || UT == 7;
+        if ((g == (t >> 1)) == diag) {
+            fold64_job<T, false>(blk, qo, bf[2], d1[o], d2[o]);      // xz . u_z -> y_x
+            fold64_job<T, true>(blk, to, bf[o], d1[2], d2[2]);       // -xz^T . u_x -> y_z
+        }
     }
 }
 
 // One step of the unit stream: wait for unit U of block b (slot s % NSLOT); after the barrier every
-// wave is done with the previous step's slot, so unit U + NSLOT - 1 is copied into it; then contract.
+// wave is done with the previous step's slot, so unit U + NSLOT - 1 is copied into it; then contract
+// (unit 0: read this wave's B values).
 template <typename T, int U>
 __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t M, int64_t b,
                                             int64_t bn, bool has_next, const FoldImages &im, const FoldImages &imn,
-                                            int nrhs, int64_t s, cplx<T> *ring, cplx<T> *ua, unsigned base, int wave,
-                                            int lane, int t, typename MfmaAcc<T>::type (&acc)[3])
+                                            int nrhs, int64_t s, cplx<T> *ring, unsigned base, int wave, int lane,
+                                            T (&bf)[3][fold64::KS], const unsigned (&po)[fold64::KS],
+                                            const unsigned (&qo)[fold64::KS], const unsigned (&to)[fold64::KS],
+                                            typename MfmaAcc<T>::type (&d1)[3], typename MfmaAcc<T>::type (&d2)[3])
 {
     using namespace fold64;
     using K = Cfg<T>;
     if (has_next) fold64_wait_wave<T, U, true>(wave);
     else fold64_wait_wave<T, U, false>(wave);
-    XTR_BARRIER();                                         // unit U in LDS; previous slot and ua reads done
+    XTR_BARRIER();                                         // unit U in LDS; previous slot's reads done
     constexpr int V = U + K::NSLOT - 1;                    // unit copied now, into the previous step's slot
     const unsigned vdst = base + (unsigned)(((s + K::NSLOT - 1) % K::NSLOT) * K::SLOT * K::E);
     if constexpr (V < NUNIT) fold64_issue<T, V>(G, buf, M, b, im, nrhs, vdst, wave, lane);
     else if (has_next) fold64_issue<T, V - NUNIT>(G, buf, M, bn, imn, nrhs, vdst, wave, lane);
     const cplx<T> *blk = ring + (s % K::NSLOT) * K::SLOT;
+    const int t = wave & 3, g = wave >> 2;
     if constexpr (U == 0) {
-        // spectrum columns -> block-long area, mirror signs applied, column q rotated by 2q: thread t
-        // copies elements k = t % 64 + 64 m (m < 3, the component) of column q = t / 64
-        static_assert(UCOLS == NTHR * 3 && C == 3 * 64, "copy mapping");
-        const int q = t >> 6, k0 = t & 63;
+        // lane (kl, n): B[k][n] for k = 64 ci + 32 g + 4 s + kl, column q = n % 8, Re (n < 8) or Im
+        const int n = lane & 15, q = n & 7, part = n >> 3, kl = lane >> 4;
         const int img = (q / nrhs) & 3;
         const bool fx = im.mx(img) != 0, fy = im.my(img) != 0;
+        const T *ucol = reinterpret_cast<const T *>(blk + q * C) + part;
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const int k = k0 + 64 * m;
-            cplx<T> v = blk[q * C + k];
-            const bool flip = m == 0 ? fx : m == 1 ? fy : false;
-            if (flip) v = mk<T>(-v.x, -v.y);
-            int j = k + 2 * q;
-            if (j >= C) j -= C;
-            ua[q * C + j] = v;
-        }
+        for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+            for (int st = 0; st < KS; ++st) {
+                int j = 64 * ci + 32 * g + 4 * st + kl + 2 * q;
+                if (j >= C) j -= C;
+                const T v = ucol[2 * j];
+                const bool flip = (ci == 0 && fx) || (ci == 1 && fy);
+                bf[ci][st] = flip ? -v : v;
+            }
     } else {
-        fold64_compute<T, U>(blk, ua, wave, lane, acc);
+        fold64_compute<T, U>(blk, bf, po, qo, to, t, g, d1, d2);
     }
 }
 
+// Tile O of rows 16t.. from D1 and D2 (the Re/Im exchange between lanes n and n ^ 8).
+template <typename T>
+__device__ __forceinline__ typename MfmaAcc<T>::type fold64_combine(typename MfmaAcc<T>::type d1,
+                                                                   typename MfmaAcc<T>::type d2, int lane)
+{
+    const int part = (lane >> 3) & 1;
+    typename MfmaAcc<T>::type y;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const T other = __shfl_xor(d2[r], 8, 64);
+        y[r] = part ? d1[r] + other : d1[r] - other;
+    }
+    return y;
+}
+
 template <typename T, int O>
-__device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type d, cplx<T> *buf, int64_t M,
-                                             const FoldImages &im, int nrhs, int wave, int lane)
+__device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type y, cplx<T> *buf, int64_t M,
+                                             const FoldImages &im, int nrhs, int t, int lane)
 {
     using namespace fold64;
     const int n = lane & 15, q = n & 7, part = n >> 3;
@@ -1753,8 +1787,8 @@ __device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type d, cplx<T
     T *dst = reinterpret_cast<T *>(buf + h * M + im.l(img) * C + O * NZ) + part;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int rr = 16 * wave + mfma_row<T>(lane, r);
-        const T v = flip ? -d[r] : d[r];
+        const int rr = 16 * t + mfma_row<T>(lane, r);
+        const T v = flip ? -y[r] : y[r];
         if (live) dst[2 * rr] = v;
     }
 }
@@ -1767,12 +1801,15 @@ __global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cpl
     using K = Cfg<T>;
     using acc_t = typename MfmaAcc<T>::type;
     __shared__ cplx<T> ring[K::NSLOT * K::SLOT];
-    __shared__ __attribute__((aligned(32))) cplx<T> ua[UCOLS];   // 32: the end-of-block exchange
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = threadIdx.x;
+    __shared__ acc_t exch[K::EXCH];                              // [t][tile][lane]: waves 4..7 -> 0..3
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int t = wave & 3, g = wave >> 2;
     const unsigned base = (unsigned)(size_t)(__attribute__((address_space(3))) cplx<T> *)ring;
-    const int64_t g = gridDim.x;                                 // the launch keeps gridDim.x <= Lq
+    const int64_t gs = gridDim.x;                                // the launch keeps gridDim.x <= Lq
     const int64_t b0 = blockIdx.x;
-    const int64_t cnt = (Lq - 1 - b0) / g + 1;
+    const int64_t cnt = (Lq - 1 - b0) / gs + 1;
+    unsigned po[KS], qo[KS], to[KS];
+    fold64_offsets<T>(t, g, lane, po, qo, to);
     {
         // units 0 .. NSLOT-2 of the first block into slots 0 .. NSLOT-2
         const FoldImages im0 = fold_images(b0, nx, ny);
@@ -1786,39 +1823,40 @@ __global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cpl
             fold64_issue<T, 6>(G, buf, M, b0, im0, nrhs, base + 6u * K::SLOT * K::E, wave, lane);
         }
     }
+    T bf[3][KS];
     for (int64_t i = 0; i < cnt; ++i) {
-        const int64_t b = b0 + i * g, bn = b + g;
+        const int64_t b = b0 + i * gs, bn = b + gs;
         const bool hn = i + 1 < cnt;
         const FoldImages im = fold_images(b, nx, ny);
         const FoldImages imn = fold_images(hn ? bn : b, nx, ny);
         const int64_t s = i * NUNIT;
-        acc_t acc[3];
-        acc[0] = acc_t{0, 0, 0, 0};
-        acc[1] = acc[0];
-        acc[2] = acc[0];
-        fold64_step<T, 0>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 0, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 1>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 1, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 2>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 2, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 3>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 3, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 4>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 4, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 5>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 5, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 6>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 6, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 7>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 7, ring, ua, base, wave, lane, t, acc);
-        fold64_step<T, 8>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 8, ring, ua, base, wave, lane, t, acc);
-        // waves 4..7 hand their partial sums to waves 0..3 through the spectrum area, once every wave
-        // is done reading it; the next block's spectrum copy comes after its first step's barrier
-        acc_t *ex = reinterpret_cast<acc_t *>(ua);                 // [4][3][64]
-        XTR_BARRIER();
-        if (wave >= 4)
+        acc_t d1[3], d2[3];
 #pragma unroll
-            for (int o = 0; o < 3; ++o) ex[((wave - 4) * 3 + o) * 64 + lane] = acc[o];
-        XTR_BARRIER();
-        if (wave < 4) {
+        for (int o = 0; o < 3; ++o) d1[o] = d2[o] = acc_t{0, 0, 0, 0};
+        fold64_step<T, 0>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 0, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 1>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 1, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 2>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 2, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 3>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 3, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 4>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 4, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 5>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 5, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 6>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 6, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 7>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 7, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_step<T, 8>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 8, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        acc_t y[3];
 #pragma unroll
-            for (int o = 0; o < 3; ++o) acc[o] += ex[(wave * 3 + o) * 64 + lane];
-            fold64_store<T, 0>(acc[0], buf, M, im, nrhs, wave, lane);
-            fold64_store<T, 1>(acc[1], buf, M, im, nrhs, wave, lane);
-            fold64_store<T, 2>(acc[2], buf, M, im, nrhs, wave, lane);
+        for (int o = 0; o < 3; ++o) y[o] = fold64_combine<T>(d1[o], d2[o], lane);
+        // waves 4..7 hand their partial sums to waves 0..3; the exchange area was last read before
+        // this block's first barrier
+        if (g == 1)
+#pragma unroll
+            for (int o = 0; o < 3; ++o) exch[(t * 3 + o) * 64 + lane] = y[o];
+        XTR_BARRIER();
+        if (g == 0) {
+#pragma unroll
+            for (int o = 0; o < 3; ++o) y[o] += exch[(t * 3 + o) * 64 + lane];
+            fold64_store<T, 0>(y[0], buf, M, im, nrhs, t, lane);
+            fold64_store<T, 1>(y[1], buf, M, im, nrhs, t, lane);
+            fold64_store<T, 2>(y[2], buf, M, im, nrhs, t, lane);
         }
     }
     wait_vm<0>();                                                // no copy may land after exit
