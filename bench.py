@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--profile-krylov", action="store_true",
                     help="only run one single-right-hand-side FGMRES solve of the workload's first frequency on the "
                          "main thread (for traces and counters of the Krylov kernels)")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the HBM stream probe (read-only, the contraction's read/write mix, 1:1 copy)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -113,6 +115,14 @@ def main():
         profile_run(model, args)
         return
     line = sweep(model, args)
+    if not args.no_probe:
+        probe = hbm_probe(model.nz)
+        line["hbm_probe"] = probe
+        rl = line["roofline"]
+        mix = probe["contraction_mix"]["gbs"]
+        rl["practical_ceiling"] = {"gbs": mix, "frac": rl["achieved"] / mix,
+                                   "what": "the contraction's read/write mix streamed by the same copy path "
+                                           "without arithmetic (hbm_probe.contraction_mix)"}
     if not args.no_models and args.config != "c1":
         line["model_batching"] = model_batching(line["_settings"])
     if args.c3_models > 0:
@@ -373,6 +383,40 @@ def profile_run(model, args):
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.profile_applies
     log(f"profile: {args.profile_applies} applies at {f:g} Hz, {t64 * 1e3:.3f} ms FP64, {t32 * 1e3:.3f} ms complex64")
+
+
+def hbm_probe(nz, reps=10):
+    """HBM rates of the streaming contraction's copy path without arithmetic (xtr_probe_stream): blocks
+    read into LDS with non-temporal LDS-DMA loads through a ring of 32 KB units, one 512-thread
+    workgroup per CU, and KB written after each block with 16-byte lanes. Three mixes: read only; the
+    contraction's (per folded block at nz = 64: 264 KB of Green tensor + 24.6 KB of spectrum read,
+    24.6 KB written, modelled as 288 KB read / 24 KB written; at nz = 32: 78.8 / 12.3 KB as 192 / 30);
+    and 1:1 (the FFT passes read and write about as much as each other). About 4.7 GB read per launch."""
+    import torch
+    from extreme_amd import xtreme as X
+    read_kb, write_kb = (288, 24) if nz == 64 else (192, 30)
+    blocks = (4700 << 20) // (read_kb << 10)
+    src = torch.ones(blocks * read_kb * 128, dtype=torch.float64, device="cuda")
+    dst = torch.empty(blocks * read_kb * 128, dtype=torch.float64, device="cuda")
+    out = {}
+    for name, wkb in (("read_only", 0), ("contraction_mix", write_kb), ("copy_1to1", read_kb)):
+        for _ in range(2):
+            X.probe_stream(src, dst, blocks, read_kb, wkb)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            X.probe_stream(src, dst, blocks, read_kb, wkb)
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b) / reps
+        nbytes = blocks * (read_kb + wkb) * 1024
+        out[name] = {"read_kb_per_block": read_kb, "write_kb_per_block": wkb, "ms": ms,
+                     "gbs": nbytes / (ms * 1e-3) / 1e9}
+        log(f"hbm probe {name}: {out[name]['gbs']:.0f} GB/s")
+    del src, dst
+    torch.cuda.empty_cache()
+    out["blocks"] = blocks
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["op_kernels.hip", "krylov_kernels.hip", "green_assembly.hip", "obs_kernels.hip", "learned_kernels.hip",
-           "xtr_api.cpp"]
+           "probe_kernels.hip", "xtr_api.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1428,6 +1428,14 @@ int xtr_op_sizes(xtr_op *op, int64_t *n, int64_t *green_bytes, int64_t *buffer_b
     });
 }
 
+int xtr_probe_stream(const void *src, void *dst, int64_t blocks, int read_kb, int write_kb, void *stream)
+{
+    return guarded([&] {
+        require(src && (dst || write_kb == 0), "null argument");
+        xtr::launch_probe_stream(src, dst, blocks, read_kb, write_kb, (hipStream_t)stream);
+    });
+}
+
 int xtr_fgmres_solve_device(xtr_op *op, const xtr_gmres_params *params, const void *b, void *x,
                             xtr_result_info *info, xtr_iteration_fn on_iteration, void *user)
 {

@@ -143,6 +143,7 @@ void launch_green_unfold(const double2 *folded, double2 *full, const Geom &g, hi
 // tensor, c in device block order (xx, xy, yy, zz, xz, yz); stat (12 doubles) cleared here
 void launch_green_mirror_check(const double2 *full, const Geom &g, double *stat, hipStream_t s);
 void launch_to_f32(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
+void launch_probe_stream(const void *src, void *dst, int64_t blocks, int read_kb, int write_kb, hipStream_t s);
 
 // Observation level (obs_kernels.hip): jQ from chi (operator prepared as A: R and f of that type),
 // and the anomalous field at a level from jQ with the AtoO tensor G ([L][ncomp][nz]); spec holds

@@ -112,6 +112,7 @@ def lib():
         L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.xtr_op_green_mirror_check.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.xtr_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.xtr_probe_stream.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p]
         f32p = C.POINTER(C.c_float)
         vpp = C.POINTER(C.c_void_p)
         L.xtr_net_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp]
@@ -148,6 +149,24 @@ def hankel_table():
     """The 431-tap ndec=40 J0/J1 filter (HankelCoefficients.LoadN40, HankelCoefficients.cs:37,40-65)."""
     t = np.loadtxt(os.path.join(_HERE, "data", "hankel_ndec40.txt"))
     return np.ascontiguousarray(t[:, 0]), np.ascontiguousarray(t[:, 1])
+
+
+def probe_stream(src, dst, blocks, read_kb, write_kb):
+    """Launches the HBM stream probe on torch's current stream: `blocks` blocks of read_kb KB read from
+    the device tensor src into LDS as the streaming contraction kernels copy the Green tensor, write_kb
+    KB written to dst after each block (xtr_probe_stream). No counterpart in the reference; a
+    measurement aid for the read/write mix the contraction streams."""
+    torch = _torch()
+    for t, kb, what in ((src, read_kb, "src"), (dst, write_kb, "dst")):
+        if kb == 0 and what == "dst":
+            continue
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+            raise ValueError(f"probe_stream: {what} must be a contiguous GPU tensor")
+        if t.numel() * t.element_size() < blocks * kb * 1024:
+            raise ValueError(f"probe_stream: {what} holds fewer than blocks * {what}_kb KB")
+    stream = torch.cuda.current_stream().cuda_stream
+    _check(lib().xtr_probe_stream(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr() if write_kb else 0),
+                                  blocks, read_kb, write_kb, C.c_void_p(stream)))
 
 
 @dataclass

@@ -290,6 +290,14 @@ void xtr_op_destroy(xtr_op *op);
 /* Message of the last failed call on this thread; returns its length. */
 int xtr_last_error(char *buf, int len);
 
+/* Measurement aid with no counterpart in the reference: the copy path of the streaming contraction
+ * kernels without their arithmetic. `blocks` blocks of read_kb KB (a positive multiple of 32) are
+ * read from src (device memory of blocks * read_kb KB) into LDS with non-temporal LDS-DMA loads, one
+ * 512-thread workgroup per CU, a ring of four 32 KB units; after each block write_kb KB are stored to
+ * dst (blocks * write_kb KB). Launched on `stream` (a hipStream_t; null: the default stream) and not
+ * waited for. Timing it gives the HBM rate the part sustains for the contraction's read/write mix. */
+int xtr_probe_stream(const void *src, void *dst, int64_t blocks, int read_kb, int write_kb, void *stream);
+
 /* ---- learned right preconditioner (no counterpart in the reference) --------------------------------
  * A two-layer 3-D convolution network on the residual field, z = s (v/s + N(v/s)) with s the RMS of
  * v, N = conv(16 -> 6) o relu o conv(8 -> 16) over 3 x 3 x 3 cell neighbourhoods of the anomaly grid,

@@ -274,3 +274,23 @@ def test_register_fft_tiles_other_lengths(oracle, torch_cuda, monkeypatch, shape
         e = rel(op.apply(xs), ref)
         print(f"{shape} {tiles} {prec}: {op.fft_tiles()} vs oracle {e:.2e}", flush=True)
         assert e < tol
+
+
+def test_hbm_stream_probe_writes_its_blocks(torch_cuda):
+    """The stream probe (a measurement aid) reads every block and writes write_kb KB after each: the
+    second double of element e of block b's output is e, and nothing past blocks * write_kb KB is
+    touched."""
+    torch = torch_cuda
+    blocks, read_kb, write_kb = 300, 64, 2
+    src = torch.ones(blocks * read_kb * 128, dtype=torch.float64, device="cuda")
+    dst = torch.full((blocks * write_kb * 128 + 256,), -7.0, dtype=torch.float64, device="cuda")
+    X.probe_stream(src, dst, blocks, read_kb, write_kb)
+    torch.cuda.synchronize()
+    out = dst[: blocks * write_kb * 128].view(blocks, write_kb * 64, 2)
+    e = torch.arange(write_kb * 64, dtype=torch.float64, device="cuda")
+    assert torch.equal(out[:, :, 1], e.expand(blocks, -1))
+    assert torch.all(dst[blocks * write_kb * 128:] == -7.0)
+    with pytest.raises(X.XtrError):
+        X.probe_stream(src, dst, blocks, 48, write_kb)               # not a multiple of 32 KB
+    with pytest.raises(ValueError):
+        X.probe_stream(src, dst, blocks + 1, read_kb, write_kb)      # src too small
