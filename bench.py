@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--profile-krylov", action="store_true",
                     help="only run one single-right-hand-side FGMRES solve of the workload's first frequency on the "
                          "main thread (for traces and counters of the Krylov kernels)")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="only run one single-right-hand-side mixed-precision solve of the workload's first frequency "
+                         "on the main thread (for traces of the complex64 kernels)")
     ap.add_argument("--no-probe", action="store_true",
                     help="skip the HBM stream probe (read-only, the contraction's read/write mix, 1:1 copy)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -111,7 +114,7 @@ def main():
     from extreme_amd import cartesian_model as cm
 
     model = {"c5": cm.config_c5, "c2": cm.config_c2, "c1": cm.config_c1}[args.config]()
-    if args.profile_applies or args.profile_krylov:
+    if args.profile_applies or args.profile_krylov or args.profile_mixed:
         profile_run(model, args)
         return
     line = sweep(model, args)
@@ -354,13 +357,21 @@ def batched_uneven(nmodels, settings):
 
 def profile_run(model, args):
     """Kernel-trace workloads on one host thread: operator applies of both polarisations' chi0 (FP64,
-    then complex64 on complex64 vectors), or one FGMRES solve of one polarisation."""
+    then complex64 on complex64 vectors), one FGMRES solve of one polarisation, or one mixed-precision
+    solve of one polarisation."""
     import torch
     from extreme_amd import forward as F
     from extreme_amd import xtreme as X
     f = model.frequencies[0]
     p = F.FrequencySolve(model, f, precision="mixed")
     torch.cuda.synchronize()
+    if args.profile_mixed:
+        x = p.chi0[0].clone()
+        t0 = time.perf_counter()
+        info = p.solver.solve_refined_batch([p.chi0[0]], [x])[0]
+        log(f"profile: one mixed-precision solve at {f:g} Hz, {info.refinements} refinements, {info.matvecs32} "
+            f"complex64 and {info.matvecs64} FP64 applies, residual {info.residual:.2e}, {time.perf_counter() - t0:.3f} s")
+        return
     if args.profile_krylov:
         x = p.chi0[0].clone()
         t0 = time.perf_counter()

@@ -55,7 +55,27 @@ __global__ __launch_bounds__(kThreads) void block_dots_kernel(const VT *__restri
     double2 acc[kDotChunk];
 #pragma unroll
     for (int k = 0; k < kDotChunk; ++k) acc[k] = make_double2(0, 0);
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    // two elements per iteration, i and i + stride, all loads issued before the products (the
+    // accumulation order per thread is the plain grid-stride order)
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+        const VT w0 = w[i], w1 = w[i + stride];
+        VT v0[kDotChunk], v1[kDotChunk];
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < kc) {
+                v0[k] = V[(int64_t)(k0 + k) * n + i];
+                v1[k] = V[(int64_t)(k0 + k) * n + i + stride];
+            }
+#pragma unroll
+        for (int k = 0; k < kDotChunk; ++k)
+            if (k < kc) {
+                acc[k] = cfma_conj<double>(acc[k], ld(v0[k]), ld(w0));
+                acc[k] = cfma_conj<double>(acc[k], ld(v1[k]), ld(w1));
+            }
+    }
+    if (i < n) {
         const double2 wv = ld(w[i]);
 #pragma unroll
         for (int k = 0; k < kDotChunk; ++k)
@@ -123,7 +143,43 @@ __global__ __launch_bounds__(kThreads) void cgs_update_kernel(const VT *__restri
     if (threadIdx.x < cnt) hs[threadIdx.x] = h[threadIdx.x];
     __syncthreads();
     double nn = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    // two elements per iteration, i and i + stride, so that twice the loads are in flight; each
+    // element's sum over k and the norm's sum over elements keep the plain grid-stride order
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+        double2 a0 = make_double2(0, 0), a1 = a0;
+        int k = 0;
+        for (; k + kDotChunk <= cnt; k += kDotChunk) {          // 2 x 8 loads issued together
+            VT v0[kDotChunk], v1[kDotChunk];
+#pragma unroll
+            for (int j = 0; j < kDotChunk; ++j) {
+                v0[j] = V[(int64_t)(k + j) * n + i];
+                v1[j] = V[(int64_t)(k + j) * n + i + stride];
+            }
+#pragma unroll
+            for (int j = 0; j < kDotChunk; ++j) {
+                a0 = cfma<double>(a0, ld(v0[j]), hs[k + j]);
+                a1 = cfma<double>(a1, ld(v1[j]), hs[k + j]);
+            }
+        }
+        for (; k < cnt; ++k) {
+            const VT v0 = V[(int64_t)k * n + i], v1 = V[(int64_t)k * n + i + stride];
+            a0 = cfma<double>(a0, ld(v0), hs[k]);
+            a1 = cfma<double>(a1, ld(v1), hs[k]);
+        }
+        VT t0, t1;
+        st(t0, csub<double>(ld(w[i]), a0));
+        st(t1, csub<double>(ld(w[i + stride]), a1));
+        w[i] = t0;
+        w[i + stride] = t1;
+        const double2 n0 = ld(t0), n1 = ld(t1);
+        nn = fma(n0.x, n0.x, nn);
+        nn = fma(n0.y, n0.y, nn);
+        nn = fma(n1.x, n1.x, nn);
+        nn = fma(n1.y, n1.y, nn);
+    }
+    if (i < n) {
         double2 a = make_double2(0, 0);
         for (int k = 0; k < cnt; ++k) a = cfma<double>(a, ld(V[(int64_t)k * n + i]), hs[k]);
         VT t;
