@@ -1539,36 +1539,41 @@ __global__ __launch_bounds__(256, 2) void contract_fold32_kernel(const cplx<T> *
 }
 
 // ---- folded contraction on MFMA, nz = 64 -------------------------------------------------------------
-// A folded block (264 KB in FP64) is streamed through LDS in nine units: the spectrum columns of its
-// images, xx, xy, yy, zz, and the xz and yz blocks each as a diagonal and an off-diagonal pair of
-// 32 x 32 quadrants, through a ring of slots (FP64: 4 of 33 wave-loads, FP32: 8 of 17) with NSLOT-1
-// units in flight. Wave w = 4 g + t owns rows 16t..16t+15 of x, y and z and the k-half g (k in
-// 32g..32g+31 of every input component); waves t and 4 + t add their partial sums at the end of the
-// block. The spectrum unit is read once per block into registers: lane (kl, n) keeps B[k][n] of its
-// 24 k-steps, B = [Re U | Im U] with the image's mirror signs applied. Two accumulators per tile,
-// D1 = Re G . B and D2 = Im G . B, give Re Y = D1[:, :8] - D2[:, 8:] and Im Y = D1[:, 8:] + D2[:, :8]
-// through one lane exchange at the end; xy's A values serve both of its products. The -xz^T, -yz^T
-// blocks enter through the FP64 MFMA's negate modifier (FP32: negated values). A quadrant pair holds
-// the quadrants (row half h, column half h) (diagonal unit) or (h, 1 - h) (off-diagonal unit); the
-// direct and the transposed product of wave (t, g) both need the quadrant (t / 2, g) or (g, t / 2), so
-// each quadrant unit is contracted by the four waves with g == t / 2 (diagonal) or g != t / 2, one per
-// SIMD.
+// A folded block (264 KB in FP64, 132 KB in FP32) is streamed through LDS in units, through a ring of
+// four slots of 33 wave-loads (33 KB) with three units in flight. FP64: nine units (the spectrum
+// columns of the block's images, xx, xy, yy, zz, and the xz and yz blocks each as a diagonal and an
+// off-diagonal pair of 32 x 32 quadrants); FP32: five (the spectrum, xx + xy, yy + zz, xz, yz: each
+// a contiguous run of the block). Wave w = 4 g + t owns rows 16t..16t+15 of x, y and z and the k-half
+// g (k in 32g..32g+31 of every input component); waves t and 4 + t add their partial sums at the end
+// of the block. The spectrum unit is read once per block into registers: lane (kl, n) keeps B[k][n]
+// of its 24 k-steps, B = [Re U | Im U] with the image's mirror signs applied. Two accumulators per
+// tile, D1 = Re G . B and D2 = Im G . B, give Re Y = D1[:, :8] - D2[:, 8:] and Im Y = D1[:, 8:] +
+// D2[:, :8] through one lane exchange at the end; xy's A values serve both of its products. The
+// -xz^T, -yz^T blocks enter through the FP64 MFMA's negate modifier (FP32: negated values). In FP64 a
+// quadrant pair holds the quadrants (row half h, column half h) (diagonal unit) or (h, 1 - h); the
+// direct and the transposed product of wave (t, g) both need the quadrant (t / 2, g) or (g, t / 2),
+// so each quadrant unit is contracted by the four waves with g == t / 2 (diagonal) or g != t / 2, one
+// per SIMD. In FP32 every wave works on every unit.
 namespace fold64 {
 constexpr int NZ = 64, C = 3 * NZ, S = NZ * (NZ + 1) / 2, A = NZ * NZ, GB = 4 * S + 2 * A;
 constexpr int MASK = 15;                  // asym_swizzle_mask(64)
 constexpr int NCOL = 8, UCOLS = NCOL * C;
-constexpr int NUNIT = 9;                  // spectrum, xx, xy, yy, zz, xz diag, xz off, yz diag, yz off
 constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
 constexpr int KS = 8;                     // k-steps of one component's k-half (32 values, 4 per step)
-__host__ __device__ constexpr int uelems(int u) { return u == 0 ? UCOLS : u <= 4 ? S : A / 2; }
+constexpr int NSLOT = 4;
 template <typename T>
 struct Cfg {
+    static constexpr bool WIDE = sizeof(T) == 4;            // FP32: whole xz, yz blocks in one slot
+    static constexpr int NUNIT = WIDE ? 5 : 9;
     static constexpr int E = (int)sizeof(cplx<T>);
     static constexpr int EPL = 16 / E;
     static constexpr int EPC = 64 * EPL;
-    static constexpr int NSLOT = E == 16 ? 4 : 8;
+    __host__ __device__ static constexpr int uelems(int u)
+    {
+        return u == 0 ? UCOLS : WIDE ? (u <= 2 ? 2 * S : A) : (u <= 4 ? S : A / 2);
+    }
     __host__ __device__ static constexpr int nw(int u) { return (uelems(u) + EPC - 1) / EPC; }
-    static constexpr int SLOTW = (S + EPC - 1) / EPC;       // the largest unit, in wave-loads
+    static constexpr int SLOTW = 33;                        // the largest unit, in wave-loads
     static constexpr int SLOT = SLOTW * EPC;
     static constexpr int EXCH = 4 * 3 * 64;                 // exchange: [t][tile][lane] of 4 T
     // wave-loads of unit u issued by wave w
@@ -1582,13 +1587,18 @@ struct Cfg {
             if (u + k < NUNIT || has_next) n += cnt((u + k) % NUNIT, w);
         return n;
     }
+    __host__ __device__ static constexpr bool fits()
+    {
+        for (int u = 0; u < NUNIT; ++u)
+            if (nw(u) > SLOTW) return false;
+        return true;
+    }
 };
-static_assert(Cfg<double>::nw(0) <= Cfg<double>::SLOTW && Cfg<double>::nw(5) <= Cfg<double>::SLOTW, "slot (FP64)");
-static_assert(Cfg<float>::nw(0) <= Cfg<float>::SLOTW && Cfg<float>::nw(5) <= Cfg<float>::SLOTW, "slot (FP32)");
-static_assert(4 * Cfg<double>::SLOT * 16 + Cfg<double>::EXCH * 4 * 8 <= 160 * 1024, "LDS (FP64)");
-static_assert(8 * Cfg<float>::SLOT * 8 + Cfg<float>::EXCH * 4 * 4 <= 160 * 1024, "LDS (FP32)");
+static_assert(Cfg<double>::fits() && Cfg<float>::fits(), "every unit fits a slot");
+static_assert(NSLOT * Cfg<double>::SLOT * 16 + Cfg<double>::EXCH * 4 * 8 <= 160 * 1024, "LDS (FP64)");
+static_assert(NSLOT * Cfg<float>::SLOT * 8 + Cfg<float>::EXCH * 4 * 4 <= 160 * 1024, "LDS (FP32)");
 static_assert(Cfg<double>::ahead(0, true, 0) < 64 && Cfg<float>::ahead(0, true, 0) < 64, "vmcnt range");
-static_assert(Cfg<double>::NSLOT - 1 < NUNIT && Cfg<float>::NSLOT - 1 < NUNIT, "look-ahead within one block");
+static_assert(NSLOT - 1 < Cfg<double>::NUNIT && NSLOT - 1 < Cfg<float>::NUNIT, "look-ahead within one block");
 static_assert(S % 2 == 0 && C % 2 == 0, "two-element lanes stay inside one run");
 static_assert(Cfg<double>::nw(0) * Cfg<double>::EPC == UCOLS && Cfg<float>::nw(0) * Cfg<float>::EPC == UCOLS,
               "the spectrum unit has no padding lanes");
@@ -1630,6 +1640,11 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
             if (kk < 0) kk += C;                             // stay contiguous
             const int img = q / nrhs, h = q - img * nrhs;
             src = buf + h * M + im.l(img) * C + kk;
+        } else if constexpr (K::WIDE) {
+            // contiguous runs: xx + xy, yy + zz (2S elements), xz, yz (A)
+            constexpr int base = UT == 1 ? 0 : UT == 2 ? 2 * S : UT == 3 ? 4 * S : 4 * S + A;
+            constexpr int len = K::uelems(UT);
+            src = G + b * GB + base + (e < len ? e : len - K::EPL);   // padding lanes re-read
         } else if constexpr (UT <= 4) {
             src = G + b * GB + (UT - 1) * S + (e < S ? e : S - K::EPL);   // padding lanes re-read
         } else {
@@ -1648,37 +1663,63 @@ __device__ __forceinline__ void fold64_issue(const cplx<T> *__restrict__ G, cons
 #pragma clang diagnostic pop
 
 // Per-lane byte offsets of the A values of k-step s (k = 32 g + 4 s + kl, local to the component)
-// for row r = 16 t + i: packed symmetric unit (po), the quadrant of the direct product (qo: row r,
-// column k) and of the transposed one (to: row k, column r). They depend only on the lane and the
+// for row r = 16 t + i, within a slot: the packed symmetric layout (po), and the direct (row r,
+// column k: qo) and the transposed (row k, column r: to) element of an xz or yz block, as a
+// quadrant pair (FP64) or the whole swizzled block (FP32). They depend only on the lane and the
 // wave, so each wave computes them once.
 template <typename T>
 __device__ __forceinline__ void fold64_offsets(int t, int g, int lane, unsigned (&po)[fold64::KS],
                                                unsigned (&qo)[fold64::KS], unsigned (&to)[fold64::KS])
 {
+    using namespace fold64;
     constexpr unsigned E = sizeof(cplx<T>);
     const int r = 16 * t + (lane & 15), kl = lane >> 4;
 #pragma unroll
-    for (int s = 0; s < fold64::KS; ++s) {
+    for (int s = 0; s < KS; ++s) {
         const int c = 32 * g + 4 * s + kl, cq = 4 * s + kl;  // cq = c & 31
         po[s] = E * (unsigned)(c >= r ? r + c * (c + 1) / 2 : c + r * (r + 1) / 2);
-        qo[s] = E * (unsigned)((r >> 5) * 1024 + (r & 31) * 32 + (cq ^ (r & fold64::MASK)));
-        to[s] = E * (unsigned)(g * 1024 + cq * 32 + ((r & 31) ^ (cq & fold64::MASK)));
+        if constexpr (Cfg<T>::WIDE) {
+            qo[s] = E * (unsigned)(r * NZ + (c ^ (r & MASK)));
+            to[s] = E * (unsigned)(c * NZ + (r ^ (c & MASK)));
+        } else {
+            qo[s] = E * (unsigned)((r >> 5) * 1024 + (r & 31) * 32 + (cq ^ (r & MASK)));
+            to[s] = E * (unsigned)(g * 1024 + cq * 32 + ((r & 31) ^ (cq & MASK)));
+        }
     }
 }
 
-// D1 += (NEG ? -Re a : Re a) b, D2 += (NEG ? -Im a : Im a) b over the 8 k-steps of one A tile.
-template <typename T, bool NEG>
+// D1 += (NEG ? -Re a : Re a) b, D2 += (NEG ? -Im a : Im a) b over the 8 k-steps of one A tile at byte
+// offset OFF + off[s] of the slot.
+template <typename T, bool NEG, unsigned OFF = 0>
 __device__ __forceinline__ void fold64_job(const cplx<T> *__restrict__ blk, const unsigned (&off)[fold64::KS],
                                            const T (&b)[fold64::KS], typename MfmaAcc<T>::type &d1,
                                            typename MfmaAcc<T>::type &d2)
 {
     cplx<T> a[fold64::KS];
 #pragma unroll
-    for (int s = 0; s < fold64::KS; ++s) a[s] = lds_cplx<T>(blk, off[s]);
+    for (int s = 0; s < fold64::KS; ++s) a[s] = lds_cplx<T>(blk, OFF + off[s]);
 #pragma unroll
     for (int s = 0; s < fold64::KS; ++s) {
         d1 = mfma16s<NEG>(a[s].x, b[s], d1);
         d2 = mfma16s<NEG>(a[s].y, b[s], d2);
+    }
+}
+
+// xy . u_y -> y_x and xy . u_x -> y_y from the same A values (xy at byte offset OFF of the slot).
+template <typename T, unsigned OFF>
+__device__ __forceinline__ void fold64_xy(const cplx<T> *__restrict__ blk, const unsigned (&po)[fold64::KS],
+                                          const T (&bf)[3][fold64::KS], typename MfmaAcc<T>::type (&d1)[3],
+                                          typename MfmaAcc<T>::type (&d2)[3])
+{
+    cplx<T> a[fold64::KS];
+#pragma unroll
+    for (int s = 0; s < fold64::KS; ++s) a[s] = lds_cplx<T>(blk, OFF + po[s]);
+#pragma unroll
+    for (int s = 0; s < fold64::KS; ++s) {
+        d1[0] = mfma16s<false>(a[s].x, bf[1][s], d1[0]);
+        d2[0] = mfma16s<false>(a[s].y, bf[1][s], d2[0]);
+        d1[1] = mfma16s<false>(a[s].x, bf[0][s], d1[1]);
+        d2[1] = mfma16s<false>(a[s].y, bf[0][s], d2[1]);
     }
 }
 
@@ -1690,29 +1731,33 @@ __device__ __forceinline__ void fold64_compute(const cplx<T> *__restrict__ blk, 
                                                typename MfmaAcc<T>::type (&d1)[3], typename MfmaAcc<T>::type (&d2)[3])
 {
     using namespace fold64;
-    if constexpr (UT == 1) {
-        fold64_job<T, false>(blk, po, bf[0], d1[0], d2[0]);          // xx . u_x -> y_x
-    } else if constexpr (UT == 2) {
-        cplx<T> a[KS];                                               // xy . u_y -> y_x, xy . u_x -> y_y
-#pragma unroll
-        for (int s = 0; s < KS; ++s) a[s] = lds_cplx<T>(blk, po[s]);
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            d1[0] = mfma16s<false>(a[s].x, bf[1][s], d1[0]);
-            d2[0] = mfma16s<false>(a[s].y, bf[1][s], d2[0]);
-            d1[1] = mfma16s<false>(a[s].x, bf[0][s], d1[1]);
-            d2[1] = mfma16s<false>(a[s].y, bf[0][s], d2[1]);
-        }
-    } else if constexpr (UT == 3) {
-        fold64_job<T, false>(blk, po, bf[1], d1[1], d2[1]);          // yy . u_y -> y_y
-    } else if constexpr (UT == 4) {
-        fold64_job<T, false>(blk, po, bf[2], d1[2], d2[2]);          // zz . u_z -> y_z
-    } else {
-        constexpr int o = UT <= 6 ? 0 : 1;                           // xz (x) or yz (y)
-        constexpr bool diag = UT == 5 || UT == 7;
-        if ((g == (t >> 1)) == diag) {
+    constexpr unsigned SB = (unsigned)S * sizeof(cplx<T>);     // bytes of one packed block
+    if constexpr (Cfg<T>::WIDE) {
+        if constexpr (UT == 1) {
+            fold64_job<T, false>(blk, po, bf[0], d1[0], d2[0]);      // xx . u_x -> y_x
+            fold64_xy<T, SB>(blk, po, bf, d1, d2);
+        } else if constexpr (UT == 2) {
+            fold64_job<T, false>(blk, po, bf[1], d1[1], d2[1]);      // yy . u_y -> y_y
+            fold64_job<T, false, SB>(blk, po, bf[2], d1[2], d2[2]);  // zz . u_z -> y_z
+        } else {
+            constexpr int o = UT == 3 ? 0 : 1;                       // xz (x) or yz (y)
             fold64_job<T, false>(blk, qo, bf[2], d1[o], d2[o]);      // xz . u_z -> y_x
             fold64_job<T, true>(blk, to, bf[o], d1[2], d2[2]);       // -xz^T . u_x -> y_z
+        }
+    } else if constexpr (UT == 1) {
+        fold64_job<T, false>(blk, po, bf[0], d1[0], d2[0]);
+    } else if constexpr (UT == 2) {
+        fold64_xy<T, 0>(blk, po, bf, d1, d2);
+    } else if constexpr (UT == 3) {
+        fold64_job<T, false>(blk, po, bf[1], d1[1], d2[1]);
+    } else if constexpr (UT == 4) {
+        fold64_job<T, false>(blk, po, bf[2], d1[2], d2[2]);
+    } else {
+        constexpr int o = UT <= 6 ? 0 : 1;
+        constexpr bool diag = UT == 5 || UT == 7;
+        if ((g == (t >> 1)) == diag) {
+            fold64_job<T, false>(blk, qo, bf[2], d1[o], d2[o]);
+            fold64_job<T, true>(blk, to, bf[o], d1[2], d2[2]);
         }
     }
 }
@@ -1733,11 +1778,11 @@ __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<
     if (has_next) fold64_wait_wave<T, U, true>(wave);
     else fold64_wait_wave<T, U, false>(wave);
     XTR_BARRIER();                                         // unit U in LDS; previous slot's reads done
-    constexpr int V = U + K::NSLOT - 1;                    // unit copied now, into the previous step's slot
-    const unsigned vdst = base + (unsigned)(((s + K::NSLOT - 1) % K::NSLOT) * K::SLOT * K::E);
-    if constexpr (V < NUNIT) fold64_issue<T, V>(G, buf, M, b, im, nrhs, vdst, wave, lane);
-    else if (has_next) fold64_issue<T, V - NUNIT>(G, buf, M, bn, imn, nrhs, vdst, wave, lane);
-    const cplx<T> *blk = ring + (s % K::NSLOT) * K::SLOT;
+    constexpr int V = U + NSLOT - 1;                       // unit copied now, into the previous step's slot
+    const unsigned vdst = base + (unsigned)(((s + NSLOT - 1) % NSLOT) * K::SLOT * K::E);
+    if constexpr (V < K::NUNIT) fold64_issue<T, V>(G, buf, M, b, im, nrhs, vdst, wave, lane);
+    else if (has_next) fold64_issue<T, V - K::NUNIT>(G, buf, M, bn, imn, nrhs, vdst, wave, lane);
+    const cplx<T> *blk = ring + (s % NSLOT) * K::SLOT;
     const int t = wave & 3, g = wave >> 2;
     if constexpr (U == 0) {
         // lane (kl, n): B[k][n] for k = 64 ci + 32 g + 4 s + kl, column q = n % 8, Re (n < 8) or Im
@@ -1760,7 +1805,7 @@ __device__ __forceinline__ void fold64_step(const cplx<T> *__restrict__ G, cplx<
     }
 }
 
-// Tile O of rows 16t.. from D1 and D2 (the Re/Im exchange between lanes n and n ^ 8).
+// Tile of rows 16t.. from D1 and D2 (the Re/Im exchange between lanes n and n ^ 8).
 template <typename T>
 __device__ __forceinline__ typename MfmaAcc<T>::type fold64_combine(typename MfmaAcc<T>::type d1,
                                                                    typename MfmaAcc<T>::type d2, int lane)
@@ -1793,6 +1838,20 @@ __device__ __forceinline__ void fold64_store(typename MfmaAcc<T>::type y, cplx<T
     }
 }
 
+// All the block's steps, 0 .. NUNIT - 1.
+template <typename T, int... Us>
+__device__ __forceinline__ void fold64_block(std::integer_sequence<int, Us...>, const cplx<T> *__restrict__ G,
+                                             cplx<T> *buf, int64_t M, int64_t b, int64_t bn, bool hn,
+                                             const FoldImages &im, const FoldImages &imn, int nrhs, int64_t s,
+                                             cplx<T> *ring, unsigned base, int wave, int lane,
+                                             T (&bf)[3][fold64::KS], const unsigned (&po)[fold64::KS],
+                                             const unsigned (&qo)[fold64::KS], const unsigned (&to)[fold64::KS],
+                                             typename MfmaAcc<T>::type (&d1)[3], typename MfmaAcc<T>::type (&d2)[3])
+{
+    (fold64_step<T, Us>(G, buf, M, b, bn, hn, im, imn, nrhs, s + Us, ring, base, wave, lane, bf, po, qo, to, d1, d2),
+     ...);
+}
+
 template <typename T>
 __global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf, int64_t Lq,
                                                               int64_t M, int nrhs, int nx, int ny)
@@ -1800,7 +1859,7 @@ __global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cpl
     using namespace fold64;
     using K = Cfg<T>;
     using acc_t = typename MfmaAcc<T>::type;
-    __shared__ cplx<T> ring[K::NSLOT * K::SLOT];
+    __shared__ cplx<T> ring[NSLOT * K::SLOT];
     __shared__ acc_t exch[K::EXCH];                              // [t][tile][lane]: waves 4..7 -> 0..3
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int t = wave & 3, g = wave >> 2;
@@ -1816,32 +1875,19 @@ __global__ __launch_bounds__(fold64::NTHR) void contract_fold64_kernel(const cpl
         fold64_issue<T, 0>(G, buf, M, b0, im0, nrhs, base, wave, lane);
         fold64_issue<T, 1>(G, buf, M, b0, im0, nrhs, base + 1u * K::SLOT * K::E, wave, lane);
         fold64_issue<T, 2>(G, buf, M, b0, im0, nrhs, base + 2u * K::SLOT * K::E, wave, lane);
-        if constexpr (K::NSLOT > 4) {
-            fold64_issue<T, 3>(G, buf, M, b0, im0, nrhs, base + 3u * K::SLOT * K::E, wave, lane);
-            fold64_issue<T, 4>(G, buf, M, b0, im0, nrhs, base + 4u * K::SLOT * K::E, wave, lane);
-            fold64_issue<T, 5>(G, buf, M, b0, im0, nrhs, base + 5u * K::SLOT * K::E, wave, lane);
-            fold64_issue<T, 6>(G, buf, M, b0, im0, nrhs, base + 6u * K::SLOT * K::E, wave, lane);
-        }
     }
+    static_assert(NSLOT == 4, "the prologue issues units 0 .. 2");
     T bf[3][KS];
     for (int64_t i = 0; i < cnt; ++i) {
         const int64_t b = b0 + i * gs, bn = b + gs;
         const bool hn = i + 1 < cnt;
         const FoldImages im = fold_images(b, nx, ny);
         const FoldImages imn = fold_images(hn ? bn : b, nx, ny);
-        const int64_t s = i * NUNIT;
         acc_t d1[3], d2[3];
 #pragma unroll
         for (int o = 0; o < 3; ++o) d1[o] = d2[o] = acc_t{0, 0, 0, 0};
-        fold64_step<T, 0>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 0, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 1>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 1, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 2>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 2, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 3>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 3, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 4>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 4, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 5>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 5, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 6>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 6, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 7>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 7, ring, base, wave, lane, bf, po, qo, to, d1, d2);
-        fold64_step<T, 8>(G, buf, M, b, bn, hn, im, imn, nrhs, s + 8, ring, base, wave, lane, bf, po, qo, to, d1, d2);
+        fold64_block<T>(std::make_integer_sequence<int, K::NUNIT>{}, G, buf, M, b, bn, hn, im, imn, nrhs,
+                        i * K::NUNIT, ring, base, wave, lane, bf, po, qo, to, d1, d2);
         acc_t y[3];
 #pragma unroll
         for (int o = 0; o < 3; ++o) y[o] = fold64_combine<T>(d1[o], d2[o], lane);
