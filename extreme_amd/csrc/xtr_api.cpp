@@ -196,6 +196,7 @@ struct xtr_op {
     bool profiling = false;
     std::array<int, 4> last_tiles{{-1, -1, -1, -1}};   // FFT column tiles of the latest apply
     std::vector<std::array<hipEvent_t, 6>> ev_sets;
+    std::vector<int> ev_nrhs;                        // right-hand sides of the apply each set timed
     size_t ev_used = 0;
     std::mutex apply_mu;
 
@@ -208,13 +209,15 @@ struct xtr_op {
 
     ~xtr_op();
 
-    hipEvent_t *next_events()
+    hipEvent_t *next_events(int nrhs)
     {
         if (ev_used == ev_sets.size()) {
             std::array<hipEvent_t, 6> set{};
             for (auto &e : set) XTR_CHECK(hipEventCreate(&e));
             ev_sets.push_back(set);
+            ev_nrhs.push_back(0);
         }
+        ev_nrhs[ev_used] = nrhs;
         return ev_sets[ev_used++].data();
     }
 
@@ -337,7 +340,7 @@ struct xtr_op {
         a.v = v;
         last_tiles = fft_tiles(a, s);
         for (int k = 0; k < 4; ++k) a.fft_tile[k] = last_tiles[k];
-        hipEvent_t *ev = profiling ? next_events() : nullptr;
+        hipEvent_t *ev = profiling ? next_events(nrhs) : nullptr;
         if (ev) XTR_CHECK(hipEventRecord(ev[0], s));
         xtr::launch_fwd_y(a, s);
         if (ev) XTR_CHECK(hipEventRecord(ev[1], s));
@@ -1368,11 +1371,20 @@ int xtr_op_set_profiling(xtr_op *op, int enable)
 
 int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies)
 {
+    return xtr_op_stage_times_rhs(op, 0, out5, applies);
+}
+
+int xtr_op_stage_times_rhs(xtr_op *op, int nrhs, double *out5, int64_t *applies)
+{
     return guarded([&] {
         require(op && out5, "null argument");
+        require(nrhs >= 0, "nrhs must be >= 0");
         std::lock_guard<std::mutex> lk(op->apply_mu);
         for (int i = 0; i < 5; ++i) out5[i] = 0;
+        int64_t counted = 0;
         for (size_t k = 0; k < op->ev_used; ++k) {
+            if (nrhs > 0 && op->ev_nrhs[k] != nrhs) continue;
+            ++counted;
             auto &ev = op->ev_sets[k];
             XTR_CHECK(hipEventSynchronize(ev[5]));
             for (int i = 0; i < 5; ++i) {
@@ -1381,7 +1393,7 @@ int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies)
                 out5[i] += ms;
             }
         }
-        if (applies) *applies = (int64_t)op->ev_used;
+        if (applies) *applies = counted;
     });
 }
 

@@ -107,6 +107,7 @@ def lib():
         L.xtr_op_level_field_noalloc_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.xtr_op_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.xtr_op_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.xtr_op_stage_times_rhs.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.xtr_op_fft_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.xtr_op_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.xtr_op_green_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int64)]
@@ -440,11 +441,12 @@ class ConvolutionOperator:
 
     STAGES = ("fwd_y", "fwd_x", "contract", "inv_x", "inv_y")
 
-    def stage_times(self):
-        """(summed milliseconds per stage, number of applies) since profiling was switched on."""
+    def stage_times(self, nrhs: int = 0):
+        """(summed milliseconds per stage, number of applies) since profiling was switched on, over
+        the applies of nrhs right-hand sides (0: all)."""
         out = (C.c_double * 5)()
         n = C.c_int64()
-        _check(lib().xtr_op_stage_times(self._h, out, C.byref(n)))
+        _check(lib().xtr_op_stage_times_rhs(self._h, int(nrhs), out, C.byref(n)))
         return dict(zip(self.STAGES, list(out))), n.value
 
     FFT_TILES = ("default", "narrow", "wide", "narrow512", "reg", "reg_alt")

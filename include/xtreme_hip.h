@@ -212,6 +212,10 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
  * the kernels run on (the reference's OperatorA* profiler events, ProfilerEvent.cs:40-49). */
 int xtr_op_set_profiling(xtr_op *op, int enable);
 int xtr_op_stage_times(xtr_op *op, double *out5, int64_t *applies);
+/* The same summed over the applies of exactly nrhs right-hand sides (nrhs = 0: all applies, as
+ * xtr_op_stage_times). No counterpart in the reference, whose profiler does not split by right-hand
+ * sides; a launch's bytes depend on them. */
+int xtr_op_stage_times_rhs(xtr_op *op, int nrhs, double *out5, int64_t *applies);
 
 /* No counterpart in the reference (its FFTs are FFTW plans): the column tile each of the four FFT
  * passes (y forward, x forward, x inverse, y inverse) used in the latest apply, 0 = default,
