@@ -179,12 +179,17 @@ class FrequencySolve:
         self.green_seconds = time.perf_counter() - t0
         self.atoo_green_seconds = 0.0
         self.atoo_field_seconds = [0.0, 0.0]
+        t0 = time.perf_counter()
         e0 = normal_field(model, self.omega)
         self.js = [scattered_current(model, self.omega, pol, e0) for pol in (0, 1)]
+        self.js_seconds = time.perf_counter() - t0               # CalcJScattered, both polarisations
+        t0 = time.perf_counter()
         self.op.prepare_operator("Chi0")
         js_dev = torch.from_numpy(np.stack(self.js)).to("cuda")
         self.chi0 = self.op.apply(js_dev)                       # CalculateChi0From, both pols in one apply
         self.op.prepare_operator("A")
+        torch.cuda.synchronize()
+        self.chi0_seconds = time.perf_counter() - t0             # CalcChi0
         self.solver = X.AnomalyCurrentFgmresSolver(self.op, self.settings)
         self.chi = self.chi0.clone()
         self.level_z = None

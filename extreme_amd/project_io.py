@@ -441,13 +441,102 @@ def append_common_dat(path: str, line: str) -> None:
         f.write(line)
 
 
+# Events of the per-frequency profiling report, in the order and nesting of
+# ProfilerResultsTextExporter.SaveProfilingResultsTo (EndUser/Profiling/ProfilerResultsTextExporter.cs:37-55,
+# 160-238): top-level events with their sub-events; an event without a measurement is left out, as the
+# exporter's WriteIfExist leaves out events the profiler never recorded.
+PROFILING_SECTIONS = (
+    ("CustomFft", ("CustomFftInitialTranspose", "CustomFftFourierY", "CustomFftBlockTransposeYtoX",
+                   "CustomFftDistributedTranspose", "CustomFftFourierX", "CustomFftBlockTransposeXtoY",
+                   "CustomFftFinalTranspose")),
+    ("GreenAtoATotal", (("GreenScalarAtoA", ("GreenScalarAtoACalcCalc", "GreenScalarAtoAUnion",
+                                             "GreenScalarAtoASegments",
+                                             ("GreenScalarAtoACommunicate", ("GreenScalarAtoATrans",)))),
+                        ("GreenTensorAtoA", ("GreenTensorAtoACalc", "GreenTensorAtoAFft")))),
+    ("SolveCie", ("CalcDotProduct",
+                  ("ApplyOperatorA", ("OperatorAApplyR", "OperatorAPrepareForForwardFft", "OperatorAForwardFft",
+                                      "OperatorAMultiplication", "OperatorABackwardFft",
+                                      "OperatorAExtractAfterBackwardFft", "OperatorAFinish")))),
+    ("ObservationsFullCalculation", ()),
+    ("CalcChi0", ()),
+    ("CalcJScattered", ()),
+    ("CalcEScattered", ()),
+    ("FftwPlanCalculation", ()),
+)
+
+
+def profiling_file_name(frequency: float, nmpi: int = 1, nthreads: int = 1) -> str:
+    """profiling_mpi{nMpi:0000}_threads{nThreads}_freq{frequency:####0.0000}.txt
+    (EndUser/ExtremeMt/ExtremeMtSolverRunner.cs:118,135)."""
+    return f"profiling_mpi{nmpi:04d}_threads{nthreads}_freq{frequency:.4f}.txt"
+
+
+def profiling_report(model: cm.CartesianModel, stats: dict, nmpi: int = 1, nthreads: int = 1, when=None) -> str:
+    """Text of the per-frequency profiling file, laid out as ProfilerResultsTextExporter.
+    SaveProfilingResultsTo writes it (ProfilerResultsTextExporter.cs:37-55, 57-83, 125-151, 213-264):
+    the model block, the counts of operator multiplications and dot products, the total SOLVING time,
+    then each top-level event with its time, count and percent of ForwardSolving and its sub-events'
+    time, percent of their parent and count, with "Covered" sums, and the total covered percent.
+    stats maps an event name to (seconds, count); ForwardSolving must be present."""
+    import datetime
+    now = when or datetime.datetime.now()
+    total = stats["ForwardSolving"][0]
+    out = []
+    w = out.append
+    w(f"DateTime: {now.strftime('%m/%d/%Y %H:%M:%S')}\n")
+    w("model parameters:\n")
+    w(f"nx = {model.nx}, ny = {model.ny}, nz = {model.nz}\n")
+    w(f"Number of MPI processes = {nmpi}\n")
+    w(f"Number of threads = {nthreads}\n")
+    w("\n")
+    if "OperatorAMultiplication" in stats:
+        w(f"Total number of multiplications: {stats['OperatorAMultiplication'][1]}\n")
+    if "CalcDotProduct" in stats:
+        w(f"Total number of dot products: {stats['CalcDotProduct'][1]}\n")
+    w("\n")
+    w(f"Total SOLVING time {total:.5f}, {stats['ForwardSolving'][1]} times\n")
+
+    def sub(parent, node, depth):
+        name, children = (node, ()) if isinstance(node, str) else node
+        if name not in stats:
+            return 0.0
+        t, n = stats[name]
+        pct = 100.0 * t / stats[parent][0]
+        w("\t" * depth + f"{name.ljust(30)} {t:.5f}, {pct:.1f} %, {n} times\n")
+        if children:
+            w("\n")
+            covered = sum(sub(name, c, depth + 1) for c in children)
+            if covered != 0:
+                w("\t" * depth + f"\t\t\t\t\t\t\tCovered: {covered:.1f}%\n")
+            w("\n")
+        return pct
+
+    percent = 0.0
+    for top, children in PROFILING_SECTIONS:
+        if top not in stats:
+            continue
+        t, n = stats[top]
+        pct = 100.0 * t / total
+        w("\n\n")
+        w(f"{top}:\n")
+        w(f"\tTotal time: {t:.4f}, {n} times, \t\t\t\t\tpercent of SOLVING: {pct:.1f}%\n")
+        w("\n")
+        covered = sum(sub(top, c, 1) for c in children)
+        if covered != 0:
+            w(f"\t\t\t\t\t\t\tCovered: {covered:.1f}%\n")
+        percent += pct
+    w(f"\n\n                                               Total Covered: {percent:.1f} %\n")
+    return "".join(out)
+
+
 def run_project(path: str, precision: str = "fp64", results_path: str | None = None, log=print) -> list:
     """ExtremeMtSolverRunner.Run (EndUser/ExtremeMt/ExtremeMtSolverRunner.cs:47-76) on the GPU:
     for each frequency, solve both polarisations, compute every observation level's fields and
-    write fields_freq* and responses_freq* into the results path, and append the frequency's
-    timings to common.dat in the results path's parent (ExportProfiling, :114-141; one process and
-    one host thread driving the GPU). Observation sites (single points off the level grid) are not
-    built here."""
+    write fields_freq* and responses_freq* into the results path, the frequency's event breakdown
+    into profiling_mpi0001_threads1_freq*.txt there, and append its timings to common.dat in the
+    results path's parent (ExportProfiling, :114-141; one process and one host thread driving the
+    GPU). Returns the fields and responses files written. Observation sites (single points off the
+    level grid) are not built here."""
     from . import forward as F
     project = read_project(path)
     if project.model_file is None:
@@ -459,10 +548,15 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
     os.makedirs(out_dir, exist_ok=True)
     written = []
     for freq in project.frequencies:
+        t_freq = time.perf_counter()
         fs = F.FrequencySolve(model, freq, project.settings, precision=precision)
+        fs.op.set_profiling(True)
         t0 = time.perf_counter()
         infos = fs.solve() if precision != "mixed" else fs.solve_mixed()
         solve_s = time.perf_counter() - t0
+        stage_ms, applies = fs.op.stage_times()
+        fs.op.set_profiling(False)
+        t0 = time.perf_counter()
         log(f"frequency {freq}: {[getattr(i, 'number_of_iterations', getattr(i, 'refinements', None)) for i in infos]}")
         levels = []
         for lev in project.levels:
@@ -471,6 +565,7 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
             ne = tuple(np.broadcast_to(v, shape) for v in ob["E_normal"])
             nh = tuple(np.broadcast_to(v, shape) for v in ob["H_normal"])
             levels.append(LevelFields(lev, ne, nh, tuple(ob["E_anomalous"]), tuple(ob["H_anomalous"])))
+        observe_s = time.perf_counter() - t0
         fpath = os.path.join(out_dir, fields_file_name(freq))
         rpath = os.path.join(out_dir, responses_file_name(freq))
         export_raw_fields(fpath, model, levels, freq)
@@ -479,6 +574,23 @@ def run_project(path: str, precision: str = "fp64", results_path: str | None = N
         # SolveCie per polarisation: each FP64 solve's own time (the two run concurrently); the
         # mixed-precision pair is timed together, so both columns carry that time
         pol_s = [i.seconds for i in infos] if precision != "mixed" else [solve_s, solve_s]
+        # the frequency's event breakdown: operator stages from HIP events on the operator's stream
+        # (device time of every apply of the solve, both polarisations together; R, the padding, the
+        # crop and the last step are fused into the FFT passes, so they carry no separate events)
+        st = {k: v * 1e-3 for k, v in stage_ms.items()}
+        stats = {"ForwardSolving": (time.perf_counter() - t_freq, 1),
+                 "GreenAtoATotal": (fs.green_seconds, 1),
+                 "SolveCie": (solve_s, 2),          # one span for the two concurrent solves
+                 "ApplyOperatorA": (sum(st.values()), applies),
+                 "OperatorAForwardFft": (st["fwd_y"] + st["fwd_x"], applies),
+                 "OperatorAMultiplication": (st["contract"], applies),
+                 "OperatorABackwardFft": (st["inv_x"] + st["inv_y"], applies),
+                 "ObservationsFullCalculation": (observe_s, 1),
+                 "CalcChi0": (fs.chi0_seconds, 1),
+                 "CalcJScattered": (fs.js_seconds, 1)}
+        ppath = os.path.join(out_dir, profiling_file_name(freq))
+        with open(ppath, "w", newline="") as f:
+            f.write(profiling_report(model, stats))
         append_common_dat(os.path.join(os.path.dirname(os.path.abspath(out_dir)), "common.dat"),
                           common_dat_line(model, project.settings.number_of_hankels, fs.green_seconds, pol_s,
                                           fs.atoo_green_seconds, fs.atoo_field_seconds))

@@ -68,6 +68,19 @@ def test_commemi_end_to_end(torch_cuda, tmp_path):
     times = [float(c) for c in cols[8:]]
     print(f"common.dat: {rows[1]}", flush=True)
     assert len(times) == 6 and all(t > 0 for t in times)
+    # the frequency's event breakdown beside the result tables (ExtremeMtSolverRunner.cs:135-137)
+    prof = (tmp_path / "results" / "profiling_mpi0001_threads1_freq0.1000.txt").read_text()
+    print(prof, flush=True)
+    lines = prof.splitlines()
+    assert lines[2] == "nx = 40, ny = 40, nz = 40"
+    mult = int(next(l for l in lines if l.startswith("Total number of multiplications:")).split(":")[1])
+    assert mult > 0
+    for ev in ("GreenAtoATotal:", "SolveCie:", "ObservationsFullCalculation:", "CalcChi0:"):
+        assert ev in lines
+    stage = [l.split() for l in lines if l.strip().startswith("OperatorA")]
+    assert [s[0] for s in stage] == ["OperatorAForwardFft", "OperatorAMultiplication", "OperatorABackwardFft"]
+    assert all(int(s[4]) == mult for s in stage)
+    assert "Total Covered:" in lines[-1]
 
 
 def test_commemi_reference_project(torch_cuda, tmp_path):

@@ -167,3 +167,60 @@ def test_common_dat_format(tmp_path):
     assert text[0] == ("Date           Time        nx    ny    nz    nhank mpi_size nthreads AtoA_Green     polX"
                        "           polY           AtoO_Green     AtoO_calc_X    AtoO_calc_Y    ")
     assert text[1] == text[2] == line[:-1] and text[3] == ""
+
+
+def test_profiling_report_format():
+    """The per-frequency profiling file as ProfilerResultsTextExporter.SaveProfilingResultsTo writes it
+    (EndUser/Profiling/ProfilerResultsTextExporter.cs:37-83, 125-151, 213-264): model block, counts,
+    total SOLVING time, each recorded top-level event ("0.0000" seconds, count, percent of
+    ForwardSolving) with its recorded sub-events (name padded to 30, "0.00000" seconds, percent of the
+    parent, count) and "Covered" sums; events that were not recorded are left out. The expected text
+    is written out from those format strings."""
+    import datetime
+    m = cm.small_model(4, 6, 2)
+    stats = {"ForwardSolving": (10.0, 1), "GreenAtoATotal": (2.0, 1), "SolveCie": (6.0, 2),
+             "ApplyOperatorA": (4.5, 300), "OperatorAForwardFft": (1.0, 300), "OperatorAMultiplication": (3.0, 300),
+             "OperatorABackwardFft": (0.5, 300), "ObservationsFullCalculation": (1.0, 1), "CalcChi0": (0.25, 1),
+             "CalcJScattered": (0.13, 1)}
+    got = P.profiling_report(m, stats, when=datetime.datetime(2026, 12, 2, 14, 5, 9))
+    want = ("DateTime: 12/02/2026 14:05:09\n"
+            "model parameters:\n"
+            "nx = 4, ny = 6, nz = 2\n"
+            "Number of MPI processes = 1\n"
+            "Number of threads = 1\n"
+            "\n"
+            "Total number of multiplications: 300\n"
+            "\n"
+            "Total SOLVING time 10.00000, 1 times\n"
+            "\n\n"
+            "GreenAtoATotal:\n"
+            "\tTotal time: 2.0000, 1 times, \t\t\t\t\tpercent of SOLVING: 20.0%\n"
+            "\n"
+            "\n\n"
+            "SolveCie:\n"
+            "\tTotal time: 6.0000, 2 times, \t\t\t\t\tpercent of SOLVING: 60.0%\n"
+            "\n"
+            "\tApplyOperatorA                 4.50000, 75.0 %, 300 times\n"
+            "\n"
+            "\t\tOperatorAForwardFft            1.00000, 22.2 %, 300 times\n"
+            "\t\tOperatorAMultiplication        3.00000, 66.7 %, 300 times\n"
+            "\t\tOperatorABackwardFft           0.50000, 11.1 %, 300 times\n"
+            "\t\t\t\t\t\t\t\tCovered: 100.0%\n"
+            "\n"
+            "\t\t\t\t\t\t\tCovered: 75.0%\n"
+            "\n\n"
+            "ObservationsFullCalculation:\n"
+            "\tTotal time: 1.0000, 1 times, \t\t\t\t\tpercent of SOLVING: 10.0%\n"
+            "\n"
+            "\n\n"
+            "CalcChi0:\n"
+            "\tTotal time: 0.2500, 1 times, \t\t\t\t\tpercent of SOLVING: 2.5%\n"
+            "\n"
+            "\n\n"
+            "CalcJScattered:\n"
+            "\tTotal time: 0.1300, 1 times, \t\t\t\t\tpercent of SOLVING: 1.3%\n"
+            "\n"
+            "\n\n                                               Total Covered: 93.8 %\n")
+    assert got == want
+    assert P.profiling_file_name(0.01) == "profiling_mpi0001_threads1_freq0.0100.txt"
+    assert P.profiling_file_name(100.0, nthreads=8) == "profiling_mpi0001_threads8_freq100.0000.txt"
