@@ -1946,6 +1946,152 @@ static bool contract_fold32(const ApplyArgs &a, hipStream_t s)
     return true;
 }
 
+// ---- folded contraction for small nz ---------------------------------------------------------------
+// ApplyGreenTensorAlongZ (ConvolutionOperator.cs:274-327) over a folded tensor for the depths that
+// have no kernel of their own (nz <= 31). A workgroup takes folded blocks b = blockIdx.x,
+// += gridDim.x; each block is read from HBM once, expanded into six full nz x ld matrices in LDS
+// (xx, xy, yy, zz, xz, yz; ld = nz rounded up to odd, so that rows and columns are both read without
+// bank conflicts), and applied to the spectrum columns of all its images (<= 4) and right-hand sides
+// (<= 8) at once, with the image's mirror signs on the x and y components of u and y. Thread
+// (group cg, row q = (a, i)) forms row q of y for columns cg, cg + ngroups, ...: J accumulators.
+// The mirror signs are formed as integer flags with bitwise operators and applied as a factor of
+// +-1: with hipcc (ROCm 7.2, gfx950) the short-circuit form
+//   if ((a == 0 && mx) || (a == 1 && my)) v = -v;
+// in this divergent code compiled to an empty conditional block for the second term, so lanes where
+// only a == 1 && my held stored v unnegated (the same source run on CPU threads is exact).
+namespace foldn {
+constexpr int NT = 256;
+constexpr int MAX_NZ = 31;
+__host__ __device__ inline int ld_of(int nz) { return nz | 1; }
+}  // namespace foldn
+
+template <typename T, int J>
+__global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
+                                                                  int64_t Lq, int64_t M, int nz, int nrhs, int nx,
+                                                                  int ny, const uint16_t *__restrict__ packed_rc)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ld = foldn::ld_of(nz), C = 3 * nz, S = nz * (nz + 1) / 2, A = nz * nz;
+    const int gblk = 4 * S + 2 * A;
+    const int msz = nz * ld;
+    cplx<T> *gm = reinterpret_cast<cplx<T> *>(smem_raw);   // [6][nz][ld]
+    cplx<T> *u = gm + 6 * msz;                             // [images * nrhs][C]
+    const int swz = asym_swizzle_mask(nz);
+    const int tid = threadIdx.x;
+    const int ngroups = foldn::NT / C;                     // the launch keeps C <= NT
+    const int q = tid % C, cg = tid / C;
+    const int a = q / nz, i = q % nz;
+    // row q of the 3x3 block operator, one segment per component of u:
+    //   y_x = XX u_x + XY u_y + XZ u_z; y_y = XY u_x + YY u_y + YZ u_z; y_z = -XZ^T u_x - YZ^T u_y + ZZ u_z
+    int mat[3];
+    bool trn[3];
+    mat[0] = a == 0 ? 0 : a == 1 ? 1 : 4;
+    mat[1] = a == 0 ? 1 : a == 1 ? 2 : 5;
+    mat[2] = a == 0 ? 4 : a == 1 ? 5 : 3;
+    trn[0] = a == 2;
+    trn[1] = a == 2;
+    trn[2] = false;
+    for (int64_t b = blockIdx.x; b < Lq; b += gridDim.x) {
+        const FoldImages im = fold_images(b, nx, ny);
+        const int ncol = im.n * nrhs;
+        __syncthreads();                                   // the previous block's LDS reads are done
+        const cplx<T> *gb = G + b * (int64_t)gblk;
+        for (int e = tid; e < gblk; e += foldn::NT) {
+            const cplx<T> g = gb[e];
+            if (e < 4 * S) {
+                const int bi = e / S, o = e - bi * S;
+                const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
+                cplx<T> *m = gm + bi * msz;
+                m[rc * ld + tr] = g;
+                m[tr * ld + rc] = g;
+            } else {
+                const int o = e - 4 * S, bi = 4 + o / A, pos = o % A;
+                const int rc = pos / nz, tr = (pos % nz) ^ (rc & swz);
+                gm[bi * msz + rc * ld + tr] = g;
+            }
+        }
+        // column j = image * nrhs + h
+        for (int e = tid; e < ncol * C; e += foldn::NT) {
+            const int j = e / C, k = e - j * C;
+            const int img = j / nrhs, h = j - img * nrhs;
+            const cplx<T> v = buf[h * M + im.l(img) * C + k];
+            const int flip = ((k < nz) & im.mx(img)) | ((k >= nz) & (k < 2 * nz) & im.my(img));
+            const T sg = flip ? T(-1) : T(1);
+            u[e] = mk<T>(sg * v.x, sg * v.y);
+        }
+        __syncthreads();
+        if (cg >= ngroups) continue;
+        cplx<T> acc[J];
+#pragma unroll
+        for (int jj = 0; jj < J; ++jj) acc[jj] = mk<T>(0, 0);
+#pragma unroll
+        for (int sgm = 0; sgm < 3; ++sgm) {
+            const cplx<T> *m = gm + mat[sgm] * msz;
+            const cplx<T> *row = trn[sgm] ? m + i : m + i * ld;
+            const int step = trn[sgm] ? ld : 1;
+            const cplx<T> *us = u + sgm * nz;
+            for (int c = 0; c < nz; ++c) {
+                cplx<T> g = row[c * step];
+                if (trn[sgm]) g = mk<T>(-g.x, -g.y);
+#pragma unroll
+                for (int jj = 0; jj < J; ++jj) {
+                    const int j = cg + jj * ngroups;
+                    if (j < ncol) acc[jj] = cfma<T>(acc[jj], g, us[j * C + c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < J; ++jj) {
+            const int j = cg + jj * ngroups;
+            if (j < ncol) {
+                const int img = j / nrhs, h = j - img * nrhs;
+                const int flip = ((a == 0) & im.mx(img)) | ((a == 1) & im.my(img));
+                const T sg = flip ? T(-1) : T(1);
+                buf[h * M + im.l(img) * C + q] = mk<T>(sg * acc[jj].x, sg * acc[jj].y);
+            }
+        }
+    }
+}
+
+static int device_max_lds()
+{
+    static int lds[64] = {0};
+    int dev = 0;
+    XTR_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw hip_error("device index out of range");
+    if (!lds[dev]) XTR_CHECK(hipDeviceGetAttribute(&lds[dev], hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return lds[dev];
+}
+
+template <typename T, int J>
+static void launch_foldn(const ApplyArgs &a, unsigned grid, size_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((contract_foldn_kernel<T, J>), dim3(grid), dim3(foldn::NT), lds, s, (const cplx<T> *)a.G,
+                       (cplx<T> *)a.buf, a.g.Lq(), a.g.M(), a.g.nz, a.nrhs, a.g.nx, a.g.ny, a.packed_rc);
+}
+
+// Folded tensor, nz <= 31, 1-8 right-hand sides in one launch (one Green-tensor read).
+template <typename T>
+static bool contract_foldn(const ApplyArgs &a, hipStream_t s)
+{
+    const int nz = a.g.nz;
+    if (!a.folded || nz < 1 || nz > foldn::MAX_NZ || a.nrhs < 1 || a.nrhs > kMaxRhs || contract_force_generic())
+        return false;
+    const int C = 3 * nz;
+    const size_t lds = sizeof(cplx<T>) * ((size_t)6 * nz * foldn::ld_of(nz) + (size_t)4 * a.nrhs * C);
+    if (lds > (size_t)device_max_lds()) return false;
+    const int ngroups = foldn::NT / C;
+    const int need = (4 * a.nrhs + ngroups - 1) / ngroups;    // columns per thread
+    const int per_cu = std::max(1, std::min(8, (int)(device_max_lds() / lds)));
+    const unsigned grid = (unsigned)std::min<int64_t>(a.g.Lq(), (int64_t)per_cu * device_cus());
+    if (need <= 2) launch_foldn<T, 2>(a, grid, lds, s);
+    else if (need <= 4) launch_foldn<T, 4>(a, grid, lds, s);
+    else if (need <= 8) launch_foldn<T, 8>(a, grid, lds, s);
+    else launch_foldn<T, 16>(a, grid, lds, s);
+    XTR_LAUNCH_CHECK("contract_foldn_kernel");
+    return true;
+}
+
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
@@ -1953,6 +2099,7 @@ static void contract(const ApplyArgs &a, hipStream_t s)
     if (!a.folded && contract_stream64(a, s)) return;
     if (contract_fold32(a, s)) return;
     if (contract_fold64(a, s)) return;
+    if (contract_foldn<T>(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;
     int W = per_w >= 256 ? 1 : 256 / per_w;

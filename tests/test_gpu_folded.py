@@ -16,19 +16,20 @@ pytestmark = pytest.mark.gpu
 TOL_FP64 = 1e-12
 
 
-def _op(oracle, m, monkeypatch=None, full=False, precision="fp64", G=None):
+def _op(oracle, m, monkeypatch=None, full=False, precision="fp64", G=None, max_rhs=2):
     w = cm.omega_of(m.frequencies[0])
     G = oracle.green_atoa(m, w) if G is None else G
     if full:
         monkeypatch.setenv("XTR_GREEN_LAYOUT", "full")
-    op = X.ConvolutionOperator(m, w, precision=precision, max_rhs=2)
+    op = X.ConvolutionOperator(m, w, precision=precision, max_rhs=max_rhs)
     op.set_green_tensor(G)
     if full:
         monkeypatch.delenv("XTR_GREEN_LAYOUT")
     return w, G, op
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 3), (6, 4, 5), (8, 12, 2), (4, 4, 32), (6, 6, 32), (4, 4, 64)])
+@pytest.mark.parametrize("shape", [(4, 4, 3), (6, 4, 5), (8, 12, 2), (6, 4, 8), (5, 6, 12), (4, 6, 16), (6, 4, 24),
+                                   (4, 4, 32), (6, 6, 32), (4, 4, 64)])
 def test_folded_apply_matches_oracle_and_full(oracle, torch_cuda, monkeypatch, shape):
     nx, ny, nz = shape
     m = cm.small_model(nx, ny, nz, seed=nx * 7 + ny + nz)
@@ -52,6 +53,46 @@ def test_folded_apply_matches_oracle_and_full(oracle, torch_cuda, monkeypatch, s
         assert rel(y2, ref) < TOL_FP64
         assert rel(y1, ref[1]) < TOL_FP64
         assert rel(y2, yf) < 1e-13
+
+
+@pytest.mark.parametrize("shape,nrhs", [((6, 4, 8), 1), ((5, 6, 12), 2), ((4, 6, 16), 4), ((6, 4, 24), 8),
+                                        ((3, 5, 31), 2), ((5, 3, 1), 3)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_folded_small_nz_kernel_matches_generic(oracle, torch_cuda, monkeypatch, shape, nrhs, precision):
+    """The folded contraction for nz <= 31 (one Green-block read for all images and right-hand sides)
+    against the generic per-wavenumber kernel and the oracle, 1-8 right-hand sides, complex128 and
+    complex64."""
+    torch = torch_cuda
+    nx, ny, nz = shape
+    m = cm.small_model(nx, ny, nz, seed=nz + 3 * nrhs)
+    w, G, op = _op(oracle, m, precision=precision, max_rhs=nrhs)
+    assert op.green_layout()[0] == "folded"
+    z, z0 = m.zeta(w).reshape(-1), m.zeta0(w)
+    oop = oracle.Operator(m, G)
+    rng = np.random.default_rng(nz + nrhs)
+    xs = rng.standard_normal((nrhs, m.n)) + 1j * rng.standard_normal((nrhs, m.n))
+    x32 = torch.from_numpy(xs).to("cuda").to(torch.complex64)
+    for kind in ("A", "Chi0"):
+        op.prepare_operator(kind, z)
+        oop.prepare(kind, z, z0)
+        ref = oop.apply(xs[-1])
+        if precision == "fp64":
+            y = op.apply(xs)
+            monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
+            yg = op.apply(xs)
+            monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+            tol_g, tol_o = 1e-13, 1e-12
+        else:
+            y = op.apply_c64(x32).cpu().numpy()
+            monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
+            yg = op.apply_c64(x32).cpu().numpy()
+            monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+            tol_g, tol_o = 2e-6, 2e-6
+        y, yg = y.reshape(nrhs, m.n), yg.reshape(nrhs, m.n)
+        print(f"small-nz folded {shape} nrhs {nrhs} {precision} {kind}: vs generic {rel(y, yg):.2e}, "
+              f"vs oracle {rel(y[-1], ref):.2e}", flush=True)
+        assert rel(y, yg) < tol_g
+        assert rel(y[-1], ref) < tol_o
 
 
 @pytest.mark.parametrize("shape", [(4, 4, 3), (6, 4, 5)])

@@ -21,7 +21,7 @@ def _group(nx, ny, nz, k, seed0=10):
     return out
 
 
-@pytest.mark.parametrize("shape,k", [((6, 4, 3), 2), ((12, 6, 32), 2), ((12, 6, 32), 3)])
+@pytest.mark.parametrize("shape,k", [((6, 4, 3), 2), ((6, 8, 12), 4), ((5, 4, 24), 4), ((12, 6, 32), 2), ((12, 6, 32), 3)])
 def test_models_apply_equals_single_model(oracle, torch_cuda, shape, k):
     torch = torch_cuda
     models = _group(*shape, k)
