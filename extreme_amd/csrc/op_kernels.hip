@@ -1954,25 +1954,35 @@ static bool contract_fold32(const ApplyArgs &a, hipStream_t s)
 // bank conflicts), and applied to the spectrum columns of all its images (<= 4) and right-hand sides
 // (<= 8) at once, with the image's mirror signs on the x and y components of u and y. Thread
 // (group cg, row q = (a, i)) forms row q of y for columns cg, cg + ngroups, ...: J accumulators.
+// Each thread holds E elements of the block in registers, and the next block's are loaded while this
+// one is contracted.
 // The mirror signs are formed as integer flags with bitwise operators and applied as a factor of
 // +-1: with hipcc (ROCm 7.2, gfx950) the short-circuit form
 //   if ((a == 0 && mx) || (a == 1 && my)) v = -v;
 // in this divergent code compiled to an empty conditional block for the second term, so lanes where
 // only a == 1 && my held stored v unnegated (the same source run on CPU threads is exact).
 namespace foldn {
+// f(integral_constant<int, t>) for t = 0 .. N-1: array indices known at compile time, so per-thread
+// arrays stay in registers
+template <typename F, int... Ts>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, Ts...>, F &&f)
+{
+    (f(std::integral_constant<int, Ts>{}), ...);
+}
 constexpr int NT = 256;
 constexpr int MAX_NZ = 31;
+static_assert(4 * (MAX_NZ * (MAX_NZ + 1) / 2) + 2 * MAX_NZ * MAX_NZ <= 16 * NT, "a block is at most 16 elements per thread");
 __host__ __device__ inline int ld_of(int nz) { return nz | 1; }
 }  // namespace foldn
 
-template <typename T, int J>
+template <typename T, int J, int E>
 __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
                                                                   int64_t Lq, int64_t M, int nz, int nrhs, int nx,
                                                                   int ny, const uint16_t *__restrict__ packed_rc)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ld = foldn::ld_of(nz), C = 3 * nz, S = nz * (nz + 1) / 2, A = nz * nz;
-    const int gblk = 4 * S + 2 * A;
+    const int gblk = 4 * S + 2 * A;                        // the launch keeps gblk <= E * NT
     const int msz = nz * ld;
     cplx<T> *gm = reinterpret_cast<cplx<T> *>(smem_raw);   // [6][nz][ld]
     cplx<T> *u = gm + 6 * msz;                             // [images * nrhs][C]
@@ -1991,35 +2001,64 @@ __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T>
     trn[0] = a == 2;
     trn[1] = a == 2;
     trn[2] = false;
+    // the block's elements tid + t NT, in registers: all E loads are in flight together, and the next
+    // block's are issued before this block's arithmetic
+    T gre[E], gim[E];                                      // (real arrays of complex ones end up in scratch)
+    auto load_g = [&](int64_t bb) {
+        const cplx<T> *gb = G + bb * (int64_t)gblk;
+        foldn::static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
+            const int e = tid + t * foldn::NT;
+            if (e < gblk) {
+                const cplx<T> g = gb[e];
+                gre[t] = g.x;
+                gim[t] = g.y;
+            }
+        });
+    };
+    if ((int64_t)blockIdx.x < Lq) load_g(blockIdx.x);
     for (int64_t b = blockIdx.x; b < Lq; b += gridDim.x) {
         const FoldImages im = fold_images(b, nx, ny);
         const int ncol = im.n * nrhs;
         __syncthreads();                                   // the previous block's LDS reads are done
-        const cplx<T> *gb = G + b * (int64_t)gblk;
-        for (int e = tid; e < gblk; e += foldn::NT) {
-            const cplx<T> g = gb[e];
+        foldn::static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
+            const int e = tid + t * foldn::NT;
+            const cplx<T> g = mk<T>(gre[t], gim[t]);
             if (e < 4 * S) {
                 const int bi = e / S, o = e - bi * S;
                 const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
                 cplx<T> *m = gm + bi * msz;
                 m[rc * ld + tr] = g;
                 m[tr * ld + rc] = g;
-            } else {
+            } else if (e < gblk) {
                 const int o = e - 4 * S, bi = 4 + o / A, pos = o % A;
                 const int rc = pos / nz, tr = (pos % nz) ^ (rc & swz);
                 gm[bi * msz + rc * ld + tr] = g;
             }
-        }
-        // column j = image * nrhs + h
-        for (int e = tid; e < ncol * C; e += foldn::NT) {
-            const int j = e / C, k = e - j * C;
-            const int img = j / nrhs, h = j - img * nrhs;
-            const cplx<T> v = buf[h * M + im.l(img) * C + k];
-            const int flip = ((k < nz) & im.mx(img)) | ((k >= nz) & (k < 2 * nz) & im.my(img));
-            const T sg = flip ? T(-1) : T(1);
-            u[e] = mk<T>(sg * v.x, sg * v.y);
+        });
+        // column j = image * nrhs + h; four loads in flight per thread and round
+        for (int e0 = tid; e0 < ncol * C; e0 += 4 * foldn::NT) {
+            T vre[4], vim[4], sg[4];
+            foldn::static_for(std::make_integer_sequence<int, 4>{}, [&](auto t) {
+                const int e = e0 + t * foldn::NT;
+                vre[t] = vim[t] = T(0);
+                sg[t] = T(1);
+                if (e < ncol * C) {
+                    const int j = e / C, k = e - j * C;
+                    const int img = j / nrhs, h = j - img * nrhs;
+                    const cplx<T> v = buf[h * M + im.l(img) * C + k];
+                    vre[t] = v.x;
+                    vim[t] = v.y;
+                    const int flip = ((k < nz) & im.mx(img)) | ((k >= nz) & (k < 2 * nz) & im.my(img));
+                    sg[t] = flip ? T(-1) : T(1);
+                }
+            });
+            foldn::static_for(std::make_integer_sequence<int, 4>{}, [&](auto t) {
+                const int e = e0 + t * foldn::NT;
+                if (e < ncol * C) u[e] = mk<T>(sg[t] * vre[t], sg[t] * vim[t]);
+            });
         }
         __syncthreads();
+        if (b + gridDim.x < Lq) load_g(b + gridDim.x);
         if (cg >= ngroups) continue;
         cplx<T> acc[J];
 #pragma unroll
@@ -2063,11 +2102,21 @@ static int device_max_lds()
     return lds[dev];
 }
 
-template <typename T, int J>
+template <typename T, int J, int E>
 static void launch_foldn(const ApplyArgs &a, unsigned grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((contract_foldn_kernel<T, J>), dim3(grid), dim3(foldn::NT), lds, s, (const cplx<T> *)a.G,
+    hipLaunchKernelGGL((contract_foldn_kernel<T, J, E>), dim3(grid), dim3(foldn::NT), lds, s, (const cplx<T> *)a.G,
                        (cplx<T> *)a.buf, a.g.Lq(), a.g.M(), a.g.nz, a.nrhs, a.g.nx, a.g.ny, a.packed_rc);
+}
+
+template <typename T, int J>
+static void launch_foldn_e(const ApplyArgs &a, unsigned grid, size_t lds, hipStream_t s)
+{
+    const int64_t per_thread = ceil_div(a.g.gblk(), foldn::NT);   // <= 16 for nz <= 31
+    if (per_thread <= 4) launch_foldn<T, J, 4>(a, grid, lds, s);
+    else if (per_thread <= 8) launch_foldn<T, J, 8>(a, grid, lds, s);
+    else if (per_thread <= 12) launch_foldn<T, J, 12>(a, grid, lds, s);
+    else launch_foldn<T, J, 16>(a, grid, lds, s);
 }
 
 // Folded tensor, nz <= 31, 1-8 right-hand sides in one launch (one Green-tensor read).
@@ -2084,10 +2133,10 @@ static bool contract_foldn(const ApplyArgs &a, hipStream_t s)
     const int need = (4 * a.nrhs + ngroups - 1) / ngroups;    // columns per thread
     const int per_cu = std::max(1, std::min(8, (int)(device_max_lds() / lds)));
     const unsigned grid = (unsigned)std::min<int64_t>(a.g.Lq(), (int64_t)per_cu * device_cus());
-    if (need <= 2) launch_foldn<T, 2>(a, grid, lds, s);
-    else if (need <= 4) launch_foldn<T, 4>(a, grid, lds, s);
-    else if (need <= 8) launch_foldn<T, 8>(a, grid, lds, s);
-    else launch_foldn<T, 16>(a, grid, lds, s);
+    if (need <= 2) launch_foldn_e<T, 2>(a, grid, lds, s);
+    else if (need <= 4) launch_foldn_e<T, 4>(a, grid, lds, s);
+    else if (need <= 8) launch_foldn_e<T, 8>(a, grid, lds, s);
+    else launch_foldn_e<T, 16>(a, grid, lds, s);
     XTR_LAUNCH_CHECK("contract_foldn_kernel");
     return true;
 }
