@@ -347,7 +347,8 @@ def batched_uneven(nmodels, settings):
     """The batched uneven-grid workload: seeded models of {32,48,64,96}^2 x {8,12,16,24} cells, each at
     two of {0.01, 0.1, 1} Hz, both polarisations. Solved one model and frequency at a time (the
     reference's order, on one handle at a time) and by the batch scheduler (groups of up to four models
-    sharing grid and frequency, four groups in flight). Green-tensor assembly is included in both."""
+    sharing grid and frequency, four groups in flight), in FP64 and in mixed precision. Green-tensor
+    assembly is included in all three."""
     import torch
     from extreme_amd import batch as B
     from extreme_amd import cartesian_model as cm
@@ -358,6 +359,10 @@ def batched_uneven(nmodels, settings):
     torch.cuda.synchronize()
     grp = B.run(models, settings, group_size=4, workers=4)
     torch.cuda.synchronize()
+    B.run(models[:2], settings, group_size=1, workers=1, precision="mixed")     # warm the complex64 paths
+    torch.cuda.synchronize()
+    mix = B.run(models, settings, group_size=4, workers=4, precision="mixed")
+    torch.cuda.synchronize()
     same = all(seq.iterations[k] == grp.iterations[k] for k in seq.iterations)
     log(f"batched uneven models: {seq.solves} solves, one at a time {seq.seconds:.2f} s, grouped {grp.seconds:.2f} s")
     return {"workload": f"C3: {nmodels} of 256 seeded models, (Nx, Ny) in {{32,48,64,96}}^2, Nz in {{8,12,16,24}}, "
@@ -366,7 +371,12 @@ def batched_uneven(nmodels, settings):
             "solves_per_s_one_at_a_time": seq.solves / seq.seconds, "solves_per_s_grouped": grp.solves / grp.seconds,
             "speedup": seq.seconds / grp.seconds, "groups": grp.groups, "group_size": 4, "workers": 4,
             "iterations_equal": same, "converged": seq.converged and grp.converged,
-            "true_residual_max": max(seq.max_true_residual, grp.max_true_residual)}
+            "true_residual_max": max(seq.max_true_residual, grp.max_true_residual),
+            "mixed_precision": {"grouped_s": mix.seconds, "solves_per_s": mix.solves / mix.seconds,
+                                "speedup_vs_fp64_grouped": grp.seconds / mix.seconds, "converged": mix.converged,
+                                "true_residual_max": mix.max_true_residual,
+                                "what": "the same groups solved by FP64 refinement around complex64 corrections "
+                                        "(complex64 tensor, FFTs, contraction and Krylov vectors)"}}
 
 
 def profile_run(model, args):

@@ -56,8 +56,10 @@ class BatchResult:
 
 
 def run(models, settings: X.ForwardSettings | None = None, group_size: int = 4, workers: int = 4,
-        log=None) -> BatchResult:
-    """Solves both polarisations of every model at each of its frequencies."""
+        log=None, precision: str = "fp64") -> BatchResult:
+    """Solves both polarisations of every model at each of its frequencies; precision="mixed"
+    solves each group by FP64 refinement around complex64 corrections (ModelGroupSolve.solve_mixed),
+    to the same FP64 true residual."""
     settings = settings or X.ForwardSettings()
     groups = make_groups(models, group_size)
     res = BatchResult(groups=len(groups))
@@ -76,10 +78,10 @@ def run(models, settings: X.ForwardSettings | None = None, group_size: int = 4, 
             f, idx = groups[g]
             try:
                 t0 = time.perf_counter()
-                grp = F.ModelGroupSolve([models[i] for i in idx], f, settings)
+                grp = F.ModelGroupSolve([models[i] for i in idx], f, settings, precision=precision)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
-                infos = grp.solve()
+                infos = grp.solve_mixed() if precision == "mixed" else grp.solve()
                 t2 = time.perf_counter()
                 with lock:
                     res.green_seconds += t1 - t0
@@ -87,9 +89,14 @@ def run(models, settings: X.ForwardSettings | None = None, group_size: int = 4, 
                     res.solves += len(infos)
                     for r, i in enumerate(idx):
                         a, b = infos[2 * r], infos[2 * r + 1]
-                        res.iterations[(i, f)] = [a.number_of_iterations, b.number_of_iterations]
-                        res.converged = res.converged and a.convergence_achieved and b.convergence_achieved
-                        res.max_true_residual = max(res.max_true_residual, a.backward_error, b.backward_error)
+                        if precision == "mixed":
+                            res.iterations[(i, f)] = [a.inner_iterations, b.inner_iterations]
+                            res.converged = res.converged and a.converged and b.converged
+                            res.max_true_residual = max(res.max_true_residual, a.residual, b.residual)
+                        else:
+                            res.iterations[(i, f)] = [a.number_of_iterations, b.number_of_iterations]
+                            res.converged = res.converged and a.convergence_achieved and b.convergence_achieved
+                            res.max_true_residual = max(res.max_true_residual, a.backward_error, b.backward_error)
                 del grp
                 if log:
                     log(f"group {g}: {len(idx)} models at {f:g} Hz ({models[idx[0]].nx}x{models[idx[0]].ny}x"

@@ -1511,15 +1511,20 @@ int xtr_fgmres_solve_models_device(xtr_op *op, const xtr_gmres_params *params, i
     });
 }
 
-int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const void *const *b,
-                                   void *const *x, xtr_refine_info *info)
+// The refined solve of nrhs right-hand sides; model[r] (when model is not null) names right-hand side
+// r's model among those of xtr_op_prepare_models, whose R both the FP64 residual applies and the
+// complex64 correction solves use.
+static void refined_solve(xtr_op *op, const xtr_refine_params *params, int nrhs, const int *model,
+                          const void *const *b, void *const *x, xtr_refine_info *info)
 {
-    return guarded([&] {
+    {
         require(op && params && b && x && info, "null argument");
         require(op->precision == XTR_MIXED, "the refined solve needs an operator created with XTR_MIXED");
         require(op->type == XTR_OP_A, "prepare the operator as XTR_OP_A before solving");
         require(nrhs >= 1 && nrhs <= op->max_rhs, "nrhs must be within 1..max_rhs");
         require(params->tolerance > 0 && params->max_refinements >= 1, "tolerance > 0 and max_refinements >= 1");
+        if (model)
+            for (int r = 0; r < nrhs; ++r) (void)op->model_r(model[r]);
         const int64_t n = op->g.n();
         const auto t0 = std::chrono::steady_clock::now();
         XTR_CHECK(hipDeviceSynchronize());   // inputs written on other streams
@@ -1567,6 +1572,7 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
             for (size_t i = 0; i < live.size(); ++i) {
                 v.x[i] = (const double2 *)x[live[i]];
                 v.y[i] = r[live[i]]->p;
+                if (model) v.R[i] = op->model_r(model[live[i]]);
             }
             op->apply_raw(v, (int)live.size(), op->stream);
             std::vector<int> next;
@@ -1605,8 +1611,11 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
             ip.tolerance = tol_in;
             XTR_CHECK(hipStreamSynchronize(op->stream));
             std::vector<xtr_result_info> ii(live.size());
-            if (vec32) solve_batch<float2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
-            else solve_batch<double2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr);
+            std::vector<int> ml;
+            for (int k : live) ml.push_back(model ? model[k] : 0);
+            const int *mp = model ? ml.data() : nullptr;
+            if (vec32) solve_batch<float2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr, mp);
+            else solve_batch<double2>(op, &ip, (int)live.size(), rb.data(), dx.data(), ii.data(), nullptr, nullptr, mp);
             for (size_t i = 0; i < live.size(); ++i) {
                 const int k = live[i];
                 info[k].refinements++;
@@ -1625,6 +1634,21 @@ int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, 
         XTR_CHECK(hipStreamSynchronize(op->stream));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (int k = 0; k < nrhs; ++k) info[k].seconds = secs;
+    }
+}
+
+int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const void *const *b,
+                                   void *const *x, xtr_refine_info *info)
+{
+    return guarded([&] { refined_solve(op, params, nrhs, nullptr, b, x, info); });
+}
+
+int xtr_refined_solve_models_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const int *model,
+                                    const void *const *b, void *const *x, xtr_refine_info *info)
+{
+    return guarded([&] {
+        require(model, "null argument");
+        refined_solve(op, params, nrhs, model, b, x, info);
     });
 }
 

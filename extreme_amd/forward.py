@@ -256,9 +256,11 @@ class ModelGroupSolve:
     tensor, chi0 of every (model, polarisation) from one Chi0 apply, and one batched FGMRES whose
     every apply streams the Green tensor once for all right-hand sides (the reference solves each
     model and polarisation in turn, Mt3DForwardSolver.cs:78-129, reusing the tensor through
-    TensorCache, Forward/TensorCache.cs:8-35)."""
+    TensorCache, Forward/TensorCache.cs:8-35). precision="mixed" keeps a complex64 copy of the
+    tensor for solve_mixed."""
 
-    def __init__(self, models, frequency: float, settings: X.ForwardSettings | None = None):
+    def __init__(self, models, frequency: float, settings: X.ForwardSettings | None = None,
+                 precision: str = "fp64"):
         import torch
         self.torch = torch
         base = models[0]
@@ -274,7 +276,7 @@ class ModelGroupSolve:
         self.omega = cm.omega_of(frequency)
         self.settings = settings or X.ForwardSettings()
         K = len(models)
-        self.op = X.ConvolutionOperator(base, self.omega, self.settings, max_rhs=2 * K)
+        self.op = X.ConvolutionOperator(base, self.omega, self.settings, precision=precision, max_rhs=2 * K)
         self.op.calculate_green_tensor()
         e0 = normal_field(base, self.omega)
         self.rhs_model = [k for k in range(K) for _ in (0, 1)]
@@ -292,3 +294,13 @@ class ModelGroupSolve:
             c.copy_(c0)
         self.solver.history.clear()
         return self.solver.solve_models_batch(self.chi0, self.chi, self.rhs_model)
+
+    def solve_mixed(self, inner_floor=1e-6, max_refinements=10, vectors="fp32"):
+        """solve() in mixed precision (needs precision="mixed"): FP64 refinement around complex64
+        FGMRES corrections of all right-hand sides side by side, each with its own model's R, until
+        every FP64 true residual reaches the tolerance."""
+        for c, c0 in zip(self.chi, self.chi0):
+            c.copy_(c0)
+        return self.solver.solve_refined_batch(self.chi0, self.chi, inner_floor=inner_floor,
+                                               max_refinements=max_refinements, vectors=vectors,
+                                               models=self.rhs_model)

@@ -93,6 +93,9 @@ def lib():
         L.xtr_refined_solve_batch_device.argtypes = [C.c_void_p, C.POINTER(_RefineParams), C.c_int,
                                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                                      C.POINTER(_RefineInfo)]
+        L.xtr_refined_solve_models_device.argtypes = [C.c_void_p, C.POINTER(_RefineParams), C.c_int,
+                                                      C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                                      C.POINTER(C.c_void_p), C.POINTER(_RefineInfo)]
         L.xtr_op_prepare_models.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.xtr_op_apply_models_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p),
                                                  C.POINTER(C.c_void_p)]
@@ -569,21 +572,26 @@ class AnomalyCurrentFgmresSolver:
         return [ResultInfo._from(infos[i]) for i in range(n)]
 
     def solve_refined_batch(self, bs, xs, tolerance=None, inner_floor=1e-6, max_refinements=10,
-                            vectors: str = "fp32") -> list:
+                            vectors: str = "fp32", models=None) -> list:
         """Mixed precision (operator created with precision="mixed"): FP64 iterative refinement
         around complex64 FGMRES corrections, stopping on the FP64 true residual. The corrections'
         Krylov vectors are complex64 (vectors="fp32") or complex128 ("fp64"); their dots, norms and
-        Hessenberg matrix are FP64 either way. Device tensors."""
+        Hessenberg matrix are FP64 either way. models[r] (operator prepared with
+        prepare_models("A", ...)) names right-hand side r's model. Device tensors."""
         n = len(bs)
-        if len(xs) != n:
-            raise ValueError("bs and xs must have the same length")
+        if len(xs) != n or (models is not None and len(models) != n):
+            raise ValueError("bs, xs and models must have the same length")
         _device_vectors(list(bs) + list(xs), self.op.n, "b and x")
         infos = (_RefineInfo * n)()
         bp = (C.c_void_p * n)(*[b.data_ptr() for b in bs])
         xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
         tol = float(self.params.tolerance if tolerance is None else tolerance)
         rp = _RefineParams(tol, float(inner_floor), int(max_refinements), self.params, _PRECISION[vectors])
-        _check(lib().xtr_refined_solve_batch_device(self.op._h, C.byref(rp), n, bp, xp, infos))
+        if models is None:
+            _check(lib().xtr_refined_solve_batch_device(self.op._h, C.byref(rp), n, bp, xp, infos))
+        else:
+            mi = (C.c_int * n)(*[int(m) for m in models])
+            _check(lib().xtr_refined_solve_models_device(self.op._h, C.byref(rp), n, mi, bp, xp, infos))
         return [RefineInfo._from(infos[i]) for i in range(n)]
 
 

@@ -205,6 +205,13 @@ typedef struct {
 int xtr_refined_solve_batch_device(xtr_op *op, const xtr_refine_params *params, int nrhs,
                                    const void *const *b, void *const *x, xtr_refine_info *info);
 
+/* xtr_refined_solve_batch_device for right-hand sides of different models sharing the Green tensor
+ * (prepared by xtr_op_prepare_models as XTR_OP_A): right-hand side r solves model model[r]'s system,
+ * its FP64 residuals and complex64 corrections using that model's R. No counterpart in the
+ * reference (complex128 throughout, one model at a time, Mt3DForwardSolver.cs:78-129). */
+int xtr_refined_solve_models_device(xtr_op *op, const xtr_refine_params *params, int nrhs, const int *model,
+                                    const void *const *b, void *const *x, xtr_refine_info *info);
+
 /* Device time of the operator applies since xtr_op_set_profiling(op, 1), summed by stage, in
  * milliseconds: out[0] forward pass along y (with R and f fused), out[1] forward pass along x,
  * out[2] contraction, out[3] inverse pass along x, out[4] inverse pass along y (with b and the
