@@ -70,8 +70,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c5", choices=["c1", "c2", "c5"],
-                    help="c5 (default): the 256x256x64 model, the largest that fits one GPU; c2: 128x128x32")
+    ap.add_argument("--config", default="c5", choices=["c1", "c2", "c4", "c5"],
+                    help="c5 (default): the 256x256x64 model, the largest that fits one GPU; c2: 128x128x32; "
+                         "c4: 64x64x16 (model 0 of the preconditioner-training set)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision solves (complex64 operator inside FP64 refinement)")
     ap.add_argument("--no-observe", action="store_true",
@@ -91,6 +92,9 @@ def main():
     ap.add_argument("--profile-mixed", action="store_true",
                     help="only run one single-right-hand-side mixed-precision solve of the workload's first frequency "
                          "on the main thread (for traces of the complex64 kernels)")
+    ap.add_argument("--profile-batch", type=int, default=0,
+                    help="only run the uneven-model scheduler on this many of its models on one host thread "
+                         "(groups of four, FP64; for kernel traces of the small-nz kernels)")
     ap.add_argument("--no-probe", action="store_true",
                     help="skip the HBM stream probe (read-only, the contraction's read/write mix, 1:1 copy)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -113,7 +117,10 @@ def main():
     import torch
     from extreme_amd import cartesian_model as cm
 
-    model = {"c5": cm.config_c5, "c2": cm.config_c2, "c1": cm.config_c1}[args.config]()
+    if args.profile_batch:
+        profile_batch(args.profile_batch)
+        return
+    model = {"c5": cm.config_c5, "c2": cm.config_c2, "c4": cm.config_c4, "c1": cm.config_c1}[args.config]()
     if args.profile_applies or args.profile_krylov or args.profile_mixed:
         profile_run(model, args)
         return
@@ -418,6 +425,25 @@ def profile_run(model, args):
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.profile_applies
     log(f"profile: {args.profile_applies} applies at {f:g} Hz, {t64 * 1e3:.3f} ms FP64, {t32 * 1e3:.3f} ms complex64")
+
+
+def profile_batch(nmodels):
+    """The uneven-model scheduler on the first nmodels models on one host thread (groups of up to
+    four models sharing grid and frequency, FP64), after a warm-up on two of them."""
+    import torch
+    from extreme_amd import batch as B
+    from extreme_amd import cartesian_model as cm
+    from extreme_amd import xtreme as X
+    models = [cm.config_c3(k) for k in range(nmodels)]
+    settings = X.ForwardSettings()
+    B.run(models[:2], settings, group_size=1, workers=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = B.run(models, settings, group_size=4, workers=1)
+    torch.cuda.synchronize()
+    log(f"profile: {nmodels} uneven models, {r.groups} groups, {r.solves} solves on one thread, "
+        f"{time.perf_counter() - t0:.3f} s (Green tensors and chi0 {r.green_seconds:.3f} s, solves {r.solve_seconds:.3f} s), "
+        f"converged {r.converged}, max true residual {r.max_true_residual:.2e}")
 
 
 def hbm_probe(nz, reps=10):
