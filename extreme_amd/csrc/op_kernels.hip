@@ -19,6 +19,8 @@
 // so the stream does not allocate in L2 and the Infinity Cache over the lines the kernel writes
 // (the folded contraction at 128 x 128 x 32: 0.337 -> 0.300 ms).
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <utility>
 #include <cstdlib>
 #include <cstring>
@@ -1973,6 +1975,109 @@ constexpr int NT = 256;
 constexpr int MAX_NZ = 31;
 static_assert(4 * (MAX_NZ * (MAX_NZ + 1) / 2) + 2 * MAX_NZ * MAX_NZ <= 16 * NT, "a block is at most 16 elements per thread");
 __host__ __device__ inline int ld_of(int nz) { return nz | 1; }
+// row stride of the spectrum columns in LDS: 3nz rounded up to odd, so that the columns of a block's
+// images (3nz complex apart; a multiple of 256 bytes at nz = 16) fall on different LDS banks
+__host__ __device__ inline int ldu_of(int nz) { return (3 * nz) | 1; }
+
+// A thread's share of one folded block's loads: Green elements e = tid + t NT (t < E) and spectrum
+// elements tid + t NT (t < U; a block with more spectrum values loads the rest directly). What does
+// not depend on the block is computed once per kernel: each Green element's LDS destinations (two for
+// an off-diagonal element of a symmetric block), each spectrum element's column (image, right-hand
+// side) and k, and the divisions behind them. The values of the next block are loaded while the
+// current one is contracted.
+template <typename T, int E>
+struct Loads {
+    static constexpr int U = 4;
+    int gd1[E], gd2[E];                  // LDS destinations, -1: none
+    T gre[E], gim[E];
+    int uimg[U], uh[U], uk[U], ucomp[U]; // uimg -1: outside the largest block
+    int us[U];                           // LDS position: column * ldu + k
+    T ure[U], uim[U];
+
+    __device__ void init(int tid, int nz, int nrhs, const uint16_t *__restrict__ packed_rc)
+    {
+        const int ld = ld_of(nz), msz = nz * ld, S = nz * (nz + 1) / 2, A = nz * nz, gblk = 4 * S + 2 * A;
+        const int C = 3 * nz, swz = asym_swizzle_mask(nz);
+        static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
+            const int e = tid + t * NT;
+            gd1[t] = gd2[t] = -1;
+            if (e < 4 * S) {
+                const int bi = e / S, o = e - bi * S;
+                const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
+                gd1[t] = bi * msz + rc * ld + tr;
+                if (rc != tr) gd2[t] = bi * msz + tr * ld + rc;
+            } else if (e < gblk) {
+                const int o = e - 4 * S, bi = 4 + o / A, pos = o % A;
+                const int rc = pos / nz, tr = (pos % nz) ^ (rc & swz);
+                gd1[t] = bi * msz + rc * ld + tr;
+            }
+        });
+        static_for(std::make_integer_sequence<int, U>{}, [&](auto t) {
+            const int e = tid + t * NT;
+            uimg[t] = -1;
+            uh[t] = uk[t] = ucomp[t] = us[t] = 0;
+            if (e < 4 * nrhs * C) {
+                const int j = e / C;
+                uk[t] = e - j * C;
+                us[t] = j * ldu_of(nz) + uk[t];
+                uimg[t] = j / nrhs;
+                uh[t] = j - uimg[t] * nrhs;
+                ucomp[t] = uk[t] < nz ? 0 : uk[t] < 2 * nz ? 1 : 2;
+            }
+        });
+    }
+    // block b's values into registers; spectrum values with the image's mirror signs
+    __device__ void load(const cplx<T> *__restrict__ G, int64_t b, int gblk, const cplx<T> *buf, int64_t M, int C,
+                         const FoldImages &im)
+    {
+        const cplx<T> *gb = G + b * (int64_t)gblk;
+        static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
+            if (gd1[t] >= 0) {
+                const cplx<T> g = gb[threadIdx.x + t * NT];
+                gre[t] = g.x;
+                gim[t] = g.y;
+            }
+        });
+        static_for(std::make_integer_sequence<int, U>{}, [&](auto t) {
+            ure[t] = uim[t] = T(0);
+            if (uimg[t] >= 0 && uimg[t] < im.n) {
+                const cplx<T> v = buf[uh[t] * M + im.l(uimg[t]) * C + uk[t]];
+                const int flip = ((ucomp[t] == 0) & im.mx(uimg[t])) | ((ucomp[t] == 1) & im.my(uimg[t]));
+                const T sg = flip ? T(-1) : T(1);
+                ure[t] = sg * v.x;
+                uim[t] = sg * v.y;
+            }
+        });
+    }
+    // the registers into LDS (gm: six nz x ld matrices, u: [images * nrhs][ldu])
+    __device__ void store(cplx<T> *gm, cplx<T> *u, int ncols) const
+    {
+        static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
+            const cplx<T> g = mk<T>(gre[t], gim[t]);
+            if (gd1[t] >= 0) gm[gd1[t]] = g;
+            if (gd2[t] >= 0) gm[gd2[t]] = g;
+        });
+        static_for(std::make_integer_sequence<int, U>{}, [&](auto t) {
+            if (uimg[t] >= 0 && threadIdx.x + t * NT < ncols) u[us[t]] = mk<T>(ure[t], uim[t]);
+        });
+    }
+};
+
+// Spectrum values of block im beyond the U NT that Loads carries (more than 1024 values: several
+// right-hand sides at larger nz), loaded directly.
+template <typename T>
+__device__ void load_rest(cplx<T> *u, const cplx<T> *buf, int64_t M, int nz, int nrhs, const FoldImages &im, int from)
+{
+    const int C = 3 * nz, ncols = im.n * nrhs * C;
+    for (int e = from + (int)threadIdx.x; e < ncols; e += NT) {
+        const int j = e / C, k = e - j * C;
+        const int img = j / nrhs, h = j - img * nrhs;
+        const cplx<T> v = buf[h * M + im.l(img) * C + k];
+        const int flip = ((k < nz) & im.mx(img)) | ((k >= nz) & (k < 2 * nz) & im.my(img));
+        const T sg = flip ? T(-1) : T(1);
+        u[j * ldu_of(nz) + k] = mk<T>(sg * v.x, sg * v.y);
+    }
+}
 }  // namespace foldn
 
 template <typename T, int J, int E>
@@ -1985,8 +2090,8 @@ __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T>
     const int gblk = 4 * S + 2 * A;                        // the launch keeps gblk <= E * NT
     const int msz = nz * ld;
     cplx<T> *gm = reinterpret_cast<cplx<T> *>(smem_raw);   // [6][nz][ld]
-    cplx<T> *u = gm + 6 * msz;                             // [images * nrhs][C]
-    const int swz = asym_swizzle_mask(nz);
+    cplx<T> *u = gm + 6 * msz;                             // [images * nrhs][ldu]
+    const int ldu = foldn::ldu_of(nz);
     const int tid = threadIdx.x;
     const int ngroups = foldn::NT / C;                     // the launch keeps C <= NT
     const int q = tid % C, cg = tid / C;
@@ -2001,64 +2106,18 @@ __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T>
     trn[0] = a == 2;
     trn[1] = a == 2;
     trn[2] = false;
-    // the block's elements tid + t NT, in registers: all E loads are in flight together, and the next
-    // block's are issued before this block's arithmetic
-    T gre[E], gim[E];                                      // (real arrays of complex ones end up in scratch)
-    auto load_g = [&](int64_t bb) {
-        const cplx<T> *gb = G + bb * (int64_t)gblk;
-        foldn::static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
-            const int e = tid + t * foldn::NT;
-            if (e < gblk) {
-                const cplx<T> g = gb[e];
-                gre[t] = g.x;
-                gim[t] = g.y;
-            }
-        });
-    };
-    if ((int64_t)blockIdx.x < Lq) load_g(blockIdx.x);
+    using Lo = foldn::Loads<T, E>;
+    Lo lo;
+    lo.init(tid, nz, nrhs, packed_rc);
+    if ((int64_t)blockIdx.x < Lq) lo.load(G, blockIdx.x, gblk, buf, M, C, fold_images(blockIdx.x, nx, ny));
     for (int64_t b = blockIdx.x; b < Lq; b += gridDim.x) {
         const FoldImages im = fold_images(b, nx, ny);
         const int ncol = im.n * nrhs;
         __syncthreads();                                   // the previous block's LDS reads are done
-        foldn::static_for(std::make_integer_sequence<int, E>{}, [&](auto t) {
-            const int e = tid + t * foldn::NT;
-            const cplx<T> g = mk<T>(gre[t], gim[t]);
-            if (e < 4 * S) {
-                const int bi = e / S, o = e - bi * S;
-                const int rc = packed_rc[2 * o], tr = packed_rc[2 * o + 1];
-                cplx<T> *m = gm + bi * msz;
-                m[rc * ld + tr] = g;
-                m[tr * ld + rc] = g;
-            } else if (e < gblk) {
-                const int o = e - 4 * S, bi = 4 + o / A, pos = o % A;
-                const int rc = pos / nz, tr = (pos % nz) ^ (rc & swz);
-                gm[bi * msz + rc * ld + tr] = g;
-            }
-        });
-        // column j = image * nrhs + h; four loads in flight per thread and round
-        for (int e0 = tid; e0 < ncol * C; e0 += 4 * foldn::NT) {
-            T vre[4], vim[4], sg[4];
-            foldn::static_for(std::make_integer_sequence<int, 4>{}, [&](auto t) {
-                const int e = e0 + t * foldn::NT;
-                vre[t] = vim[t] = T(0);
-                sg[t] = T(1);
-                if (e < ncol * C) {
-                    const int j = e / C, k = e - j * C;
-                    const int img = j / nrhs, h = j - img * nrhs;
-                    const cplx<T> v = buf[h * M + im.l(img) * C + k];
-                    vre[t] = v.x;
-                    vim[t] = v.y;
-                    const int flip = ((k < nz) & im.mx(img)) | ((k >= nz) & (k < 2 * nz) & im.my(img));
-                    sg[t] = flip ? T(-1) : T(1);
-                }
-            });
-            foldn::static_for(std::make_integer_sequence<int, 4>{}, [&](auto t) {
-                const int e = e0 + t * foldn::NT;
-                if (e < ncol * C) u[e] = mk<T>(sg[t] * vre[t], sg[t] * vim[t]);
-            });
-        }
+        lo.store(gm, u, ncol * C);
+        if (ncol * C > Lo::U * foldn::NT) foldn::load_rest<T>(u, buf, M, nz, nrhs, im, Lo::U * foldn::NT);
         __syncthreads();
-        if (b + gridDim.x < Lq) load_g(b + gridDim.x);
+        if (b + gridDim.x < Lq) lo.load(G, b + gridDim.x, gblk, buf, M, C, fold_images(b + gridDim.x, nx, ny));
         if (cg >= ngroups) continue;
         cplx<T> acc[J];
 #pragma unroll
@@ -2075,7 +2134,7 @@ __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T>
 #pragma unroll
                 for (int jj = 0; jj < J; ++jj) {
                     const int j = cg + jj * ngroups;
-                    if (j < ncol) acc[jj] = cfma<T>(acc[jj], g, us[j * C + c]);
+                    if (j < ncol) acc[jj] = cfma<T>(acc[jj], g, us[j * ldu + c]);
                 }
             }
         }
@@ -2092,6 +2151,136 @@ __global__ __launch_bounds__(foldn::NT) void contract_foldn_kernel(const cplx<T>
     }
 }
 
+// ---- folded contraction for small nz on MFMA ---------------------------------------------------
+// The same work as contract_foldn_kernel (same LDS layout, same loads) with the products on
+// 16x16x4 MFMAs: per (row tile of 16 rows of the 3nz x 3nz block operator, column tile of 8 complex
+// columns) a wave forms D1 = Re A . B and D2 = Im A . B with B = [Re U | Im U] (the 8 columns' spectrum
+// values, held in registers for all k-steps), and Y = D1 - D2' / D1' + D2 by a lane exchange. Each
+// A operand is read from the six expanded matrices through a per-lane address: row (a, i) of the
+// operator and k = (segment, c) pick matrix, orientation and sign (-XZ^T, -YZ^T in the z rows);
+// (segment, c) advance by 4 per k-step. The operands come from registers, so LDS carries about one
+// Green value per lane and MFMA instead of one spectrum value per complex MAC.
+namespace foldm {
+constexpr int NT = foldn::NT;               // four waves; foldn::Loads assumes this count
+constexpr int MIN_NZ = 4;                   // one k-step of 4 advances c across at most one segment
+constexpr int MAX_NZ = 24;
+constexpr int MAX_KS = (3 * MAX_NZ + 3) / 4;
+static_assert(4 * (MAX_NZ * (MAX_NZ + 1) / 2) + 2 * MAX_NZ * MAX_NZ <= 12 * NT, "a block is at most 12 elements per thread");
+}  // namespace foldm
+
+template <typename T, int E>
+__global__ __launch_bounds__(foldm::NT) void contract_foldm_kernel(const cplx<T> *__restrict__ G, cplx<T> *buf,
+                                                                  int64_t Lq, int64_t M, int nz, int nrhs, int nx,
+                                                                  int ny, const uint16_t *__restrict__ packed_rc)
+{
+    using acc_t = typename MfmaAcc<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    const int ld = foldn::ld_of(nz), C = 3 * nz, S = nz * (nz + 1) / 2, A = nz * nz;
+    const int gblk = 4 * S + 2 * A;                        // the launch keeps gblk <= E * NT
+    const int msz = nz * ld;
+    const int KS = (C + 3) / 4, RT = (C + 15) / 16;        // k-steps, row tiles
+    cplx<T> *gm = reinterpret_cast<cplx<T> *>(smem_raw);   // [6][nz][ld]: xx, xy, yy, zz, xz, yz
+    cplx<T> *u = gm + 6 * msz;                             // [images * nrhs][ldu]
+    const int ldu = foldn::ldu_of(nz);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n = lane & 15, kk = lane >> 4, part = n >> 3;
+    using Lo = foldn::Loads<T, E>;
+    Lo lo;
+    lo.init(tid, nz, nrhs, packed_rc);
+    if ((int64_t)blockIdx.x < Lq) lo.load(G, blockIdx.x, gblk, buf, M, C, fold_images(blockIdx.x, nx, ny));
+    for (int64_t b = blockIdx.x; b < Lq; b += gridDim.x) {
+        const FoldImages im = fold_images(b, nx, ny);
+        const int ncol = im.n * nrhs;
+        __syncthreads();                                   // the previous block's LDS reads are done
+        lo.store(gm, u, ncol * C);
+        if (ncol * C > Lo::U * foldn::NT) foldn::load_rest<T>(u, buf, M, nz, nrhs, im, Lo::U * foldn::NT);
+        __syncthreads();
+        if (b + gridDim.x < Lq) lo.load(G, b + gridDim.x, gblk, buf, M, C, fold_images(b + gridDim.x, nx, ny));
+        const int nct = (ncol + 7) / 8;
+        for (int it = wave; it < RT * nct; it += 4) {      // (row tile, column tile) items of this wave
+            const int rt = it % RT, ct = it / RT;
+            const int col = 8 * ct + (n & 7);
+            const bool cval = col < ncol;
+            // B[k][n] = Re (n < 8) or Im (n >= 8) of column col's value at k = 4 ks + kk
+            T bfr[foldm::MAX_KS];
+#pragma unroll
+            for (int ks = 0; ks < foldm::MAX_KS; ++ks) {
+                const int k = 4 * ks + kk;
+                T v = T(0);
+                if (ks < KS && cval && k < C) {
+                    const cplx<T> w = u[col * ldu + k];
+                    v = part ? w.y : w.x;
+                }
+                bfr[ks] = v;
+            }
+            // lane's A row: operator row 16 rt + n = (a, i); per segment the matrix base, the step along
+            // c and the sign (row-major XX XY XZ / XY YY YZ / ZZ; the z rows read -XZ, -YZ by columns)
+            const int row = 16 * rt + n;
+            const bool rval = row < C;
+            const int a = rval ? row / nz : 0, i = rval ? row - a * nz : 0;
+            const int m0 = a == 0 ? 0 : a == 1 ? 1 : 4;
+            const int m1 = a == 0 ? 1 : a == 1 ? 2 : 5;
+            const int m2 = a == 0 ? 4 : a == 1 ? 5 : 3;
+            const int zr = a == 2;                          // the z rows: segments 0, 1 transposed and negated
+            const int base0 = m0 * msz + (zr ? i : i * ld), step0 = zr ? ld : 1;
+            const int base1 = m1 * msz + (zr ? i : i * ld), step1 = zr ? ld : 1;
+            const int base2 = m2 * msz + i * ld;
+            const T s01 = zr ? T(-1) : T(1);
+            const T rv = rval ? T(1) : T(0);
+            int seg = 0, c = kk;                            // k = 4 ks + kk = seg nz + c (nz >= 4)
+            if (c >= nz) { c -= nz; seg = 1; }
+            acc_t d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < foldm::MAX_KS; ++ks) {
+                if (ks < KS) {
+                    const int base = seg == 0 ? base0 : seg == 1 ? base1 : base2;
+                    const int step = seg == 0 ? step0 : seg == 1 ? step1 : 1;
+                    const T f = seg < 2 ? s01 * rv : (seg == 2 ? rv : T(0));
+                    const cplx<T> g = gm[seg < 3 ? base + c * step : 0];
+                    d1 = mfma16(f * g.x, bfr[ks], d1);
+                    d2 = mfma16(f * g.y, bfr[ks], d2);
+                    c += 4;
+                    if (c >= nz) { c -= nz; ++seg; }
+                    if (c >= nz) { c -= nz; ++seg; }
+                }
+            }
+            const acc_t y = fold64_combine<T>(d1, d2, lane);
+            if (cval) {
+                const int img = col / nrhs, h = col - img * nrhs;
+                const int fx = im.mx(img), fy = im.my(img);
+                T *dst = reinterpret_cast<T *>(buf + h * M + im.l(img) * C) + part;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rr = 16 * rt + mfma_row<T>(lane, r);
+                    if (rr < C) {
+                        const int ar = rr / nz;
+                        const int flip = ((ar == 0) & fx) | ((ar == 1) & fy);
+                        dst[2 * rr] = flip ? -y[r] : y[r];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Workgroups of `kernel` (NT threads, lds bytes of dynamic LDS) resident per CU, from the compiled
+// kernel's registers and LDS; cached per kernel and LDS size. A persistent grid larger than
+// CUs x this would run in rounds, each workgroup paying its prologue and first loads again.
+static int resident_per_cu(const void *kernel, int nt, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({kernel, lds});
+    if (it != cache.end()) return it->second;
+    int nb = 0;
+    XTR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, nt, lds));
+    nb = std::max(1, nb);
+    cache[{kernel, lds}] = nb;
+    return nb;
+}
+
 static int device_max_lds()
 {
     static int lds[64] = {0};
@@ -2105,6 +2294,8 @@ static int device_max_lds()
 template <typename T, int J, int E>
 static void launch_foldn(const ApplyArgs &a, unsigned grid, size_t lds, hipStream_t s)
 {
+    const void *k = (const void *)contract_foldn_kernel<T, J, E>;
+    grid = (unsigned)std::min<int64_t>(grid, (int64_t)resident_per_cu(k, foldn::NT, lds) * device_cus());
     hipLaunchKernelGGL((contract_foldn_kernel<T, J, E>), dim3(grid), dim3(foldn::NT), lds, s, (const cplx<T> *)a.G,
                        (cplx<T> *)a.buf, a.g.Lq(), a.g.M(), a.g.nz, a.nrhs, a.g.nx, a.g.ny, a.packed_rc);
 }
@@ -2127,7 +2318,7 @@ static bool contract_foldn(const ApplyArgs &a, hipStream_t s)
     if (!a.folded || nz < 1 || nz > foldn::MAX_NZ || a.nrhs < 1 || a.nrhs > kMaxRhs || contract_force_generic())
         return false;
     const int C = 3 * nz;
-    const size_t lds = sizeof(cplx<T>) * ((size_t)6 * nz * foldn::ld_of(nz) + (size_t)4 * a.nrhs * C);
+    const size_t lds = sizeof(cplx<T>) * ((size_t)6 * nz * foldn::ld_of(nz) + (size_t)4 * a.nrhs * foldn::ldu_of(nz));
     if (lds > (size_t)device_max_lds()) return false;
     const int ngroups = foldn::NT / C;
     const int need = (4 * a.nrhs + ngroups - 1) / ngroups;    // columns per thread
@@ -2141,6 +2332,46 @@ static bool contract_foldn(const ApplyArgs &a, hipStream_t s)
     return true;
 }
 
+// XTR_CONTRACT_KERNEL=vector keeps small-nz folded tensors on contract_foldn_kernel, =mfma puts every
+// depth contract_foldm_kernel supports on it (to compare the two).
+static bool contract_kernel_is(const char *name)
+{
+    const char *e = std::getenv("XTR_CONTRACT_KERNEL");
+    return e && std::strcmp(e, name) == 0;
+}
+
+template <typename T, int E>
+static void launch_foldm(const ApplyArgs &a, unsigned grid, size_t lds, hipStream_t s)
+{
+    const void *k = (const void *)contract_foldm_kernel<T, E>;
+    grid = (unsigned)std::min<int64_t>(grid, (int64_t)resident_per_cu(k, foldm::NT, lds) * device_cus());
+    hipLaunchKernelGGL((contract_foldm_kernel<T, E>), dim3(grid), dim3(foldm::NT), lds, s, (const cplx<T> *)a.G,
+                       (cplx<T> *)a.buf, a.g.Lq(), a.g.M(), a.g.nz, a.nrhs, a.g.nx, a.g.ny, a.packed_rc);
+}
+
+// Folded tensor, 1-8 right-hand sides in one launch, on MFMA: taken from nz = 17 to 24. Measured
+// against contract_foldn_kernel (two right-hand sides, FP64): 163 against 186 us per launch at
+// nz = 24, but 63 against 48 us at nz = 16 and 38 against 30 us at nz = 8 and 12.
+template <typename T>
+static bool contract_foldm(const ApplyArgs &a, hipStream_t s)
+{
+    const int nz = a.g.nz;
+    const int lowest = contract_kernel_is("mfma") ? foldm::MIN_NZ : 17;
+    if (!a.folded || nz < lowest || nz > foldm::MAX_NZ || a.nrhs < 1 || a.nrhs > kMaxRhs ||
+        contract_force_generic() || contract_kernel_is("vector"))
+        return false;
+    const size_t lds = sizeof(cplx<T>) * ((size_t)6 * nz * foldn::ld_of(nz) + (size_t)4 * a.nrhs * foldn::ldu_of(nz));
+    if (lds > (size_t)device_max_lds()) return false;
+    const int per_cu = std::max(1, std::min(8, (int)(device_max_lds() / lds)));
+    const unsigned grid = (unsigned)std::min<int64_t>(a.g.Lq(), (int64_t)per_cu * device_cus());
+    const int64_t per_thread = ceil_div(a.g.gblk(), foldm::NT);
+    if (per_thread <= 4) launch_foldm<T, 4>(a, grid, lds, s);
+    else if (per_thread <= 8) launch_foldm<T, 8>(a, grid, lds, s);
+    else launch_foldm<T, 12>(a, grid, lds, s);
+    XTR_LAUNCH_CHECK("contract_foldm_kernel");
+    return true;
+}
+
 template <typename T>
 static void contract(const ApplyArgs &a, hipStream_t s)
 {
@@ -2148,6 +2379,7 @@ static void contract(const ApplyArgs &a, hipStream_t s)
     if (!a.folded && contract_stream64(a, s)) return;
     if (contract_fold32(a, s)) return;
     if (contract_fold64(a, s)) return;
+    if (contract_foldm<T>(a, s)) return;
     if (contract_foldn<T>(a, s)) return;
     const int nz = a.g.nz, C = 3 * nz;
     const int per_w = a.nrhs * C;

@@ -56,10 +56,11 @@ def test_folded_apply_matches_oracle_and_full(oracle, torch_cuda, monkeypatch, s
 
 
 @pytest.mark.parametrize("shape,nrhs", [((6, 4, 8), 1), ((5, 6, 12), 2), ((4, 6, 16), 4), ((6, 4, 24), 8),
-                                        ((3, 5, 31), 2), ((5, 3, 1), 3)])
+                                        ((5, 6, 5), 2), ((3, 5, 31), 2), ((5, 3, 1), 3)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_folded_small_nz_kernel_matches_generic(oracle, torch_cuda, monkeypatch, shape, nrhs, precision):
-    """The folded contraction for nz <= 31 (one Green-block read for all images and right-hand sides)
+    """The folded contractions for nz <= 31 (one Green-block read for all images and right-hand sides:
+    vector FMAs, or MFMA from nz = 17 to 24 by default and from nz = 4 with XTR_CONTRACT_KERNEL=mfma)
     against the generic per-wavenumber kernel and the oracle, 1-8 right-hand sides, complex128 and
     complex64."""
     torch = torch_cuda
@@ -76,22 +77,20 @@ def test_folded_small_nz_kernel_matches_generic(oracle, torch_cuda, monkeypatch,
         op.prepare_operator(kind, z)
         oop.prepare(kind, z, z0)
         ref = oop.apply(xs[-1])
-        if precision == "fp64":
-            y = op.apply(xs)
-            monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
-            yg = op.apply(xs)
-            monkeypatch.delenv("XTR_CONTRACT_KERNEL")
-            tol_g, tol_o = 1e-13, 1e-12
-        else:
-            y = op.apply_c64(x32).cpu().numpy()
-            monkeypatch.setenv("XTR_CONTRACT_KERNEL", "generic")
-            yg = op.apply_c64(x32).cpu().numpy()
-            monkeypatch.delenv("XTR_CONTRACT_KERNEL")
-            tol_g, tol_o = 2e-6, 2e-6
-        y, yg = y.reshape(nrhs, m.n), yg.reshape(nrhs, m.n)
+        def run(kernel):
+            if kernel:
+                monkeypatch.setenv("XTR_CONTRACT_KERNEL", kernel)
+            out = op.apply(xs) if precision == "fp64" else op.apply_c64(x32).cpu().numpy()
+            if kernel:
+                monkeypatch.delenv("XTR_CONTRACT_KERNEL")
+            return out.reshape(nrhs, m.n)
+        y, yv, ym, yg = run(None), run("vector"), run("mfma"), run("generic")
+        tol_g, tol_o = (1e-13, 1e-12) if precision == "fp64" else (2e-6, 2e-6)
         print(f"small-nz folded {shape} nrhs {nrhs} {precision} {kind}: vs generic {rel(y, yg):.2e}, "
-              f"vs oracle {rel(y[-1], ref):.2e}", flush=True)
+              f"vector {rel(yv, yg):.2e}, mfma {rel(ym, yg):.2e}, vs oracle {rel(y[-1], ref):.2e}", flush=True)
         assert rel(y, yg) < tol_g
+        assert rel(yv, yg) < tol_g
+        assert rel(ym, yg) < tol_g
         assert rel(y[-1], ref) < tol_o
 
 
