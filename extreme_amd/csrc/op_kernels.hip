@@ -2349,16 +2349,17 @@ static void launch_foldm(const ApplyArgs &a, unsigned grid, size_t lds, hipStrea
                        (cplx<T> *)a.buf, a.g.Lq(), a.g.M(), a.g.nz, a.nrhs, a.g.nx, a.g.ny, a.packed_rc);
 }
 
-// Folded tensor, 1-8 right-hand sides in one launch, on MFMA: taken from nz = 17 to 24. Measured
-// against contract_foldn_kernel (two right-hand sides, FP64): 163 against 186 us per launch at
-// nz = 24, but 63 against 48 us at nz = 16 and 38 against 30 us at nz = 8 and 12.
+// Folded tensor, 1-8 right-hand sides in one launch, on MFMA: by the fixed rule from nz = 17 to 24.
+// Measured against contract_foldn_kernel (two right-hand sides, FP64): 163 against 186 us per launch
+// at nz = 24, but 63 against 48 us at nz = 16 and 38 against 30 us at nz = 8 and 12. The operator
+// handle measures the two once per process and shape and sets a.contract_variant.
 template <typename T>
 static bool contract_foldm(const ApplyArgs &a, hipStream_t s)
 {
     const int nz = a.g.nz;
-    const int lowest = contract_kernel_is("mfma") ? foldm::MIN_NZ : 17;
+    const int lowest = contract_kernel_is("mfma") || a.contract_variant == CONTRACT_MFMA ? foldm::MIN_NZ : 17;
     if (!a.folded || nz < lowest || nz > foldm::MAX_NZ || a.nrhs < 1 || a.nrhs > kMaxRhs ||
-        contract_force_generic() || contract_kernel_is("vector"))
+        contract_force_generic() || contract_kernel_is("vector") || a.contract_variant == CONTRACT_VECTOR)
         return false;
     const size_t lds = sizeof(cplx<T>) * ((size_t)6 * nz * foldn::ld_of(nz) + (size_t)4 * a.nrhs * foldn::ldu_of(nz));
     if (lds > (size_t)device_max_lds()) return false;
@@ -2370,6 +2371,12 @@ static bool contract_foldm(const ApplyArgs &a, hipStream_t s)
     else launch_foldm<T, 12>(a, grid, lds, s);
     XTR_LAUNCH_CHECK("contract_foldm_kernel");
     return true;
+}
+
+bool contract_variants_apply(const ApplyArgs &a)
+{
+    return a.folded && a.g.nz >= foldm::MIN_NZ && a.g.nz <= foldm::MAX_NZ && a.nrhs >= 1 && a.nrhs <= kMaxRhs &&
+           !std::getenv("XTR_CONTRACT_KERNEL");
 }
 
 template <typename T>

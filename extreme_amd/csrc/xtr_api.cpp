@@ -124,6 +124,21 @@ std::mutex &tile_cache_mu()
     return m;
 }
 
+// Small-nz contraction kernel measured per process: (precision, nx, ny, nz, right-hand sides) ->
+// CONTRACT_VECTOR or CONTRACT_MFMA (guarded by tile_cache_mu as well).
+struct ContractKey {
+    int p, nx, ny, nz, nrhs;
+    bool operator<(const ContractKey &o) const
+    {
+        return std::tie(p, nx, ny, nz, nrhs) < std::tie(o.p, o.nx, o.ny, o.nz, o.nrhs);
+    }
+};
+std::map<ContractKey, int> &contract_cache()
+{
+    static std::map<ContractKey, int> m;
+    return m;
+}
+
 // Shortest decimal that reads back as the same double (what .NET's default double formatting
 // prints in the reference's messages, up to the exponent's spelling).
 std::string shortest(double v)
@@ -304,6 +319,58 @@ struct xtr_op {
         return best;
     }
 
+    // Contraction kernel of a folded small-nz apply (contract_foldn_kernel or contract_foldm_kernel),
+    // measured once per process and shape the way fft_tiles measures tiles: whole applies on this
+    // apply's own buffers, one warm-up and three timed per kernel, the contraction stage timed.
+    int contract_variant(xtr::ApplyArgs &a, hipStream_t s)
+    {
+        if (!xtr::contract_variants_apply(a)) return xtr::CONTRACT_RULE;
+        const ContractKey key{a.precision, g.nx, g.ny, g.nz, a.nrhs};
+        {
+            std::lock_guard<std::mutex> lk(tile_cache_mu());
+            auto it = contract_cache().find(key);
+            if (it != contract_cache().end()) return it->second;
+        }
+        using Pass = void (*)(const xtr::ApplyArgs &, hipStream_t);
+        const Pass stage[5] = {xtr::launch_fwd_y, xtr::launch_fwd_x, xtr::launch_contract, xtr::launch_inv_x,
+                               xtr::launch_inv_y};
+        hipEvent_t e0, e1;
+        XTR_CHECK(hipEventCreate(&e0));
+        XTR_CHECK(hipEventCreate(&e1));
+        int best = xtr::CONTRACT_RULE;
+        try {
+            float best_ms = 0;
+            for (int variant : {xtr::CONTRACT_VECTOR, xtr::CONTRACT_MFMA}) {
+                a.contract_variant = variant;
+                float total = 0;
+                for (int r = 0; r < 4; ++r) {
+                    for (int st = 0; st < 5; ++st) {
+                        if (st == 2) XTR_CHECK(hipEventRecord(e0, s));
+                        stage[st](a, s);
+                        if (st == 2) XTR_CHECK(hipEventRecord(e1, s));
+                    }
+                    XTR_CHECK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    XTR_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (r > 0) total += ms;
+                }
+                if (best == xtr::CONTRACT_RULE || total < best_ms) {
+                    best_ms = total;
+                    best = variant;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(tile_cache_mu());
+        contract_cache()[key] = best;
+        return best;
+    }
+
     // One operator apply for nrhs vectors on stream s (ConvolutionOperator.CalculateOperatorAorKr);
     // vec32: the vectors are complex64 (the FP32 apply of a mixed-precision correction solve).
     void apply_raw(const xtr::RhsPtrs &v, int nrhs, hipStream_t s, bool vec32 = false)
@@ -340,6 +407,7 @@ struct xtr_op {
         a.v = v;
         last_tiles = fft_tiles(a, s);
         for (int k = 0; k < 4; ++k) a.fft_tile[k] = last_tiles[k];
+        a.contract_variant = contract_variant(a, s);
         hipEvent_t *ev = profiling ? next_events(nrhs) : nullptr;
         if (ev) XTR_CHECK(hipEventRecord(ev[0], s));
         xtr::launch_fwd_y(a, s);

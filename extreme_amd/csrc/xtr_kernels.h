@@ -106,8 +106,14 @@ struct ApplyArgs {
     const void *tw_y;         // twiddles for P = 2 ny
     int fft_tile[4];          // column tile of fwd_y, fwd_x, inv_x, inv_y (FFT_TILE_*)
     int vec32;                // v.x and v.y point to complex64 vectors (float2; precision FP32 only)
+    int contract_variant;     // folded nz <= 31: CONTRACT_* (CONTRACT_RULE: MFMA from nz = 17)
     RhsPtrs v;
 };
+
+enum { CONTRACT_RULE = 0, CONTRACT_VECTOR = 1, CONTRACT_MFMA = 2 };
+// true when the apply can take either small-nz contraction (folded, 4 <= nz <= 24) and no
+// XTR_CONTRACT_KERNEL override fixes the kernel
+bool contract_variants_apply(const ApplyArgs &a);
 
 enum {
     FFT_TILE_DEFAULT = 0,
