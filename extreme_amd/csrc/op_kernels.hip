@@ -1950,7 +1950,8 @@ static bool contract_fold32(const ApplyArgs &a, hipStream_t s)
 
 // ---- folded contraction for small nz ---------------------------------------------------------------
 // ApplyGreenTensorAlongZ (ConvolutionOperator.cs:274-327) over a folded tensor for the depths that
-// have no kernel of their own (nz <= 31). A workgroup takes folded blocks b = blockIdx.x,
+// have no kernel of their own (nz <= 31; from nz = 4 to 24 the operator handle times it against
+// contract_foldm_kernel once per shape and takes the faster). A workgroup takes folded blocks b = blockIdx.x,
 // += gridDim.x; each block is read from HBM once, expanded into six full nz x ld matrices in LDS
 // (xx, xy, yy, zz, xz, yz; ld = nz rounded up to odd, so that rows and columns are both read without
 // bank conflicts), and applied to the spectrum columns of all its images (<= 4) and right-hand sides
